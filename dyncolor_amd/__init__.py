@@ -1,0 +1,310 @@
+"""dyncolor_amd -- MI355X (gfx950) dynamic graph recoloring.
+
+Python binding (ctypes) of the C ABI in include/dyncolor.h, used by the tests
+and the benchmark.  All compute runs in libdyncolor.so's HIP kernels; if the
+library is missing this module raises instead of falling back to anything.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libdyncolor.so")
+
+RULE_INDEPENDENT = 0
+RULE_LDF = 1
+DRIVER_PLAIN = 0
+DRIVER_FUSED = 1
+
+OK = 0
+ERR_INVALID = -1
+ERR_VERTEX = -2
+ERR_HIP = -3
+ERR_NOMEM = -4
+ERR_ROUNDS = -5
+ERR_STATE = -6
+ERR_ADMIT = -7
+
+# every symbol include/dyncolor.h declares
+EXPORTS = [
+    "dc_options_default", "dc_graph_create", "dc_partition_create", "dc_graph_destroy",
+    "dc_graph_info", "dc_colors_set", "dc_colors_get", "dc_static_color", "dc_apply_batch",
+    "dc_batch_create", "dc_batch_destroy", "dc_apply_device_batch", "dc_recolor",
+    "dc_process_batch", "dc_validate", "dc_trace_get", "dc_neighbors", "dc_prop_get",
+    "dc_compact", "dc_set_profiling", "dc_synchronize", "dc_last_error", "dc_device_count",
+    "dc_gen_er", "dc_gen_rmat", "dc_gen_batches",
+]
+
+
+class Options(C.Structure):
+    _fields_ = [("slack_frac", C.c_float), ("slack_min", C.c_int32), ("max_rounds", C.c_int32),
+                ("driver", C.c_int32), ("device", C.c_int32), ("reserved", C.c_int32 * 3)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("updates", C.c_int64), ("insertions", C.c_int64), ("deletions", C.c_int64),
+                ("phases", C.c_int32), ("loop_rounds", C.c_int32), ("evaluated", C.c_int64),
+                ("changed", C.c_int64), ("frontier_sum", C.c_int64),
+                ("slots_scanned", C.c_int64), ("live_scanned", C.c_int64),
+                ("recolor_bytes", C.c_int64), ("ms_apply", C.c_double),
+                ("ms_recolor", C.c_double), ("ms_recolor_kernel", C.c_double),
+                ("recolor_launches", C.c_int32), ("regrows", C.c_int32),
+                ("reserved", C.c_int32 * 6)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
+class DynColorError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"dyncolor error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+_i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_i64p = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+_u8p = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
+_vp = C.c_void_p
+
+
+def lib():
+    """Load libdyncolor.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: run python -m dyncolor_amd.build")
+    L = C.CDLL(LIB_PATH)
+    sig = {
+        "dc_options_default": (None, [C.POINTER(Options)]),
+        "dc_graph_create": (C.c_int, [C.c_int32, C.c_int64, _vp, _vp, C.POINTER(Options), C.POINTER(_vp)]),
+        "dc_partition_create": (C.c_int, [C.c_int32, C.c_int32, C.c_int64, _vp, _vp, _vp,
+                                          C.POINTER(Options), C.POINTER(_vp)]),
+        "dc_graph_destroy": (C.c_int, [_vp]),
+        "dc_graph_info": (C.c_int, [_vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                    C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+        "dc_colors_set": (C.c_int, [_vp, _i32p]),
+        "dc_colors_get": (C.c_int, [_vp, _i32p]),
+        "dc_static_color": (C.c_int, [_vp, C.c_int32]),
+        "dc_apply_batch": (C.c_int, [_vp, C.c_int64, _vp, _vp, _vp]),
+        "dc_batch_create": (C.c_int, [_vp, C.c_int64, _vp, _vp, _vp, C.POINTER(_vp)]),
+        "dc_batch_destroy": (C.c_int, [_vp]),
+        "dc_apply_device_batch": (C.c_int, [_vp, _vp]),
+        "dc_recolor": (C.c_int, [_vp, C.c_int32, C.POINTER(Stats)]),
+        "dc_process_batch": (C.c_int, [_vp, C.c_int32, C.c_int64, _vp, _vp, _vp, C.POINTER(Stats)]),
+        "dc_validate": (C.c_int, [_vp, C.POINTER(C.c_int64), C.POINTER(C.c_int32)]),
+        "dc_trace_get": (C.c_int, [_vp, _i64p, C.c_int32, C.POINTER(C.c_int32)]),
+        "dc_neighbors": (C.c_int, [_vp, C.c_int32, _i32p, C.c_int64, C.POINTER(C.c_int64)]),
+        "dc_prop_get": (C.c_int, [_vp, _i32p]),
+        "dc_compact": (C.c_int, [_vp]),
+        "dc_set_profiling": (C.c_int, [_vp, C.c_int32]),
+        "dc_synchronize": (C.c_int, [_vp]),
+        "dc_last_error": (C.c_char_p, []),
+        "dc_device_count": (C.c_int, [C.POINTER(C.c_int32)]),
+        "dc_gen_er": (C.c_int, [C.c_int32, C.c_int64, C.c_uint64, _i32p, _i32p]),
+        "dc_gen_rmat": (C.c_int, [C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_double,
+                                  C.c_uint64, _i32p, _i32p, C.POINTER(C.c_int64)]),
+        "dc_gen_batches": (C.c_int, [C.c_int32, C.c_int64, _vp, _vp, C.c_int32, C.c_int64,
+                                     C.c_double, C.c_uint64, _i32p, _i32p, _u8p]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def _check(rc):
+    if rc != OK:
+        raise DynColorError(rc, lib().dc_last_error().decode(errors="replace"))
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None and a.size else None
+
+
+def _arr(x, dt):
+    return np.ascontiguousarray(np.asarray(x, dtype=dt))
+
+
+def device_count() -> int:
+    c = C.c_int32(0)
+    rc = lib().dc_device_count(C.byref(c))
+    return int(c.value) if rc == OK else 0
+
+
+def options(**kw) -> Options:
+    o = Options()
+    lib().dc_options_default(C.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+# ---------------------------------------------------------------- generators
+def gen_er(n, m, seed):
+    u = np.zeros(max(m, 1), np.int32)
+    v = np.zeros(max(m, 1), np.int32)
+    _check(lib().dc_gen_er(n, m, seed, u, v))
+    return u[:m], v[:m]
+
+
+def gen_rmat(scale, edge_factor=16, a=0.57, b=0.19, c=0.19, seed=1):
+    samples = edge_factor << scale
+    u = np.zeros(samples, np.int32)
+    v = np.zeros(samples, np.int32)
+    m = C.c_int64(0)
+    _check(lib().dc_gen_rmat(scale, edge_factor, a, b, c, seed, u, v, C.byref(m)))
+    return u[: m.value].copy(), v[: m.value].copy()
+
+
+def gen_batches(n, u, v, nb, bs, ins_frac, seed):
+    u = _arr(u, np.int32)
+    v = _arr(v, np.int32)
+    tot = max(nb * bs, 1)
+    bu = np.zeros(tot, np.int32)
+    bv = np.zeros(tot, np.int32)
+    bi = np.zeros(tot, np.uint8)
+    _check(lib().dc_gen_batches(n, u.size, _ptr(u), _ptr(v), nb, bs, ins_frac, seed, bu, bv, bi))
+    return [(bu[i * bs:(i + 1) * bs], bv[i * bs:(i + 1) * bs], bi[i * bs:(i + 1) * bs])
+            for i in range(nb)]
+
+
+# --------------------------------------------------------------------- graph
+class Graph:
+    """A device-resident graph and its coloring (one dc_graph handle)."""
+
+    def __init__(self, n, u, v, opts: Options | None = None, n_ghost=0, ghost_colors=None):
+        u = _arr(u, np.int32)
+        v = _arr(v, np.int32)
+        h = C.c_void_p()
+        o = C.byref(opts) if opts is not None else None
+        if n_ghost:
+            gc = _arr(ghost_colors, np.int32)
+            _check(lib().dc_partition_create(n, n_ghost, u.size, _ptr(u), _ptr(v), _ptr(gc), o,
+                                             C.byref(h)))
+            self.n = n + n_ghost
+            self.n_owned = n
+        else:
+            _check(lib().dc_graph_create(n, u.size, _ptr(u), _ptr(v), o, C.byref(h)))
+            self.n = n
+            self.n_owned = n
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().dc_graph_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def handle(self):
+        return self._h
+
+    def colors(self) -> np.ndarray:
+        out = np.zeros(self.n, np.int32)
+        _check(lib().dc_colors_get(self._h, out))
+        return out
+
+    def set_colors(self, colors):
+        _check(lib().dc_colors_set(self._h, _arr(colors, np.int32)))
+
+    def static_color(self, rule=RULE_INDEPENDENT):
+        _check(lib().dc_static_color(self._h, rule))
+
+    def apply_batch(self, u, v, ins):
+        u = _arr(u, np.int32)
+        v = _arr(v, np.int32)
+        ins = _arr(ins, np.uint8)
+        _check(lib().dc_apply_batch(self._h, u.size, _ptr(u), _ptr(v), _ptr(ins)))
+
+    def upload_batch(self, u, v, ins) -> "DeviceBatch":
+        return DeviceBatch(self, u, v, ins)
+
+    def apply_device_batch(self, b: "DeviceBatch"):
+        _check(lib().dc_apply_device_batch(self._h, b._h))
+
+    def recolor(self, rule=RULE_INDEPENDENT) -> Stats:
+        st = Stats()
+        _check(lib().dc_recolor(self._h, rule, C.byref(st)))
+        return st
+
+    def process_batch(self, u, v, ins, rule=RULE_INDEPENDENT) -> Stats:
+        self.apply_batch(u, v, ins)
+        return self.recolor(rule)
+
+    def validate(self):
+        mono = C.c_int64(0)
+        mc = C.c_int32(0)
+        _check(lib().dc_validate(self._h, C.byref(mono), C.byref(mc)))
+        return int(mono.value), int(mc.value)
+
+    def trace(self) -> np.ndarray:
+        n = C.c_int32(0)
+        buf = np.zeros(4, np.int64)
+        _check(lib().dc_trace_get(self._h, buf, 0, C.byref(n)))
+        buf = np.zeros(4 * max(n.value, 1), np.int64)
+        _check(lib().dc_trace_get(self._h, buf, n.value, C.byref(n)))
+        return buf[: 4 * n.value].reshape(-1, 4)
+
+    def neighbors(self, vtx) -> np.ndarray:
+        d = C.c_int64(0)
+        tmp = np.zeros(1, np.int32)
+        _check(lib().dc_neighbors(self._h, vtx, tmp, 0, C.byref(d)))
+        out = np.zeros(max(d.value, 1), np.int32)
+        _check(lib().dc_neighbors(self._h, vtx, out, out.size, C.byref(d)))
+        return out[: d.value]
+
+    def prop(self) -> np.ndarray:
+        out = np.zeros(self.n, np.int32)
+        _check(lib().dc_prop_get(self._h, out))
+        return out
+
+    def compact(self):
+        _check(lib().dc_compact(self._h))
+
+    def set_profiling(self, on=True):
+        _check(lib().dc_set_profiling(self._h, 1 if on else 0))
+
+    def synchronize(self):
+        _check(lib().dc_synchronize(self._h))
+
+
+class DeviceBatch:
+    """An update batch validated and copied to HBM once."""
+
+    def __init__(self, g: Graph, u, v, ins):
+        u = _arr(u, np.int32)
+        v = _arr(v, np.int32)
+        ins = _arr(ins, np.uint8)
+        h = C.c_void_p()
+        _check(lib().dc_batch_create(g.handle, u.size, _ptr(u), _ptr(v), _ptr(ins), C.byref(h)))
+        self._h = h
+        self.k = int(u.size)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().dc_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,572 @@
+// csr.hip -- device-resident undirected CSR with per-row spare capacity and
+// tombstones: build, batch apply (insert / tombstone / LDF counter), regrow,
+// compaction, and the scan used to lay rows out.
+//
+// Layout: row[v] = {start, used}; slots [start, start+used) hold neighbours,
+// -1 marks a deleted slot; [used, cap[v]) is spare room for insertions.  Both
+// directions of an edge are stored; ghost rows (partition share) are empty.
+// This replaces the reference's host-flattened AoS {col, flag} rows that are
+// rebuilt on every run (HI/supportingFunctions.cu:27-67).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+
+#include "dc_device.h"
+#include "dc_internal.h"
+
+namespace dc {
+
+// ---------------------------------------------------------------- scan -----
+// Exclusive scan of n u32 values into u64 (+ total at out[n]).  Three passes:
+// per-block sums, one-block scan of the block sums, per-block rescan.
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kBlock * kScanItems;
+
+__device__ __forceinline__ uint64_t block_exclusive_scan(uint64_t x, uint64_t* total) {
+  __shared__ uint64_t wsum[kBlock / 64];
+  int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  uint64_t inc = x;
+  for (int d = 1; d < 64; d <<= 1) {
+    uint64_t y = __shfl_up(inc, d);
+    if (lane >= d) inc += y;
+  }
+  if (lane == 63) wsum[w] = inc;
+  __syncthreads();
+  uint64_t off = 0, tot = 0;
+  for (int i = 0; i < kBlock / 64; ++i) {
+    if (i < w) off += wsum[i];
+    tot += wsum[i];
+  }
+  __syncthreads();
+  *total = tot;
+  return off + inc - x;
+}
+
+__global__ void k_scan_partials(const uint32_t* in, int32_t n, uint64_t* part) {
+  int64_t base = (int64_t)blockIdx.x * kScanTile;
+  uint64_t s = 0;
+  for (int i = 0; i < kScanItems; ++i) {
+    int64_t j = base + (int64_t)i * kBlock + threadIdx.x;
+    if (j < n) s += in[j];
+  }
+  uint64_t tot;
+  block_exclusive_scan(s, &tot);
+  if (threadIdx.x == 0) part[blockIdx.x] = tot;
+}
+
+__global__ void k_scan_partials_scan(uint64_t* part, int32_t nb) {
+  // single block: sequential over tiles of kBlock partials
+  __shared__ uint64_t carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (int32_t t = 0; t < nb; t += kBlock) {
+    int32_t j = t + threadIdx.x;
+    uint64_t x = j < nb ? part[j] : 0;
+    uint64_t tot;
+    uint64_t ex = block_exclusive_scan(x, &tot);
+    if (j < nb) part[j] = carry + ex;
+    __syncthreads();
+    if (threadIdx.x == 0) carry += tot;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[nb] = carry;
+}
+
+__global__ void k_scan_final(const uint32_t* in, int32_t n, const uint64_t* part, uint64_t* out) {
+  int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
+  uint32_t v[kScanItems];
+  uint64_t s = 0;
+  for (int i = 0; i < kScanItems; ++i) {
+    int64_t j = base + i;
+    v[i] = j < n ? in[j] : 0u;
+    s += v[i];
+  }
+  uint64_t tot;
+  uint64_t ex = block_exclusive_scan(s, &tot) + part[blockIdx.x];
+  for (int i = 0; i < kScanItems; ++i) {
+    int64_t j = base + i;
+    if (j < n) out[j] = ex;
+    ex += v[i];
+  }
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == kBlock - 1) out[n] = part[gridDim.x];
+}
+
+int exclusive_scan_u32_to_u64(Graph* g, const uint32_t* in, uint64_t* out, int32_t n) {
+  int32_t nb = (n + kScanTile - 1) / kScanTile;
+  if (nb == 0) nb = 1;
+  if (g->scan_tmp_n < (size_t)nb + 1) {
+    if (g->scan_tmp) DC_HIP(hipFree(g->scan_tmp));
+    g->scan_tmp = nullptr;
+    DC_HIP(hipMalloc(&g->scan_tmp, sizeof(uint64_t) * ((size_t)nb + 1)));
+    g->scan_tmp_n = (size_t)nb + 1;
+  }
+  hipLaunchKernelGGL(k_scan_partials, dim3(nb), dim3(kBlock), 0, g->stream, in, n, g->scan_tmp);
+  hipLaunchKernelGGL(k_scan_partials_scan, dim3(1), dim3(kBlock), 0, g->stream, g->scan_tmp, nb);
+  hipLaunchKernelGGL(k_scan_final, dim3(nb), dim3(kBlock), 0, g->stream, in, n, g->scan_tmp, out);
+  DC_HIP(hipGetLastError());
+  return DC_OK;
+}
+
+// ------------------------------------------------------------- build -------
+static inline int grid_for(int64_t items, int per_block = kBlock) {
+  int64_t b = (items + per_block - 1) / per_block;
+  if (b < 1) b = 1;
+  if (b > 65535LL * 16) b = 65535LL * 16;
+  return (int)b;
+}
+
+__global__ void k_count_degree(int64_t m, const int32_t* u, const int32_t* v, int32_t n_owned,
+                               uint32_t* deg, int32_t* prop) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < m;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    int32_t a = u[i], b = v[i];
+    if (a < n_owned) atomicAdd(&deg[a], 1u);
+    if (b < n_owned) atomicAdd(&deg[b], 1u);
+    atomicAdd(&prop[a], 1);
+    atomicAdd(&prop[b], 1);
+  }
+}
+
+__device__ __forceinline__ uint32_t slack_for(uint32_t used, float frac, int32_t smin) {
+  uint32_t s = (uint32_t)((float)used * frac + 0.999f);
+  return s < (uint32_t)smin ? (uint32_t)smin : s;
+}
+
+// cap[v] <- max(cap[v], used + slack(used)) for owned rows, 0 for ghosts.
+// With keep_old = 0 the input cap holds the degree and is replaced.
+__global__ void k_set_cap(int32_t n, int32_t n_owned, uint32_t* cap, const uint2* row,
+                          int keep_old, float frac, int32_t smin) {
+  for (int32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
+    if (v >= n_owned) {
+      cap[v] = 0;
+      continue;
+    }
+    if (keep_old) {
+      uint32_t used = row[v].y;
+      uint32_t c = cap[v];
+      if (used > c) c = used + slack_for(used, frac, smin);
+      cap[v] = c;
+    } else {
+      uint32_t d = cap[v];
+      cap[v] = d + slack_for(d, frac, smin);
+    }
+  }
+}
+
+__global__ void k_set_rows(int32_t n, const uint64_t* start, uint2* row, int reset_used) {
+  for (int32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
+    uint2 r = row[v];
+    r.x = (uint32_t)start[v];
+    if (reset_used) r.y = 0;
+    row[v] = r;
+  }
+}
+
+__global__ void k_fill(int64_t m, const int32_t* u, const int32_t* v, int32_t n_owned, uint2* row,
+                       int32_t* col) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < m;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    int32_t a = u[i], b = v[i];
+    if (a < n_owned) {
+      uint32_t p = atomicAdd(&row[a].y, 1u);
+      col[(uint64_t)row[a].x + p] = b;
+    }
+    if (b < n_owned) {
+      uint32_t p = atomicAdd(&row[b].y, 1u);
+      col[(uint64_t)row[b].x + p] = a;
+    }
+  }
+}
+
+// count vertices / chunks that can need the hub path: rows with cap > kMidLen
+__global__ void k_hub_capacity(int32_t n, const uint32_t* cap, uint32_t* out) {
+  uint32_t hubs = 0, chunks = 0;
+  for (int32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
+    uint32_t c = cap[v];
+    if (c > kMidLen) {
+      hubs += 1;
+      chunks += (c + kChunk - 1) / kChunk;
+    }
+  }
+  wave_add_u32(hubs, &out[0]);
+  wave_add_u32(chunks, &out[1]);
+}
+
+static int ensure_hub_buffers(Graph* g) {
+  DC_HIP(hipMemsetAsync(g->ctr, 0, sizeof(Counters), g->stream));
+  uint32_t* tmp = reinterpret_cast<uint32_t*>(g->ctr);  // borrow 2 words, cleared again below
+  hipLaunchKernelGGL(k_hub_capacity, dim3(grid_for(g->n)), dim3(kBlock), 0, g->stream, g->n, g->cap,
+                     tmp);
+  uint32_t h[2];
+  DC_HIP(hipMemcpyAsync(h, tmp, sizeof(h), hipMemcpyDeviceToHost, g->stream));
+  DC_HIP(hipStreamSynchronize(g->stream));
+  DC_HIP(hipMemsetAsync(g->ctr, 0, sizeof(Counters), g->stream));
+  uint32_t need_hubs = h[0] + 1, need_chunks = h[1] + 1;
+  if (need_hubs > g->hub_cap) {
+    if (g->hubmask) DC_HIP(hipFree(g->hubmask));
+    if (g->hubdone) DC_HIP(hipFree(g->hubdone));
+    g->hubmask = nullptr;
+    g->hubdone = nullptr;
+    uint32_t c = need_hubs + need_hubs / 4 + 16;
+    DC_HIP(hipMalloc(&g->hubmask, sizeof(unsigned long long) * kWinWords * (size_t)c));
+    DC_HIP(hipMalloc(&g->hubdone, sizeof(uint32_t) * (size_t)c));
+    DC_HIP(hipMemsetAsync(g->hubmask, 0, sizeof(unsigned long long) * kWinWords * (size_t)c,
+                          g->stream));
+    DC_HIP(hipMemsetAsync(g->hubdone, 0, sizeof(uint32_t) * (size_t)c, g->stream));
+    g->hub_cap = c;
+  }
+  if (need_chunks > g->hchunk_cap) {
+    if (g->hbuf) DC_HIP(hipFree(g->hbuf));
+    g->hbuf = nullptr;
+    uint32_t c = need_chunks + need_chunks / 4 + 16;
+    DC_HIP(hipMalloc(&g->hbuf, sizeof(int4) * (size_t)c * 4));
+    g->hchunk_cap = c;
+    for (int i = 0; i < 3; ++i) g->F[i].h = g->hbuf + (size_t)c * i;
+    g->T.h = g->hbuf + (size_t)c * 3;
+  }
+  return DC_OK;
+}
+
+int alloc_state(Graph* g) {
+  size_t n = (size_t)g->n + 1;
+  DC_HIP(hipMalloc(&g->row, sizeof(uint2) * n));
+  DC_HIP(hipMalloc(&g->cap, sizeof(uint32_t) * n));
+  DC_HIP(hipMalloc(&g->prop, sizeof(int32_t) * n));
+  DC_HIP(hipMalloc(&g->color[0], sizeof(int32_t) * n));
+  DC_HIP(hipMalloc(&g->color[1], sizeof(int32_t) * n));
+  DC_HIP(hipMalloc(&g->prev, sizeof(int32_t) * n));
+  DC_HIP(hipMalloc(&g->tmark, sizeof(uint32_t) * n));
+  DC_HIP(hipMalloc(&g->mmark, sizeof(uint32_t) * n));
+  DC_HIP(hipMalloc(&g->pending, sizeof(int32_t) * n));
+  // 4 frontiers (F0, F1, F2=marked set, T) x 2 int lists (S, M)
+  DC_HIP(hipMalloc(&g->fbuf, sizeof(int32_t) * n * 8));
+  DC_HIP(hipMalloc(&g->fcnt, sizeof(uint32_t) * 16));
+  for (int i = 0; i < 3; ++i) {
+    g->F[i].s = g->fbuf + n * (2 * i);
+    g->F[i].m = g->fbuf + n * (2 * i + 1);
+    g->F[i].cnt = g->fcnt + 4 * i;
+  }
+  g->T.s = g->fbuf + n * 6;
+  g->T.m = g->fbuf + n * 7;
+  g->T.cnt = g->fcnt + 12;
+  DC_HIP(hipMalloc(&g->ctr, sizeof(Counters)));
+  DC_HIP(hipHostMalloc(&g->hctr, sizeof(Counters), hipHostMallocDefault));
+  DC_HIP(hipMemsetAsync(g->row, 0, sizeof(uint2) * n, g->stream));
+  DC_HIP(hipMemsetAsync(g->cap, 0, sizeof(uint32_t) * n, g->stream));
+  DC_HIP(hipMemsetAsync(g->prop, 0, sizeof(int32_t) * n, g->stream));
+  DC_HIP(hipMemsetAsync(g->color[0], 0, sizeof(int32_t) * n, g->stream));
+  DC_HIP(hipMemsetAsync(g->color[1], 0, sizeof(int32_t) * n, g->stream));
+  DC_HIP(hipMemsetAsync(g->prev, 0xff, sizeof(int32_t) * n, g->stream));
+  DC_HIP(hipMemsetAsync(g->tmark, 0, sizeof(uint32_t) * n, g->stream));
+  DC_HIP(hipMemsetAsync(g->mmark, 0, sizeof(uint32_t) * n, g->stream));
+  DC_HIP(hipMemsetAsync(g->fcnt, 0, sizeof(uint32_t) * 16, g->stream));
+  DC_HIP(hipMemsetAsync(g->ctr, 0, sizeof(Counters), g->stream));
+  for (int i = 0; i < 4; ++i) DC_HIP(hipEventCreate(&g->ev[i]));
+  g->epoch = 0;
+  return DC_OK;
+}
+
+void free_state(Graph* g) {
+  auto f = [](void* p) {
+    if (p) (void)hipFree(p);
+  };
+  f(g->row); f(g->cap); f(g->col); f(g->prop);
+  f(g->color[0]); f(g->color[1]); f(g->prev);
+  f(g->tmark); f(g->mmark); f(g->pending);
+  f(g->fbuf); f(g->fcnt); f(g->hbuf); f(g->hubmask); f(g->hubdone);
+  f(g->del); f(g->ins); f(g->xdel); f(g->xins); f(g->ovf);
+  f(g->ctr); f(g->scan_tmp);
+  f(g->scratch_batch.u); f(g->scratch_batch.v); f(g->scratch_batch.ins);
+  if (g->hctr) (void)hipHostFree(g->hctr);
+  for (int i = 0; i < 4; ++i)
+    if (g->ev[i]) (void)hipEventDestroy(g->ev[i]);
+}
+
+// Lay out rows from cap[] (already final) and allocate col[].
+static int layout_rows(Graph* g, int reset_used, int32_t** new_col) {
+  uint64_t* start = nullptr;
+  DC_HIP(hipMalloc(&start, sizeof(uint64_t) * ((size_t)g->n + 1)));
+  int rc = exclusive_scan_u32_to_u64(g, g->cap, start, g->n);
+  if (rc) {
+    (void)hipFree(start);
+    return rc;
+  }
+  uint64_t total = 0;
+  DC_HIP(hipMemcpyAsync(&total, start + g->n, sizeof(uint64_t), hipMemcpyDeviceToHost, g->stream));
+  DC_HIP(hipStreamSynchronize(g->stream));
+  if (total >= 0xffffffffull) {
+    (void)hipFree(start);
+    set_error("graph exceeds 2^32 adjacency slots");
+    return DC_ERR_NOMEM;
+  }
+  int32_t* col = nullptr;
+  hipError_t e = hipMalloc(&col, sizeof(int32_t) * (size_t)(total + 1));
+  if (e != hipSuccess) {
+    (void)hipFree(start);
+    return hip_fail(e, "hipMalloc(col)");
+  }
+  hipLaunchKernelGGL(k_set_rows, dim3(grid_for(g->n)), dim3(kBlock), 0, g->stream, g->n, start,
+                     g->row, reset_used);
+  DC_HIP(hipGetLastError());
+  DC_HIP(hipFree(start));
+  g->total_cap = total;
+  *new_col = col;
+  return DC_OK;
+}
+
+int csr_build(Graph* g, int64_t m, const int32_t* du, const int32_t* dv) {
+  // cap[] first holds the degree
+  DC_HIP(hipMemsetAsync(g->cap, 0, sizeof(uint32_t) * ((size_t)g->n + 1), g->stream));
+  if (m > 0)
+    hipLaunchKernelGGL(k_count_degree, dim3(grid_for(m)), dim3(kBlock), 0, g->stream, m, du, dv,
+                       g->n_owned, g->cap, g->prop);
+  hipLaunchKernelGGL(k_set_cap, dim3(grid_for(g->n)), dim3(kBlock), 0, g->stream, g->n, g->n_owned,
+                     g->cap, g->row, 0, g->opt.slack_frac, g->opt.slack_min);
+  DC_HIP(hipGetLastError());
+  int32_t* col = nullptr;
+  int rc = layout_rows(g, 1, &col);
+  if (rc) return rc;
+  g->col = col;
+  if (m > 0)
+    hipLaunchKernelGGL(k_fill, dim3(grid_for(m)), dim3(kBlock), 0, g->stream, m, du, dv,
+                       g->n_owned, g->row, g->col);
+  DC_HIP(hipGetLastError());
+  return ensure_hub_buffers(g);
+}
+
+// ------------------------------------------------------------- apply -------
+// One thread per change line: append insertions to both rows (reserving a
+// slot with an atomic on the row's used count), move the LDF counter, and
+// route the line into the staged lists.  Single graph: an insertion is kept
+// for the insertion round only when its endpoints had equal batch-start
+// colors (HI/all_structure_undir.cuh:106-108, HB/all_structure_undir.cuh:
+// 106-109).  Partition share: no pre-filter, cross edges (one ghost endpoint)
+// go to their own lists as (owned, ghost) (MG/all_structure_undir.cuh:
+// 100-188).
+__global__ void k_apply(int64_t k, const int32_t* bu, const int32_t* bv, const uint8_t* bins,
+                        GraphView G, const int32_t* color, int2* del, int2* ins, int2* xdel,
+                        int2* xins, int4* ovf, Counters* ctr) {
+  const bool partition = G.n_owned < G.n;
+  for (int64_t base = blockIdx.x * (int64_t)blockDim.x; base < k;
+       base += (int64_t)gridDim.x * blockDim.x) {
+    int64_t i = base + threadIdx.x;
+    bool ok = i < k;
+    int32_t a = 0, b = 0;
+    bool is_ins = false;
+    if (ok) {
+      a = bu[i];
+      b = bv[i];
+      is_ins = bins[i] != 0;
+      if (a >= G.n_owned) {
+        int32_t t = a;
+        a = b;
+        b = t;
+      }
+    }
+    bool cross = ok && b >= G.n_owned;
+    bool ovf_a = false, ovf_b = false;
+    uint32_t pa = 0, pb = 0;
+    if (ok && is_ins) {
+      uint2 ra = G.row[a];
+      pa = atomicAdd(&G.row[a].y, 1u);
+      if (pa < G.cap[a]) G.col[(uint64_t)ra.x + pa] = b;
+      else ovf_a = true;
+      if (!cross) {
+        uint2 rb = G.row[b];
+        pb = atomicAdd(&G.row[b].y, 1u);
+        if (pb < G.cap[b]) G.col[(uint64_t)rb.x + pb] = a;
+        else ovf_b = true;
+      }
+      atomicAdd(&G.prop[a], 1);
+      atomicAdd(&G.prop[b], 1);
+    } else if (ok) {
+      atomicSub(&G.prop[a], 1);
+      atomicSub(&G.prop[b], 1);
+    }
+    wave_append(ovf_a, make_int4(a, (int)pa, b, 0), ovf, &ctr->overflow);
+    wave_append(ovf_b, make_int4(b, (int)pb, a, 0), ovf, &ctr->overflow);
+    bool eq0 = ok && color[a] == color[b];
+    int2 e = make_int2(a, b);
+    wave_append(ok && !is_ins && !cross, e, del, &ctr->n_del);
+    wave_append(ok && !is_ins && cross, e, xdel, &ctr->n_xdel);
+    wave_append(ok && is_ins && !cross && (partition || eq0), e, ins, &ctr->n_ins);
+    wave_append(ok && is_ins && cross, e, xins, &ctr->n_xins);
+  }
+}
+
+// One wave per (deletion line, row): tombstone every slot of the row equal to
+// the other endpoint (all copies, including ones inserted by this batch;
+// deleteEdgeFromAdj, HI/gpuFunctions_undir.cuh:118-142).
+__global__ void k_tombstone(const int2* del, uint32_t ndel, const int2* xdel, uint32_t nxdel,
+                            GraphView G, Counters* ctr) {
+  const int lane = lane_id();
+  const int64_t wave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  const int64_t items = 2 * (int64_t)ndel + nxdel;
+  unsigned long long tomb = 0;
+  for (int64_t it = wave; it < items; it += nwaves) {
+    int32_t x, y;
+    if (it < 2 * (int64_t)ndel) {
+      int2 e = del[it >> 1];
+      x = (it & 1) ? e.y : e.x;
+      y = (it & 1) ? e.x : e.y;
+    } else {
+      int2 e = xdel[it - 2 * (int64_t)ndel];
+      x = e.x;
+      y = e.y;
+    }
+    uint2 r = G.row[x];
+    for (uint32_t j = lane; j < r.y; j += 64) {
+      int32_t* p = &G.col[(uint64_t)r.x + j];
+      if (*p == y) {
+        *p = -1;
+        ++tomb;
+      }
+    }
+  }
+  wave_add_u64(tomb, &ctr->tomb);
+}
+
+// regrow: copy every row's slots [0, min(used, old cap)) to the new layout
+__global__ void k_copy_rows(int32_t n, const uint32_t* old_start, const uint32_t* old_cap,
+                            const int32_t* old_col, const uint2* row, int32_t* col) {
+  const int lane = lane_id();
+  const int64_t wave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  for (int64_t v = wave; v < n; v += nwaves) {
+    uint32_t used = row[v].y;
+    uint32_t c = old_cap[v];
+    uint32_t lim = used < c ? used : c;
+    uint64_t src = old_start[v], dst = row[v].x;
+    for (uint32_t j = lane; j < lim; j += 64) col[dst + j] = old_col[src + j];
+  }
+}
+
+__global__ void k_save_starts(int32_t n, const uint2* row, const uint32_t* cap, uint32_t* st,
+                              uint32_t* oc) {
+  for (int32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
+    st[v] = row[v].x;
+    oc[v] = cap[v];
+  }
+}
+
+__global__ void k_fix_overflow(const int4* ovf, uint32_t novf, GraphView G) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < novf; i += gridDim.x * blockDim.x) {
+    int4 o = ovf[i];
+    G.col[(uint64_t)G.row[o.x].x + (uint32_t)o.y] = o.z;
+  }
+}
+
+// Give every row that over-ran its capacity new room (used + slack), relay
+// all rows, then write the slots that did not fit.
+static int regrow(Graph* g, uint32_t novf) {
+  uint32_t *old_start = nullptr, *old_cap = nullptr;
+  DC_HIP(hipMalloc(&old_start, sizeof(uint32_t) * ((size_t)g->n + 1)));
+  DC_HIP(hipMalloc(&old_cap, sizeof(uint32_t) * ((size_t)g->n + 1)));
+  hipLaunchKernelGGL(k_save_starts, dim3(grid_for(g->n)), dim3(kBlock), 0, g->stream, g->n, g->row,
+                     g->cap, old_start, old_cap);
+  hipLaunchKernelGGL(k_set_cap, dim3(grid_for(g->n)), dim3(kBlock), 0, g->stream, g->n, g->n_owned,
+                     g->cap, g->row, 1, g->opt.slack_frac, g->opt.slack_min);
+  DC_HIP(hipGetLastError());
+  int32_t* col = nullptr;
+  int rc = layout_rows(g, 0, &col);
+  if (rc) {
+    (void)hipFree(old_start);
+    (void)hipFree(old_cap);
+    return rc;
+  }
+  hipLaunchKernelGGL(k_copy_rows, dim3(grid_for((int64_t)g->n * 64)), dim3(kBlock), 0, g->stream,
+                     g->n, old_start, old_cap, g->col, g->row, col);
+  DC_HIP(hipGetLastError());
+  DC_HIP(hipStreamSynchronize(g->stream));
+  DC_HIP(hipFree(g->col));
+  DC_HIP(hipFree(old_start));
+  DC_HIP(hipFree(old_cap));
+  g->col = col;
+  hipLaunchKernelGGL(k_fix_overflow, dim3(grid_for(novf)), dim3(kBlock), 0, g->stream, g->ovf, novf,
+                     g->view());
+  DC_HIP(hipGetLastError());
+  return ensure_hub_buffers(g);
+}
+
+static int ensure_lists(Graph* g, int64_t k) {
+  if (k <= g->list_cap) return DC_OK;
+  auto f = [](void* p) {
+    if (p) (void)hipFree(p);
+  };
+  f(g->del); f(g->ins); f(g->xdel); f(g->xins); f(g->ovf);
+  g->del = g->ins = g->xdel = g->xins = nullptr;
+  g->ovf = nullptr;
+  int64_t c = k + k / 4 + 64;
+  DC_HIP(hipMalloc(&g->del, sizeof(int2) * (size_t)c));
+  DC_HIP(hipMalloc(&g->ins, sizeof(int2) * (size_t)c));
+  DC_HIP(hipMalloc(&g->xdel, sizeof(int2) * (size_t)c));
+  DC_HIP(hipMalloc(&g->xins, sizeof(int2) * (size_t)c));
+  DC_HIP(hipMalloc(&g->ovf, sizeof(int4) * (size_t)(2 * c)));
+  g->list_cap = c;
+  return DC_OK;
+}
+
+int csr_apply(Graph* g, const Batch& b, int rule) {
+  (void)rule;
+  int rc = ensure_lists(g, b.k);
+  if (rc) return rc;
+  DC_HIP(hipMemsetAsync(g->ctr, 0, sizeof(Counters), g->stream));
+  if (b.k > 0)
+    hipLaunchKernelGGL(k_apply, dim3(grid_for(b.k)), dim3(kBlock), 0, g->stream, b.k, b.u, b.v,
+                       b.ins, g->view(), g->color[g->cur], g->del, g->ins, g->xdel, g->xins, g->ovf,
+                       g->ctr);
+  DC_HIP(hipGetLastError());
+  DC_HIP(hipMemcpyAsync(g->hctr, g->ctr, sizeof(Counters), hipMemcpyDeviceToHost, g->stream));
+  DC_HIP(hipStreamSynchronize(g->stream));
+  Counters c = *g->hctr;
+  g->staged_regrows = 0;
+  if (c.overflow > 0) {
+    rc = regrow(g, c.overflow);
+    if (rc) return rc;
+    g->staged_regrows = 1;
+  }
+  uint32_t items = 2 * c.n_del + c.n_xdel;
+  if (items > 0) {
+    int64_t threads = (int64_t)items * 64;
+    hipLaunchKernelGGL(k_tombstone, dim3(grid_for(threads)), dim3(kBlock), 0, g->stream, g->del,
+                       c.n_del, g->xdel, c.n_xdel, g->view(), g->ctr);
+    DC_HIP(hipGetLastError());
+  }
+  g->staged = true;
+  return DC_OK;
+}
+
+// ------------------------------------------------------------ compact ------
+// In-place compaction of each row's live slots (ballot/mbcnt within the wave;
+// writes never pass the slot being read).
+__global__ void k_compact_rows(GraphView G) {
+  const int lane = lane_id();
+  const int64_t wave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  for (int64_t v = wave; v < G.n_owned; v += nwaves) {
+    uint2 r = G.row[v];
+    uint32_t out = 0;
+    for (uint32_t j0 = 0; j0 < r.y; j0 += 64) {
+      uint32_t j = j0 + lane;
+      int32_t w = j < r.y ? G.col[(uint64_t)r.x + j] : -1;
+      bool live = w >= 0;
+      unsigned long long m = __ballot(live);
+      if (live) G.col[(uint64_t)r.x + out + lanes_below(m)] = w;
+      out += (uint32_t)__popcll(m);
+    }
+    if (lane == 0) G.row[v].y = out;
+  }
+}
+
+int csr_compact(Graph* g) {
+  hipLaunchKernelGGL(k_compact_rows, dim3(grid_for((int64_t)g->n_owned * 64)), dim3(kBlock), 0,
+                     g->stream, g->view());
+  DC_HIP(hipGetLastError());
+  DC_HIP(hipStreamSynchronize(g->stream));
+  return DC_OK;
+}
+
+}  // namespace dc

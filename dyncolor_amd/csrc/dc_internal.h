@@ -1,0 +1,170 @@
+// dc_internal.h -- device-resident graph state shared by the library's
+// translation units.  Host-side only declarations plus POD structs that are
+// passed to kernels by value.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/dyncolor.h"
+
+namespace dc {
+
+// Degree classes of a frontier.  A visited row is read by
+//   S: an 8-lane group          (used slots < kSmallLen, mex fits one u64),
+//   M: one wave64               (kSmallLen <= slots <= kMidLen, 8192-bit LDS window),
+//   H: whole workgroups, one per kChunk-slot chunk (slots > kMidLen), partial
+//      masks merged in a per-hub global window.
+constexpr int kBlock = 256;
+constexpr int kGroup = 8;
+constexpr uint32_t kSmallLen = 64;
+constexpr uint32_t kMidLen = 4096;
+constexpr uint32_t kChunk = 4096;
+constexpr int kWinBits = 8192;               // LDS mex window (bits)
+constexpr int kWinWords = kWinBits / 64;     // u64 words per window
+
+// One frontier = three class lists + counters.
+struct Frontier {
+  int32_t* s;      // class S vertex list
+  int32_t* m;      // class M vertex list
+  int4* h;         // class H chunks: {vertex, chunk index, hub slot, chunks}
+  uint32_t* cnt;   // [0]=|S| [1]=|M| [2]=#H chunks [3]=#H vertices (hub slots)
+};
+
+// Per-round device counters (one cache line each group).
+struct Counters {
+  uint32_t changed;
+  uint32_t targets;
+  uint32_t overflow;      // insertion slots past row capacity
+  uint32_t n_del, n_ins, n_xdel, n_xins;
+  uint32_t pad0;
+  unsigned long long slots;   // row slots scanned (find / target kernels)
+  unsigned long long live;    // live entries scanned (find / target kernels)
+  unsigned long long rslots;  // row slots scanned by the recolor kernel
+  unsigned long long rlive;   // live entries scanned by the recolor kernel
+  unsigned long long tomb;    // tombstones written
+  unsigned long long mono;    // validate
+  int32_t maxc;
+  int32_t window_full;    // a hub mex window was full -> slow path used
+  uint32_t pad1[2];
+};
+
+// Everything a kernel needs about one graph, passed by value.
+struct GraphView {
+  int32_t n;          // total vertices (owned + ghosts)
+  int32_t n_owned;    // vertices >= n_owned are frozen ghosts
+  uint2* row;         // {start, used slots}
+  uint32_t* cap;      // row capacity
+  int32_t* col;       // slot -> neighbour, -1 tombstone
+  int32_t* prop;      // LDF priority counter
+};
+
+struct Batch {
+  int64_t k = 0;
+  int32_t* u = nullptr;
+  int32_t* v = nullptr;
+  uint8_t* ins = nullptr;
+};
+
+struct Graph {
+  int device = 0;
+  dc_options opt{};
+  int32_t n = 0, n_owned = 0;
+  uint64_t total_cap = 0;     // col[] slots
+  hipStream_t stream = nullptr;
+
+  // CSR
+  uint2* row = nullptr;
+  uint32_t* cap = nullptr;
+  int32_t* col = nullptr;
+  int32_t* prop = nullptr;
+
+  // colors: double buffer for round-synchronous snapshots
+  int32_t* color[2] = {nullptr, nullptr};
+  int cur = 0;
+  int32_t* prev = nullptr;
+
+  // marks (epoch stamped)
+  uint32_t* tmark = nullptr;  // per-round target/candidate dedupe
+  uint32_t* mmark = nullptr;  // per-batch marked-set dedupe
+  uint32_t epoch = 0;
+
+  // frontiers: F[0], F[1] alternate; M = pre-loop marked set; T = round targets
+  Frontier F[3] = {};
+  Frontier T = {};
+  int32_t* fbuf = nullptr;    // backing store of the int lists
+  int4* hbuf = nullptr;       // backing store of the hub-chunk lists
+  uint32_t* fcnt = nullptr;   // counters of all frontiers
+  uint32_t hub_cap = 0;       // max hub slots per round
+  uint32_t hchunk_cap = 0;    // max hub chunks per frontier
+  unsigned long long* hubmask = nullptr;  // hub_cap x kWinWords
+  uint32_t* hubdone = nullptr;            // hub_cap
+
+  // staged batch lists: int2 pairs
+  int2* del = nullptr;
+  int2* ins = nullptr;
+  int2* xdel = nullptr;
+  int2* xins = nullptr;
+  int4* ovf = nullptr;        // insertion overflow {vertex, pos, value, 0}
+  int64_t list_cap = 0;
+  bool staged = false;
+  int64_t staged_k = 0, staged_ins = 0, staged_del = 0;
+  int32_t staged_regrows = 0;
+
+  Counters* ctr = nullptr;    // device
+  Counters* hctr = nullptr;   // pinned host mirror
+
+  // upload staging for host-array batches
+  Batch scratch_batch;
+
+  // scan scratch
+  uint64_t* scan_tmp = nullptr;
+  size_t scan_tmp_n = 0;
+
+  // static coloring
+  int32_t* pending = nullptr;
+
+  // trace of the last recolor
+  std::vector<int64_t> trace;
+
+  bool profiling = false;
+  hipEvent_t ev[4] = {};
+  std::vector<hipEvent_t> evpool;   // start/stop pairs of profiled launches
+  size_t evused = 0;
+
+  GraphView view() const { return GraphView{n, n_owned, row, cap, col, prop}; }
+};
+
+// error plumbing
+void set_error(const std::string& msg);
+int hip_fail(hipError_t e, const char* where);
+
+#define DC_HIP(expr)                                       \
+  do {                                                     \
+    hipError_t _e = (expr);                                \
+    if (_e != hipSuccess) return ::dc::hip_fail(_e, #expr); \
+  } while (0)
+
+// csr.hip
+int csr_build(Graph* g, int64_t m, const int32_t* du, const int32_t* dv);
+int csr_apply(Graph* g, const Batch& b, int rule);
+int csr_compact(Graph* g);
+int alloc_state(Graph* g);
+void free_state(Graph* g);
+int exclusive_scan_u32_to_u64(Graph* g, const uint32_t* in, uint64_t* out, int32_t n);
+
+// recolor.hip
+int recolor_plain(Graph* g, int rule, dc_stats* st);
+int static_color(Graph* g, int rule);
+int validate(Graph* g, int64_t* mono, int32_t* maxc);
+
+inline uint32_t next_epoch(Graph* g) {
+  // callers reset the mark arrays when the epoch would wrap
+  return ++g->epoch;
+}
+int ensure_epochs(Graph* g, uint32_t needed);
+
+}  // namespace dc

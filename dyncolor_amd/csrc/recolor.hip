@@ -1,0 +1,1000 @@
+// recolor.hip -- the recolor rounds of one update batch, GPU static coloring
+// and validation.
+//
+// Semantics: each reference kernel launch is one round; every read in a
+// round sees the colors of the round start, every write commits at round end.
+// Colors live in two buffers: a round reads cur[] and writes nxt[]; the host
+// swaps them between rounds, and vertices changed by the previous round are
+// copied forward (cur -> nxt) by the next round's first kernel, which visits
+// exactly those vertices.  Every write to vertex t is a function of (t, the
+// snapshot), so results do not depend on thread order.
+//
+// Frontiers are sets of vertex IDs split by row length into three classes
+// (8-lane groups / one wave / whole workgroups per 4096-slot chunk); a vertex
+// joins a frontier at most once per round through an epoch-stamped claim and a
+// wave64 ballot/mbcnt append (the role of the reference's cuCompactor,
+// HI/compactor.cuh:79-132, without the O(n) flag scan).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+
+#include "dc_device.h"
+#include "dc_internal.h"
+
+namespace dc {
+
+enum Mode {
+  M_DEL = 0,       // deleteEdge: mex of all live nbrs, write if changed (HI/gpuFunctions_undir.cuh:147-177)
+  M_INS = 1,       // insEdge: unconditional (HI/gpuFunctions_undir.cuh:182-223)
+  M_LDF_P = 2,     // processChangeEdge: predecessor mex, write if changed (HB/gpuFunctions_undir.cuh:222-249)
+  M_LOOP_HI = 3,   // recolorNeighbor, unconditional (HI/gpuFunctions_undir.cuh:264-284)
+  M_LOOP_LDF = 4,  // recolorNeighbor, predecessor mex if changed (HB/gpuFunctions_undir.cuh:295-314)
+  M_STATIC_ID = 5, // greedy in ID order (dependency-counting schedule)
+  M_STATIC_LDF = 6 // greedy in LDF priority order
+};
+
+template <int MODE>
+struct ModeTraits {
+  static constexpr bool prop = MODE == M_LDF_P || MODE == M_LOOP_LDF || MODE == M_STATIC_LDF;
+  static constexpr bool stat = MODE == M_STATIC_ID || MODE == M_STATIC_LDF;
+  static constexpr bool marks = MODE == M_DEL || MODE == M_INS || MODE == M_LDF_P;
+};
+
+struct RecolorArgs {
+  GraphView G;
+  const int32_t* cur;
+  int32_t* nxt;
+  int32_t* prev;
+  Frontier in;        // vertices to evaluate
+  Frontier out;       // marked set (pre-loop rounds) or next ready set (static)
+  uint32_t* mmark;
+  uint32_t mepoch;
+  int32_t* pending;
+  unsigned long long* hubmask;
+  uint32_t* hubdone;
+  uint32_t hub_cap;
+  Counters* ctr;
+};
+
+// Is neighbour w (priority pw) part of v's mex set under MODE?
+template <int MODE>
+__device__ __forceinline__ bool include_nbr(int32_t v, int32_t pv, int32_t w, int32_t pw,
+                                            int32_t n_owned) {
+  if constexpr (MODE == M_STATIC_ID) return w >= n_owned || w < v;
+  else if constexpr (MODE == M_STATIC_LDF) return w >= n_owned || precedes(nullptr, w, pw, v, pv);
+  else if constexpr (ModeTraits<MODE>::prop) return precedes(nullptr, w, pw, v, pv);
+  else return true;
+}
+
+// Commit rule of each mode for vertex v with snapshot color cv and mex m.
+// Returns whether v joins the output marked set.
+template <int MODE>
+__device__ __forceinline__ bool finish_vertex(const RecolorArgs& A, int32_t v, int32_t cv,
+                                              int32_t m, uint32_t& changed) {
+  bool ch = m != cv;
+  changed += ch ? 1u : 0u;
+  if constexpr (MODE == M_DEL) {
+    if (ch) {
+      A.nxt[v] = m;
+      A.prev[v] = cv;
+      return claim(A.mmark, v, A.mepoch);
+    }
+    return false;
+  } else if constexpr (MODE == M_INS) {
+    A.nxt[v] = m;
+    A.prev[v] = cv;
+    return claim(A.mmark, v, A.mepoch);
+  } else if constexpr (MODE == M_LDF_P) {
+    if (ch) {
+      A.nxt[v] = m;
+      return claim(A.mmark, v, A.mepoch);
+    }
+    return false;
+  } else if constexpr (MODE == M_LOOP_HI) {
+    A.nxt[v] = m;
+    A.prev[v] = cv;
+    return false;
+  } else if constexpr (MODE == M_LOOP_LDF) {
+    if (ch) A.nxt[v] = m;
+    return false;
+  } else {
+    A.nxt[v] = m;  // static: nxt aliases cur, predecessors are final
+    return false;
+  }
+}
+
+// Static modes: a neighbour that v precedes loses one pending predecessor;
+// the last one to drop it to zero schedules it.
+template <int MODE>
+__device__ __forceinline__ bool release_nbr(const RecolorArgs& A, int32_t v, int32_t pv, int32_t w,
+                                            int32_t pw) {
+  if constexpr (ModeTraits<MODE>::stat) {
+    if (w < A.G.n_owned && !include_nbr<MODE>(v, pv, w, pw, A.G.n_owned))
+      return atomicSub(&A.pending[w], 1) == 1;
+  }
+  return false;
+}
+
+// ----------------------------------------------------------------------------
+// recolor kernel: evaluate every vertex of A.in
+template <int MODE>
+__global__ __launch_bounds__(kBlock) void k_recolor(RecolorArgs A) {
+  __shared__ unsigned long long win[kBlock / 64][kWinWords];
+  __shared__ unsigned long long bwin[kWinWords];
+  __shared__ int sh_last;
+  const int lane = lane_id();
+  const int wib = threadIdx.x >> 6;
+  const int64_t gwave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  const GraphView& G = A.G;
+  const uint32_t nS = A.in.cnt[0], nM = A.in.cnt[1], nH = A.in.cnt[2];
+  uint32_t changed = 0;
+  unsigned long long slots = 0, live = 0;
+
+  // ---- class H: one workgroup per 4096-slot chunk, partial masks merged in a
+  //      per-hub global window; the last chunk to finish computes the mex.
+  for (uint32_t ci = blockIdx.x; ci < nH; ci += gridDim.x) {
+    int4 it = A.in.h[ci];
+    const int32_t v = it.x;
+    const uint32_t c = (uint32_t)it.y, slot = (uint32_t)it.z, nch = (uint32_t)it.w;
+    const uint2 r = G.row[v];
+    const int32_t pv = ModeTraits<MODE>::prop ? G.prop[v] : 0;
+    for (int i = threadIdx.x; i < kWinWords; i += kBlock) bwin[i] = 0ull;
+    __syncthreads();
+    const uint32_t lo = c * kChunk;
+    const uint32_t hi = min(r.y, lo + kChunk);
+    for (uint32_t j = lo + threadIdx.x; j < hi; j += kBlock) {
+      int32_t w = G.col[(uint64_t)r.x + j];
+      ++slots;
+      if (w < 0) continue;
+      ++live;
+      int32_t pw = ModeTraits<MODE>::prop ? G.prop[w] : 0;
+      if (include_nbr<MODE>(v, pv, w, pw, G.n_owned)) {
+        int32_t cw = A.cur[w];
+        if (cw >= 0 && cw < kWinBits) atomicOr(&bwin[cw >> 6], 1ull << (cw & 63));
+      }
+      if constexpr (ModeTraits<MODE>::stat) {
+        bool ready = release_nbr<MODE>(A, v, pv, w, pw);
+        if (ready) {
+          // scheduled from inside a block loop: append lane by lane
+          frontier_push(true, w, G, A.out, A.hub_cap);
+        }
+      }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kWinWords; i += kBlock) {
+      unsigned long long x = bwin[i];
+      if (x) atomicOr(&A.hubmask[(uint64_t)slot * kWinWords + i], x);
+    }
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) sh_last = atomicAdd(&A.hubdone[slot], 1u) == nch - 1;
+    __syncthreads();
+    if (sh_last) {
+      __threadfence();
+      // read-and-clear the merged window
+      for (int i = threadIdx.x; i < kWinWords; i += kBlock)
+        bwin[i] = atomicExch(&A.hubmask[(uint64_t)slot * kWinWords + i], 0ull);
+      __syncthreads();
+      __shared__ int sh_mex;
+      if (threadIdx.x == 0) {
+        int m = -1;
+        for (int i = 0; i < kWinWords; ++i)
+          if (bwin[i] != ~0ull) {
+            m = i * 64 + __ffsll((long long)~bwin[i]) - 1;
+            break;
+          }
+        sh_mex = m;
+        A.hubdone[slot] = 0;
+      }
+      __syncthreads();
+      // window full: rescan the whole row, one 8192-color window at a time
+      int32_t base = kWinBits;
+      while (sh_mex < 0) {
+        if (threadIdx.x == 0) A.ctr->window_full = 1;
+        for (int i = threadIdx.x; i < kWinWords; i += kBlock) bwin[i] = 0ull;
+        __syncthreads();
+        for (uint32_t j = threadIdx.x; j < r.y; j += kBlock) {
+          int32_t w = G.col[(uint64_t)r.x + j];
+          if (w < 0) continue;
+          int32_t pw = ModeTraits<MODE>::prop ? G.prop[w] : 0;
+          if (!include_nbr<MODE>(v, pv, w, pw, G.n_owned)) continue;
+          int32_t cw = A.cur[w] - base;
+          if (cw >= 0 && cw < kWinBits) atomicOr(&bwin[cw >> 6], 1ull << (cw & 63));
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+          for (int i = 0; i < kWinWords; ++i)
+            if (bwin[i] != ~0ull) {
+              sh_mex = base + i * 64 + __ffsll((long long)~bwin[i]) - 1;
+              break;
+            }
+        }
+        __syncthreads();
+        base += kWinBits;
+      }
+      bool push = false;
+      if (threadIdx.x == 0) push = finish_vertex<MODE>(A, v, A.cur[v], sh_mex, changed);
+      if (wib == 0) frontier_push(push, v, G, A.out, A.hub_cap);
+    }
+    __syncthreads();
+  }
+
+  // ---- class M: one wave per vertex, 8192-bit LDS window (mex <= slots <= 4096)
+  unsigned long long* ww = win[wib];
+  for (int64_t i = gwave; i < nM; i += nwaves) {
+    const int32_t v = A.in.m[i];
+    const uint2 r = G.row[v];
+    const int32_t pv = ModeTraits<MODE>::prop ? G.prop[v] : 0;
+    ww[lane] = 0ull;
+    ww[lane + 64] = 0ull;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    for (uint32_t j = lane; j < r.y; j += 64) {
+      int32_t w = G.col[(uint64_t)r.x + j];
+      ++slots;
+      bool ready = false;
+      if (w >= 0) {
+        ++live;
+        int32_t pw = ModeTraits<MODE>::prop ? G.prop[w] : 0;
+        if (include_nbr<MODE>(v, pv, w, pw, G.n_owned)) {
+          int32_t cw = A.cur[w];
+          if (cw >= 0 && cw < kWinBits) atomicOr(&ww[cw >> 6], 1ull << (cw & 63));
+        }
+        ready = release_nbr<MODE>(A, v, pv, w, pw);
+      }
+      if constexpr (ModeTraits<MODE>::stat) frontier_push(ready, w, G, A.out, A.hub_cap);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    unsigned long long w0 = ww[lane], w1 = ww[lane + 64];
+    unsigned long long b0 = __ballot(w0 != ~0ull);
+    int32_t m;
+    if (b0) {
+      int wi = __ffsll((long long)b0) - 1;
+      unsigned long long word = __shfl(w0, wi);
+      m = wi * 64 + __ffsll((long long)~word) - 1;
+    } else {
+      unsigned long long b1 = __ballot(w1 != ~0ull);
+      int wi = __ffsll((long long)b1) - 1;
+      unsigned long long word = __shfl(w1, wi);
+      m = (64 + wi) * 64 + __ffsll((long long)~word) - 1;
+    }
+    bool push = false;
+    if (lane == 0) push = finish_vertex<MODE>(A, v, A.cur[v], m, changed);
+    frontier_push(push, v, G, A.out, A.hub_cap);
+  }
+
+  // ---- class S: 8-lane groups, one u64 mask (mex <= slots < 64)
+  {
+    const int grp = lane >> 3, sub = lane & 7;
+    for (int64_t b = gwave * 8; b < nS; b += nwaves * 8) {
+      const int64_t idx = b + grp;
+      const bool act = idx < nS;
+      const int32_t v = act ? A.in.s[idx] : 0;
+      const uint2 r = act ? G.row[v] : make_uint2(0u, 0u);
+      const int32_t pv = (act && ModeTraits<MODE>::prop) ? G.prop[v] : 0;
+      unsigned long long mask = 0ull;
+      for (uint32_t j = sub; j < r.y; j += kGroup) {
+        int32_t w = G.col[(uint64_t)r.x + j];
+        ++slots;
+        bool ready = false;
+        if (w >= 0) {
+          ++live;
+          int32_t pw = ModeTraits<MODE>::prop ? G.prop[w] : 0;
+          if (include_nbr<MODE>(v, pv, w, pw, G.n_owned)) {
+            int32_t cw = A.cur[w];
+            if (cw >= 0 && cw < 64) mask |= 1ull << cw;
+          }
+          ready = release_nbr<MODE>(A, v, pv, w, pw);
+        }
+        if constexpr (ModeTraits<MODE>::stat) frontier_push(ready, w, G, A.out, A.hub_cap);
+      }
+      mask |= __shfl_xor(mask, 1);
+      mask |= __shfl_xor(mask, 2);
+      mask |= __shfl_xor(mask, 4);
+      bool push = false;
+      if (act && sub == 0) {
+        int32_t m = __ffsll((long long)~mask) - 1;
+        push = finish_vertex<MODE>(A, v, A.cur[v], m, changed);
+      }
+      frontier_push(push, v, G, A.out, A.hub_cap);
+    }
+  }
+  wave_add_u32(changed, &A.ctr->changed);
+  wave_add_u64(slots, &A.ctr->rslots);
+  wave_add_u64(live, &A.ctr->rlive);
+}
+
+// ----------------------------------------------------------------------------
+// find kernel: for v in L, mark the candidates of the next recolor
+// (findEligibleNeighbors: HI/gpuFunctions_undir.cuh:229-260 for RULE 0,
+// HB/gpuFunctions_undir.cuh:259-291 for RULE 1), and copy v's color forward
+// into the write buffer.
+struct FindArgs {
+  GraphView G;
+  const int32_t* cur;
+  int32_t* nxt;
+  const int32_t* prev;
+  Frontier in;
+  Frontier out;
+  uint32_t* tmark;
+  uint32_t epoch;
+  uint32_t hub_cap;
+  Counters* ctr;
+};
+
+template <int RULE>
+__device__ __forceinline__ void find_visit(const FindArgs& A, int32_t v, int32_t cv, int32_t prv,
+                                           int32_t pv, int32_t w, bool& pw_out, bool& pv_out) {
+  // pw_out: w becomes a candidate; pv_out: v becomes a candidate
+  const GraphView& G = A.G;
+  if constexpr (RULE == 0) {
+    if (w < G.n_owned) {
+      int32_t cw = A.cur[w];
+      pw_out = (cv == cw && v < w) || prv < cw;
+    }
+  } else {
+    int32_t cw = A.cur[w];
+    int32_t pw = G.prop[w];
+    bool vfirst = precedes(nullptr, v, pv, w, pw);
+    pw_out = vfirst;                     // v precedes w: mark w (covers the conflict case too)
+    pv_out = (cv == cw) && !vfirst;      // conflict with a predecessor: mark v
+  }
+}
+
+template <int RULE>
+__global__ __launch_bounds__(kBlock) void k_find(FindArgs A) {
+  const int lane = lane_id();
+  const int64_t gwave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  const GraphView& G = A.G;
+  const uint32_t nS = A.in.cnt[0], nM = A.in.cnt[1], nH = A.in.cnt[2];
+  unsigned long long slots = 0, live = 0;
+
+  // class H chunks: the whole block walks the chunk
+  for (uint32_t ci = blockIdx.x; ci < nH; ci += gridDim.x) {
+    int4 it = A.in.h[ci];
+    const int32_t v = it.x;
+    const uint32_t c = (uint32_t)it.y;
+    const uint2 r = G.row[v];
+    const int32_t cv = A.cur[v];
+    const int32_t prv = RULE == 0 ? A.prev[v] : 0;
+    const int32_t pv = RULE == 1 ? G.prop[v] : 0;
+    if (c == 0 && threadIdx.x == 0) A.nxt[v] = cv;
+    const uint32_t lo = c * kChunk;
+    const uint32_t hi = min(r.y, lo + kChunk);
+    for (uint32_t j0 = lo; j0 < hi; j0 += kBlock) {
+      uint32_t j = j0 + threadIdx.x;
+      bool mw = false, mv = false;
+      int32_t w = -1;
+      if (j < hi) {
+        w = G.col[(uint64_t)r.x + j];
+        ++slots;
+        if (w >= 0) {
+          ++live;
+          find_visit<RULE>(A, v, cv, prv, pv, w, mw, mv);
+        }
+      }
+      bool pushw = mw && claim(A.tmark, w, A.epoch);
+      frontier_push(pushw, w, G, A.out, A.hub_cap);
+      if constexpr (RULE == 1) {
+        bool pushv = mv && claim(A.tmark, v, A.epoch);
+        frontier_push(pushv, v, G, A.out, A.hub_cap);
+      }
+    }
+  }
+
+  // class M: one wave per vertex
+  for (int64_t i = gwave; i < nM; i += nwaves) {
+    const int32_t v = A.in.m[i];
+    const uint2 r = G.row[v];
+    const int32_t cv = A.cur[v];
+    const int32_t prv = RULE == 0 ? A.prev[v] : 0;
+    const int32_t pv = RULE == 1 ? G.prop[v] : 0;
+    if (lane == 0) A.nxt[v] = cv;
+    for (uint32_t j = lane; j < r.y; j += 64) {
+      int32_t w = G.col[(uint64_t)r.x + j];
+      ++slots;
+      bool mw = false, mv = false;
+      if (w >= 0) {
+        ++live;
+        find_visit<RULE>(A, v, cv, prv, pv, w, mw, mv);
+      }
+      bool pushw = mw && claim(A.tmark, w, A.epoch);
+      frontier_push(pushw, w, G, A.out, A.hub_cap);
+      if constexpr (RULE == 1) {
+        bool pushv = mv && claim(A.tmark, v, A.epoch);
+        frontier_push(pushv, v, G, A.out, A.hub_cap);
+      }
+    }
+  }
+
+  // class S: 8-lane groups
+  {
+    const int grp = lane >> 3, sub = lane & 7;
+    for (int64_t b = gwave * 8; b < nS; b += nwaves * 8) {
+      const int64_t idx = b + grp;
+      const bool act = idx < nS;
+      const int32_t v = act ? A.in.s[idx] : 0;
+      const uint2 r = act ? G.row[v] : make_uint2(0u, 0u);
+      const int32_t cv = act ? A.cur[v] : 0;
+      const int32_t prv = (act && RULE == 0) ? A.prev[v] : 0;
+      const int32_t pv = (act && RULE == 1) ? G.prop[v] : 0;
+      if (act && sub == 0) A.nxt[v] = cv;
+      for (uint32_t j = sub; j < r.y; j += kGroup) {
+        int32_t w = G.col[(uint64_t)r.x + j];
+        ++slots;
+        bool mw = false, mv = false;
+        if (w >= 0) {
+          ++live;
+          find_visit<RULE>(A, v, cv, prv, pv, w, mw, mv);
+        }
+        bool pushw = mw && claim(A.tmark, w, A.epoch);
+        frontier_push(pushw, w, G, A.out, A.hub_cap);
+        if constexpr (RULE == 1) {
+          bool pushv = mv && claim(A.tmark, v, A.epoch);
+          frontier_push(pushv, v, G, A.out, A.hub_cap);
+        }
+      }
+    }
+  }
+  wave_add_u64(slots, &A.ctr->slots);
+  wave_add_u64(live, &A.ctr->live);
+}
+
+// ----------------------------------------------------------------------------
+// target kernels of the pre-loop rounds
+
+// deletion round: target = endpoint with the larger snapshot color, tie ->
+// node2 (HI/gpuFunctions_undir.cuh:153); a frozen ghost target is skipped
+// (MG/gpuFunctions_undir.cuh:181).
+__global__ void k_targets_del(const int2* del, const uint32_t* ndel_p, GraphView G, const int32_t* cur,
+                              uint32_t* tmark, uint32_t epoch, Frontier T, uint32_t hub_cap) {
+  const uint32_t nd = *ndel_p;
+  for (uint32_t base = blockIdx.x * blockDim.x; base < nd; base += gridDim.x * blockDim.x) {
+    uint32_t i = base + threadIdx.x;
+    bool p = false;
+    int32_t t = 0;
+    if (i < nd) {
+      int2 e = del[i];
+      t = cur[e.x] > cur[e.y] ? e.x : e.y;
+      p = t < G.n_owned && claim(tmark, t, epoch);
+    }
+    frontier_push(p, t, G, T, hub_cap);
+  }
+}
+
+// LDF change round: both endpoints of every deletion and of every insertion
+// kept by the batch-start color filter (HB/gpuFunctions_undir.cuh:227-247).
+__global__ void k_targets_ldf(const int2* del, const uint32_t* ndel_p, const int2* ins,
+                              const uint32_t* nins_p, GraphView G, uint32_t* tmark, uint32_t epoch,
+                              Frontier T, uint32_t hub_cap) {
+  const uint32_t nd = *ndel_p, ni = *nins_p, tot = nd + ni;
+  for (uint32_t base = blockIdx.x * blockDim.x; base < tot; base += gridDim.x * blockDim.x) {
+    uint32_t i = base + threadIdx.x;
+    int2 e = make_int2(0, 0);
+    bool ok = i < tot;
+    if (ok) e = i < nd ? del[i] : ins[i - nd];
+    bool pa = ok && claim(tmark, e.x, epoch);
+    frontier_push(pa, e.x, G, T, hub_cap);
+    bool pb = ok && claim(tmark, e.y, epoch);
+    frontier_push(pb, e.y, G, T, hub_cap);
+  }
+}
+
+// Number of distinct colors among v's live neighbours (saturation size, the
+// popcount of computeSC's mask, HI/gpuFunctions_undir.cuh:203-207), one wave,
+// windows of 8192 colors in LDS.
+__device__ uint32_t wave_distinct_colors(const GraphView& G, const int32_t* cur, int32_t v,
+                                         unsigned long long* ww) {
+  const int lane = lane_id();
+  const uint2 r = G.row[v];
+  uint32_t total = 0;
+  int32_t rowmax = 0;
+  for (int32_t base = 0; base <= rowmax; base += kWinBits) {
+    ww[lane] = 0ull;
+    ww[lane + 64] = 0ull;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    int32_t lmax = 0;
+    for (uint32_t j = lane; j < r.y; j += 64) {
+      int32_t w = G.col[(uint64_t)r.x + j];
+      if (w < 0) continue;
+      int32_t c = cur[w];
+      lmax = max(lmax, c);
+      int32_t cw = c - base;
+      if (cw >= 0 && cw < kWinBits) atomicOr(&ww[cw >> 6], 1ull << (cw & 63));
+    }
+    for (int off = 32; off > 0; off >>= 1) lmax = max(lmax, __shfl_xor(lmax, off));
+    if (base == 0) rowmax = lmax;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    uint32_t pc = (uint32_t)(__popcll(ww[lane]) + __popcll(ww[lane + 64]));
+    for (int off = 32; off > 0; off >>= 1) pc += __shfl_xor(pc, off);
+    total += pc;
+    __builtin_amdgcn_wave_barrier();
+  }
+  return total;
+}
+
+// insertion round: for an edge whose endpoints share a snapshot color, the
+// endpoint with fewer distinct neighbour colors is the target (tie -> node2,
+// HI/gpuFunctions_undir.cuh:186-209); a ghost endpoint counts as +inf (its
+// owner's size is not available on this GPU, MG/gpuFunctions_undir.cuh:
+// 260-263).  One wave per edge.
+__global__ __launch_bounds__(kBlock) void k_targets_ins(const int2* ins, const uint32_t* nins_p,
+                                                        GraphView G, const int32_t* cur,
+                                                        uint32_t* tmark, uint32_t epoch, Frontier T,
+                                                        uint32_t hub_cap) {
+  __shared__ unsigned long long win[kBlock / 64][kWinWords];
+  const int lane = lane_id();
+  const int64_t gwave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  unsigned long long* ww = win[threadIdx.x >> 6];
+  const uint32_t ni = *nins_p;
+  for (int64_t i = gwave; i < ni; i += nwaves) {
+    int2 e = ins[i];
+    bool push = false;
+    int32_t t = 0;
+    if (cur[e.x] == cur[e.y]) {
+      uint32_t ka = e.x >= G.n_owned ? 0xffffffffu : wave_distinct_colors(G, cur, e.x, ww);
+      uint32_t kb = e.y >= G.n_owned ? 0xffffffffu : wave_distinct_colors(G, cur, e.y, ww);
+      t = ka < kb ? e.x : e.y;
+      if (lane == 0) push = t < G.n_owned && claim(tmark, t, epoch);
+    }
+    frontier_push(push, t, G, T, hub_cap);
+  }
+}
+
+// copy the marked set's colors forward into the write buffer
+__global__ void k_catchup(Frontier F, GraphView G, const int32_t* cur, int32_t* nxt) {
+  const uint32_t nS = F.cnt[0], nM = F.cnt[1], nH = F.cnt[2];
+  const uint32_t tot = nS + nM + nH;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < tot; i += gridDim.x * blockDim.x) {
+    int32_t v;
+    if (i < nS) v = F.s[i];
+    else if (i < nS + nM) v = F.m[i - nS];
+    else v = F.h[i - nS - nM].x;
+    nxt[v] = cur[v];
+  }
+}
+
+// ----------------------------------------------------------------------------
+// static coloring: pending[v] = number of predecessors (neighbours that come
+// first in the greedy order); vertices with none form the first ready set.
+template <int MODE>
+__global__ void k_static_init(GraphView G, int32_t* pending, Frontier R, uint32_t hub_cap) {
+  const int lane = lane_id();
+  const int64_t gwave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  for (int64_t v0 = gwave; v0 < G.n_owned; v0 += nwaves) {
+    const int32_t v = (int32_t)v0;
+    const uint2 r = G.row[v];
+    const int32_t pv = ModeTraits<MODE>::prop ? G.prop[v] : 0;
+    uint32_t cnt = 0;
+    for (uint32_t j = lane; j < r.y; j += 64) {
+      int32_t w = G.col[(uint64_t)r.x + j];
+      if (w < 0 || w >= G.n_owned) continue;
+      int32_t pw = ModeTraits<MODE>::prop ? G.prop[w] : 0;
+      if (include_nbr<MODE>(v, pv, w, pw, G.n_owned)) ++cnt;
+    }
+    for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off);
+    if (lane == 0) pending[v] = (int32_t)cnt;
+    frontier_push(lane == 0 && cnt == 0, v, G, R, hub_cap);
+  }
+}
+
+// ----------------------------------------------------------------------------
+// validate: monochromatic live entries (v, w) with v < w or w a ghost, and
+// the highest owned color (HI/gpuFunctions_undir.cuh:289-305,
+// HI/Color_main.cu:218-226).
+__global__ void k_validate(GraphView G, const int32_t* color, Counters* ctr) {
+  const int lane = lane_id();
+  const int64_t gwave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  unsigned long long bad = 0;
+  int32_t mc = -1;
+  for (int64_t v0 = gwave; v0 < G.n_owned; v0 += nwaves) {
+    const int32_t v = (int32_t)v0;
+    const uint2 r = G.row[v];
+    const int32_t cv = color[v];
+    mc = max(mc, cv);
+    for (uint32_t j = lane; j < r.y; j += 64) {
+      int32_t w = G.col[(uint64_t)r.x + j];
+      if (w < 0) continue;
+      if ((w > v || w >= G.n_owned) && color[w] == cv) ++bad;
+    }
+  }
+  wave_add_u64(bad, &ctr->mono);
+  for (int off = 32; off > 0; off >>= 1) mc = max(mc, __shfl_xor(mc, off));
+  if (lane == 0) atomicMax(&ctr->maxc, mc);
+}
+
+// ============================================================================
+// host side
+static int grid_waves(int64_t waves) {
+  int64_t b = (waves * 64 + kBlock - 1) / kBlock;
+  if (b < 1) b = 1;
+  if (b > 8192) b = 8192;
+  return (int)b;
+}
+static int grid_threads(int64_t t) {
+  int64_t b = (t + kBlock - 1) / kBlock;
+  if (b < 1) b = 1;
+  if (b > 8192) b = 8192;
+  return (int)b;
+}
+
+// Persistent-style grid for frontier kernels: enough workgroups to cover the
+// device; each loop inside reads its own item counts from device memory.
+static int frontier_grid(const Graph* g) {
+  (void)g;
+  return 2048;
+}
+
+int ensure_epochs(Graph* g, uint32_t needed) {
+  if ((uint64_t)g->epoch + needed < 0xfffffff0ull) return DC_OK;
+  DC_HIP(hipMemsetAsync(g->tmark, 0, sizeof(uint32_t) * ((size_t)g->n + 1), g->stream));
+  DC_HIP(hipMemsetAsync(g->mmark, 0, sizeof(uint32_t) * ((size_t)g->n + 1), g->stream));
+  g->epoch = 0;
+  return DC_OK;
+}
+
+static int zero_frontier(Graph* g, Frontier f) {
+  DC_HIP(hipMemsetAsync(f.cnt, 0, sizeof(uint32_t) * 4, g->stream));
+  return DC_OK;
+}
+
+static RecolorArgs make_rargs(Graph* g, Frontier in, Frontier out, uint32_t mepoch) {
+  RecolorArgs A;
+  A.G = g->view();
+  A.cur = g->color[g->cur];
+  A.nxt = g->color[g->cur ^ 1];
+  A.prev = g->prev;
+  A.in = in;
+  A.out = out;
+  A.mmark = g->mmark;
+  A.mepoch = mepoch;
+  A.pending = g->pending;
+  A.hubmask = g->hubmask;
+  A.hubdone = g->hubdone;
+  A.hub_cap = g->hub_cap;
+  A.ctr = g->ctr;
+  return A;
+}
+
+// Profiled launches record a start/stop event pair from a pool; the pairs are
+// read once at the end of the batch so profiling adds no host round trip.
+static int prof_begin(Graph* g) {
+  if (!g->profiling) return DC_OK;
+  while (g->evpool.size() < g->evused + 2) {
+    hipEvent_t e;
+    DC_HIP(hipEventCreate(&e));
+    g->evpool.push_back(e);
+  }
+  DC_HIP(hipEventRecord(g->evpool[g->evused], g->stream));
+  return DC_OK;
+}
+static int prof_end(Graph* g) {
+  if (!g->profiling) return DC_OK;
+  DC_HIP(hipEventRecord(g->evpool[g->evused + 1], g->stream));
+  g->evused += 2;
+  return DC_OK;
+}
+static int prof_collect(Graph* g, double* ms) {
+  double t = 0;
+  if (g->evused) DC_HIP(hipEventSynchronize(g->evpool[g->evused - 1]));
+  for (size_t i = 0; i + 1 < g->evused; i += 2) {
+    float x = 0.f;
+    DC_HIP(hipEventElapsedTime(&x, g->evpool[i], g->evpool[i + 1]));
+    t += x;
+  }
+  g->evused = 0;
+  if (ms) *ms = t;
+  return DC_OK;
+}
+
+static int launch_recolor(Graph* g, int mode, const RecolorArgs& A, float* ms) {
+  (void)ms;
+  dim3 grid(frontier_grid(g)), block(kBlock);
+  int prc = prof_begin(g);
+  if (prc) return prc;
+  switch (mode) {
+    case M_DEL: hipLaunchKernelGGL(k_recolor<M_DEL>, grid, block, 0, g->stream, A); break;
+    case M_INS: hipLaunchKernelGGL(k_recolor<M_INS>, grid, block, 0, g->stream, A); break;
+    case M_LDF_P: hipLaunchKernelGGL(k_recolor<M_LDF_P>, grid, block, 0, g->stream, A); break;
+    case M_LOOP_HI: hipLaunchKernelGGL(k_recolor<M_LOOP_HI>, grid, block, 0, g->stream, A); break;
+    case M_LOOP_LDF: hipLaunchKernelGGL(k_recolor<M_LOOP_LDF>, grid, block, 0, g->stream, A); break;
+    case M_STATIC_ID: hipLaunchKernelGGL(k_recolor<M_STATIC_ID>, grid, block, 0, g->stream, A); break;
+    case M_STATIC_LDF: hipLaunchKernelGGL(k_recolor<M_STATIC_LDF>, grid, block, 0, g->stream, A); break;
+    default: set_error("bad recolor mode"); return DC_ERR_INVALID;
+  }
+  DC_HIP(hipGetLastError());
+  return prof_end(g);
+}
+
+struct RoundReadback {
+  uint32_t cnt[4];
+  Counters c;
+};
+
+static int readback(Graph* g, Frontier f, RoundReadback* rb) {
+  DC_HIP(hipMemcpyAsync(rb->cnt, f.cnt, sizeof(uint32_t) * 4, hipMemcpyDeviceToHost, g->stream));
+  DC_HIP(hipMemcpyAsync(g->hctr, g->ctr, sizeof(Counters), hipMemcpyDeviceToHost, g->stream));
+  DC_HIP(hipStreamSynchronize(g->stream));
+  rb->c = *g->hctr;
+  return DC_OK;
+}
+
+static inline uint64_t fsize(const uint32_t* cnt) { return (uint64_t)cnt[0] + cnt[1] + cnt[3]; }
+
+// The plain per-batch driver: one launch per phase, host reads the frontier
+// size after every round (the structure of HI/Color_main.cu:146-202 and
+// HB/Color_main.cu:160-224, with the O(n) memsets and compactions replaced by
+// epoch claims and wave-aggregated appends).
+int recolor_plain(Graph* g, int rule, dc_stats* st) {
+  if (!g->staged) {
+    set_error("dc_recolor: no staged batch");
+    return DC_ERR_STATE;
+  }
+  const bool partition = g->n_owned < g->n;
+  if (rule != DC_RULE_INDEPENDENT && rule != DC_RULE_LDF) {
+    set_error("bad rule");
+    return DC_ERR_INVALID;
+  }
+  if (partition && rule != DC_RULE_INDEPENDENT) {
+    set_error("partition share supports the independent rule only");
+    return DC_ERR_INVALID;
+  }
+  g->staged = false;
+  g->trace.clear();
+  int rc;
+  // recolor-kernel time comes from the profiling event pairs (prof_collect)
+  int launches = 0;
+  uint64_t evaluated = 0, changed_total = 0, frontier_sum = 0;
+
+  rc = ensure_epochs(g, (uint32_t)std::min<int64_t>((int64_t)g->opt.max_rounds + 16, 0x7fffffff));
+  if (rc) return rc;
+  DC_HIP(hipEventRecord(g->ev[0], g->stream));
+
+  // list sizes staged by the apply
+  RoundReadback rb;
+  DC_HIP(hipMemcpyAsync(g->hctr, g->ctr, sizeof(Counters), hipMemcpyDeviceToHost, g->stream));
+  DC_HIP(hipStreamSynchronize(g->stream));
+  const Counters staged = *g->hctr;
+  uint32_t* dctr_u32 = reinterpret_cast<uint32_t*>(g->ctr);
+  const uint32_t* d_ndel = dctr_u32 + offsetof(Counters, n_del) / 4;
+  const uint32_t* d_nins = dctr_u32 + offsetof(Counters, n_ins) / 4;
+  const uint32_t* d_nxdel = dctr_u32 + offsetof(Counters, n_xdel) / 4;
+  const uint32_t* d_nxins = dctr_u32 + offsetof(Counters, n_xins) / 4;
+  // reset the per-round counters (keep the staged list sizes)
+  auto reset_round_ctr = [&]() -> int {
+    DC_HIP(hipMemsetAsync(&g->ctr->changed, 0, sizeof(uint32_t) * 2, g->stream));
+    DC_HIP(hipMemsetAsync(&g->ctr->slots, 0, sizeof(unsigned long long) * 4, g->stream));
+    return DC_OK;
+  };
+  uint64_t slots_total = 0, live_total = 0;
+
+  // Algorithmic bytes of one recolor launch: per evaluated vertex its list
+  // entry (4), row bounds (8), own color read (4), color write (4) and prev
+  // write or own priority read (4); per row slot the neighbour ID (4); per
+  // live neighbour its color (4) and, for LDF, its priority (4).
+  uint64_t recolor_bytes = 0;
+  auto account_recolor = [&](uint64_t nvert, const Counters& c) {
+    recolor_bytes += 24ull * nvert + 4ull * c.rslots +
+                     (rule == DC_RULE_LDF ? 8ull : 4ull) * c.rlive;
+  };
+
+  Frontier Mk = g->F[2];
+  if ((rc = zero_frontier(g, Mk))) return rc;
+  const uint32_t mepoch = next_epoch(g);
+  bool change = false;
+  const dim3 blk(kBlock);
+
+  auto run_targets_round = [&](int kind, int mode, auto launch_targets) -> int {
+    int r2;
+    if ((r2 = zero_frontier(g, g->T))) return r2;
+    if ((r2 = reset_round_ctr())) return r2;
+    launch_targets(next_epoch(g));
+    DC_HIP(hipGetLastError());
+    RecolorArgs A = make_rargs(g, g->T, Mk, mepoch);
+    if ((r2 = launch_recolor(g, mode, A, nullptr))) return r2;
+    ++launches;
+    if ((r2 = readback(g, g->T, &rb))) return r2;
+    uint64_t nt = fsize(rb.cnt);
+    g->trace.insert(g->trace.end(), {(int64_t)kind, (int64_t)nt, (int64_t)rb.c.changed, 0});
+    evaluated += nt;
+    changed_total += rb.c.changed;
+    slots_total += rb.c.slots + rb.c.rslots;
+    live_total += rb.c.live + rb.c.rlive;
+    account_recolor(nt, rb.c);
+    g->cur ^= 1;
+    hipLaunchKernelGGL(k_catchup, dim3(grid_threads(g->n_owned)), blk, 0, g->stream, Mk, g->view(),
+                       g->color[g->cur], g->color[g->cur ^ 1]);
+    DC_HIP(hipGetLastError());
+    if (mode == M_INS) {
+      if (nt > 0) change = true;
+    } else if (rb.c.changed > 0) {
+      change = true;
+    }
+    return DC_OK;
+  };
+
+  const int fg = frontier_grid(g);
+  if (rule == DC_RULE_INDEPENDENT) {
+    if (staged.n_del > 0) {
+      rc = run_targets_round(1, M_DEL, [&](uint32_t e) {
+        hipLaunchKernelGGL(k_targets_del, dim3(grid_threads(staged.n_del)), blk, 0, g->stream,
+                           g->del, d_ndel, g->view(), g->color[g->cur], g->tmark, e, g->T,
+                           g->hub_cap);
+      });
+      if (rc) return rc;
+    }
+    if (staged.n_xdel > 0) {
+      rc = run_targets_round(4, M_DEL, [&](uint32_t e) {
+        hipLaunchKernelGGL(k_targets_del, dim3(grid_threads(staged.n_xdel)), blk, 0, g->stream,
+                           g->xdel, d_nxdel, g->view(), g->color[g->cur], g->tmark, e, g->T,
+                           g->hub_cap);
+      });
+      if (rc) return rc;
+    }
+    if (staged.n_ins > 0) {
+      rc = run_targets_round(2, M_INS, [&](uint32_t e) {
+        hipLaunchKernelGGL(k_targets_ins, dim3(grid_waves(staged.n_ins)), blk, 0, g->stream,
+                           g->ins, d_nins, g->view(), g->color[g->cur], g->tmark, e, g->T,
+                           g->hub_cap);
+      });
+      if (rc) return rc;
+    }
+    if (staged.n_xins > 0) {
+      rc = run_targets_round(5, M_INS, [&](uint32_t e) {
+        hipLaunchKernelGGL(k_targets_ins, dim3(grid_waves(staged.n_xins)), blk, 0, g->stream,
+                           g->xins, d_nxins, g->view(), g->color[g->cur], g->tmark, e, g->T,
+                           g->hub_cap);
+      });
+      if (rc) return rc;
+    }
+  } else {
+    if (staged.n_del + staged.n_ins > 0) {
+      rc = run_targets_round(3, M_LDF_P, [&](uint32_t e) {
+        hipLaunchKernelGGL(k_targets_ldf, dim3(grid_threads(staged.n_del + staged.n_ins)), blk, 0,
+                           g->stream, g->del, d_ndel, g->ins, d_nins, g->view(), g->tmark, e, g->T,
+                           g->hub_cap);
+      });
+      if (rc) return rc;
+    }
+  }
+
+  // recolor loop: L = marked set, then L <- C every round
+  Frontier L = Mk;
+  DC_HIP(hipMemcpyAsync(rb.cnt, Mk.cnt, sizeof(uint32_t) * 4, hipMemcpyDeviceToHost, g->stream));
+  DC_HIP(hipStreamSynchronize(g->stream));
+  uint64_t nL = fsize(rb.cnt);
+  int cidx = 0;
+  int32_t rounds = 0;
+  while (change) {
+    if (rounds >= g->opt.max_rounds) {
+      set_error("recolor loop hit max_rounds");
+      return DC_ERR_ROUNDS;
+    }
+    Frontier C = g->F[cidx];
+    if ((rc = zero_frontier(g, C))) return rc;
+    if ((rc = reset_round_ctr())) return rc;
+    FindArgs FA{g->view(), g->color[g->cur], g->color[g->cur ^ 1], g->prev, L, C,
+                g->tmark, next_epoch(g), g->hub_cap, g->ctr};
+    if (rule == DC_RULE_INDEPENDENT)
+      hipLaunchKernelGGL(k_find<0>, dim3(fg), blk, 0, g->stream, FA);
+    else
+      hipLaunchKernelGGL(k_find<1>, dim3(fg), blk, 0, g->stream, FA);
+    DC_HIP(hipGetLastError());
+    RecolorArgs A = make_rargs(g, C, Mk, 0);
+    rc = launch_recolor(g, rule == DC_RULE_INDEPENDENT ? M_LOOP_HI : M_LOOP_LDF, A, nullptr);
+    if (rc) return rc;
+    ++launches;
+    if ((rc = readback(g, C, &rb))) return rc;
+    uint64_t nC = fsize(rb.cnt);
+    if (rb.cnt[3] > g->hub_cap) {
+      set_error("hub slot capacity exceeded");
+      return DC_ERR_NOMEM;
+    }
+    g->trace.insert(g->trace.end(), {(int64_t)10, (int64_t)nL, (int64_t)nC, (int64_t)rb.c.changed});
+    evaluated += nC;
+    changed_total += rb.c.changed;
+    frontier_sum += nL;
+    slots_total += rb.c.slots + rb.c.rslots;
+    live_total += rb.c.live + rb.c.rlive;
+    account_recolor(nC, rb.c);
+    g->cur ^= 1;
+    change = rule == DC_RULE_INDEPENDENT ? nC > 0 : rb.c.changed > 0;
+    L = C;
+    nL = nC;
+    cidx ^= 1;
+    ++rounds;
+  }
+  // the write buffer still lags on the last round's changes
+  if (rounds > 0) {
+    hipLaunchKernelGGL(k_catchup, dim3(grid_threads(std::max<uint64_t>(nL, 1))), blk, 0, g->stream,
+                       L, g->view(), g->color[g->cur], g->color[g->cur ^ 1]);
+    DC_HIP(hipGetLastError());
+  }
+  DC_HIP(hipEventRecord(g->ev[1], g->stream));
+  DC_HIP(hipEventSynchronize(g->ev[1]));
+  float ms = 0.f;
+  DC_HIP(hipEventElapsedTime(&ms, g->ev[0], g->ev[1]));
+  double kernel_ms = 0.0;
+  if ((rc = prof_collect(g, &kernel_ms))) return rc;
+  if (st) {
+    st->recolor_bytes = (int64_t)recolor_bytes;
+    st->phases = (int32_t)(g->trace.size() / 4);
+    st->loop_rounds = rounds;
+    st->evaluated = (int64_t)evaluated;
+    st->changed = (int64_t)changed_total;
+    st->frontier_sum = (int64_t)frontier_sum;
+    st->slots_scanned = (int64_t)slots_total;
+    st->live_scanned = (int64_t)live_total;
+    st->ms_recolor = ms;
+    st->ms_recolor_kernel = kernel_ms;
+    st->recolor_launches = launches;
+  }
+  return DC_OK;
+}
+
+// ----------------------------------------------------------------------------
+int static_color(Graph* g, int rule) {
+  if (rule != DC_RULE_INDEPENDENT && rule != DC_RULE_LDF) {
+    set_error("bad rule");
+    return DC_ERR_INVALID;
+  }
+  const int mode = rule == DC_RULE_INDEPENDENT ? M_STATIC_ID : M_STATIC_LDF;
+  const dim3 blk(kBlock);
+  Frontier R = g->F[0], R2 = g->F[1];
+  int rc;
+  if ((rc = zero_frontier(g, R))) return rc;
+  if (mode == M_STATIC_ID)
+    hipLaunchKernelGGL(k_static_init<M_STATIC_ID>, dim3(grid_waves(g->n_owned)), blk, 0, g->stream,
+                       g->view(), g->pending, R, g->hub_cap);
+  else
+    hipLaunchKernelGGL(k_static_init<M_STATIC_LDF>, dim3(grid_waves(g->n_owned)), blk, 0, g->stream,
+                       g->view(), g->pending, R, g->hub_cap);
+  DC_HIP(hipGetLastError());
+  uint32_t cnt[4];
+  DC_HIP(hipMemcpyAsync(cnt, R.cnt, sizeof(cnt), hipMemcpyDeviceToHost, g->stream));
+  DC_HIP(hipStreamSynchronize(g->stream));
+  uint64_t colored = 0;
+  while (fsize(cnt) > 0) {
+    colored += fsize(cnt);
+    if ((rc = zero_frontier(g, R2))) return rc;
+    RecolorArgs A = make_rargs(g, R, R2, 0);
+    A.nxt = g->color[g->cur];
+    if ((rc = launch_recolor(g, mode, A, nullptr))) return rc;
+    DC_HIP(hipMemcpyAsync(cnt, R2.cnt, sizeof(cnt), hipMemcpyDeviceToHost, g->stream));
+    DC_HIP(hipStreamSynchronize(g->stream));
+    std::swap(R, R2);
+  }
+  if (colored != (uint64_t)g->n_owned) {
+    set_error("static coloring did not reach every vertex");
+    return DC_ERR_HIP;
+  }
+  DC_HIP(hipMemcpyAsync(g->color[g->cur ^ 1], g->color[g->cur], sizeof(int32_t) * (size_t)g->n,
+                        hipMemcpyDeviceToDevice, g->stream));
+  DC_HIP(hipStreamSynchronize(g->stream));
+  return DC_OK;
+}
+
+int validate(Graph* g, int64_t* mono, int32_t* maxc) {
+  DC_HIP(hipMemsetAsync(&g->ctr->mono, 0, sizeof(unsigned long long), g->stream));
+  int32_t neg = -1;
+  DC_HIP(hipMemcpyAsync(&g->ctr->maxc, &neg, sizeof(int32_t), hipMemcpyHostToDevice, g->stream));
+  hipLaunchKernelGGL(k_validate, dim3(grid_waves(std::max(g->n_owned, 1))), dim3(kBlock), 0,
+                     g->stream, g->view(), g->color[g->cur], g->ctr);
+  DC_HIP(hipGetLastError());
+  DC_HIP(hipMemcpyAsync(g->hctr, g->ctr, sizeof(Counters), hipMemcpyDeviceToHost, g->stream));
+  DC_HIP(hipStreamSynchronize(g->stream));
+  if (mono) *mono = (int64_t)g->hctr->mono;
+  if (maxc) *maxc = g->hctr->maxc;
+  return DC_OK;
+}
+
+}  // namespace dc

@@ -1,0 +1,202 @@
+/*
+ * dyncolor.h -- C ABI of the MI355X (gfx950) dynamic-recoloring library.
+ *
+ * The reference (ArindamK92/DynColor) has no library API: each variant is one
+ * CUDA executable whose main() reads a graph, a coloring and one change file,
+ * then runs a fixed kernel sequence.  Each entry point below replaces one of
+ * those host steps; the citation names the reference code it stands in for
+ * (paths relative to the reference repo root):
+ *   HI = DynColorHeuristicIndependent/DynColorHeuristicIndependent/
+ *   HB = DynColorHeuristicBased/DynColorHeuristicBased/
+ *   MG = Multi-GPU/
+ *
+ * Conventions: vertex IDs and colors are int32; every function returns DC_OK
+ * (0) or a negative DC_ERR_* code and never aborts or throws; dc_last_error()
+ * gives a message for the last failure on the calling thread.  A dc_graph is
+ * used by one host thread at a time.  Calls return once their results are
+ * valid on the device.
+ */
+#ifndef DYNCOLOR_H
+#define DYNCOLOR_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DC_OK 0
+#define DC_ERR_INVALID (-1)   /* bad argument: null pointer, negative size, bad rule */
+#define DC_ERR_VERTEX (-2)    /* vertex ID out of range, self loop, ghost-ghost edge */
+#define DC_ERR_HIP (-3)       /* HIP runtime failure (no device, launch error)      */
+#define DC_ERR_NOMEM (-4)     /* device or host allocation failed                  */
+#define DC_ERR_ROUNDS (-5)    /* recolor loop hit max_rounds without converging     */
+#define DC_ERR_STATE (-6)     /* call out of order (recolor with no staged batch)   */
+#define DC_ERR_ADMIT (-7)     /* serving admission rejected (memory budget)         */
+
+/* Rule families.  INDEPENDENT: heuristic-independent rules of HI/ (color total
+ * order, vertex-ID tie-break).  LDF: heuristic-based rules of HB/ (largest
+ * degree first, lower ID first on ties). */
+#define DC_RULE_INDEPENDENT 0
+#define DC_RULE_LDF 1
+
+/* Recolor drivers.  PLAIN: one kernel launch per phase, host-driven round loop
+ * (the drop-in for the reference's while(*change) loop).  FUSED: one persistent
+ * cooperative launch runs every round of a batch with device-side barriers. */
+#define DC_DRIVER_PLAIN 0
+#define DC_DRIVER_FUSED 1
+
+typedef struct dc_graph dc_graph;
+typedef struct dc_batch dc_batch;
+
+typedef struct {
+  float slack_frac;    /* spare row capacity as a fraction of degree (default 0.25) */
+  int32_t slack_min;   /* minimum spare slots per row (default 4)                     */
+  int32_t max_rounds;  /* recolor loop cap (default 1000000)                          */
+  int32_t driver;      /* DC_DRIVER_PLAIN (default) or DC_DRIVER_FUSED                */
+  int32_t device;      /* HIP device ordinal (default 0)                              */
+  int32_t reserved[3];
+} dc_options;
+
+/* Per-batch counters.  Byte counts follow the algorithmic model: each visited
+ * row costs its list entry, row bounds, every slot of the row and one color
+ * gather per live entry (plus a priority gather for LDF). */
+typedef struct {
+  int64_t updates;          /* insert + delete lines in the batch             */
+  int64_t insertions, deletions;
+  int32_t phases;           /* entries in the phase trace                     */
+  int32_t loop_rounds;      /* iterations of the recolor loop                 */
+  int64_t evaluated;        /* vertex evaluations (targets + candidates)      */
+  int64_t changed;          /* color changes                                  */
+  int64_t frontier_sum;     /* sum of |L| over loop rounds                    */
+  int64_t slots_scanned;    /* row slots read by find + recolor + targets     */
+  int64_t live_scanned;     /* live entries among them                        */
+  int64_t recolor_bytes;    /* algorithmic bytes of the recolor kernels       */
+  double ms_apply;          /* device time of the batch apply (hipEvent)      */
+  double ms_recolor;        /* device time of rounds + loop (hipEvent)        */
+  double ms_recolor_kernel; /* summed duration of the recolor kernels (profiling on) */
+  int32_t recolor_launches; /* recolor kernel launches in the batch           */
+  int32_t regrows;          /* row-capacity regrows triggered by the batch    */
+  int32_t reserved[6];
+} dc_stats;
+
+/* ---- graph lifetime ------------------------------------------------------ */
+void dc_options_default(dc_options* opt);
+
+/* Build a device-resident undirected graph (both directions stored, spare
+ * capacity per row for insertions) from m edges (u[i], v[i]).  Replaces
+ * read_graphEdges (HI/all_structure_undir.cuh:156-181; the degree counter of
+ * HB/all_structure_undir.cuh:158-186) and the CSR flatten + upload of
+ * transfer_data_to_GPU (HI/supportingFunctions.cu:19-67).  opt may be NULL. */
+int dc_graph_create(int32_t n, int64_t m, const int32_t* u, const int32_t* v,
+                    const dc_options* opt, dc_graph** out);
+
+/* One partition's share: vertices [0, n_owned) are owned, [n_owned,
+ * n_owned + n_ghost) are ghost copies of other partitions' vertices whose colors
+ * (ghost_colors, length n_ghost) stay frozen.  Edges are in these local IDs and
+ * must have at least one owned endpoint.  Replaces the partition ingest of
+ * MG/all_structure_undir.cuh:229-312 and MG/Supporting.h:68-211. */
+int dc_partition_create(int32_t n_owned, int32_t n_ghost, int64_t m,
+                        const int32_t* u, const int32_t* v,
+                        const int32_t* ghost_colors, const dc_options* opt,
+                        dc_graph** out);
+
+int dc_graph_destroy(dc_graph* g);
+
+int dc_graph_info(const dc_graph* g, int32_t* n, int32_t* n_owned,
+                  int64_t* live_entries, int64_t* capacity);
+
+/* ---- colors --------------------------------------------------------------- */
+/* Upload a coloring for all n vertices (ghost entries included).  Replaces
+ * read_Input_Color (HI/all_structure_undir.cuh:130-149). */
+int dc_colors_set(dc_graph* g, const int32_t* colors);
+int dc_colors_get(dc_graph* g, int32_t* colors);
+
+/* Deterministic first-fit greedy coloring on the GPU: vertex-ID order for
+ * INDEPENDENT, priority order for LDF; identical to the sequential greedy.
+ * Stands in for the external static coloring the reference reads from a file
+ * (MG/README.txt:21-23). Ghost colors are left untouched. */
+int dc_static_color(dc_graph* g, int32_t rule);
+
+/* ---- update batches ------------------------------------------------------- */
+/* Apply one batch of k lines (is_insert[i] = 1 insert, 0 delete; order as in a
+ * change file) to the device graph and stage it for dc_recolor.  Replaces
+ * readin_changes (HI/all_structure_undir.cuh:78-123, HB/all_structure_undir.cuh:
+ * 78-129, MG/all_structure_undir.cuh:78-192) and the tombstone launch
+ * deleteEdgeFromAdj (HI/supportingFunctions.cu:69-94).  Zero-insert and
+ * zero-delete batches are legal. */
+int dc_apply_batch(dc_graph* g, int64_t k, const int32_t* u, const int32_t* v,
+                   const uint8_t* is_insert);
+
+/* Device-resident batch: validated and copied to HBM once, applied later
+ * without a host copy (the form the throughput benchmarks use). */
+int dc_batch_create(dc_graph* g, int64_t k, const int32_t* u, const int32_t* v,
+                    const uint8_t* is_insert, dc_batch** out);
+int dc_batch_destroy(dc_batch* b);
+int dc_apply_device_batch(dc_graph* g, const dc_batch* b);
+
+/* Run the staged batch's rounds and the recolor loop.  Replaces the kernel
+ * sequence of HI/Color_main.cu:146-202 (INDEPENDENT), HB/Color_main.cu:160-224
+ * (LDF) and MG/kernel.cu:296-523 (partition share, INDEPENDENT only).
+ * st may be NULL. */
+int dc_recolor(dc_graph* g, int32_t rule, dc_stats* st);
+
+/* apply + recolor in one call */
+int dc_process_batch(dc_graph* g, int32_t rule, int64_t k, const int32_t* u,
+                     const int32_t* v, const uint8_t* is_insert, dc_stats* st);
+
+/* ---- checks ---------------------------------------------------------------- */
+/* Count monochromatic live edges (each undirected edge once) and the highest
+ * owned color.  Replaces the single-thread validate kernel
+ * (HI/gpuFunctions_undir.cuh:289-305) and the host max-color scan
+ * (HI/Color_main.cu:218-226). */
+int dc_validate(dc_graph* g, int64_t* monochromatic, int32_t* max_color);
+
+/* Phase trace of the last dc_recolor: 4 int64 per phase (kind, a, b, c).
+ * kinds: 1 deletion round (targets, changed), 2 insertion round (targets,
+ * changed), 3 LDF change-edge round (endpoints, changed), 4 cross-deletion
+ * round, 5 cross-insertion round, 10 loop round (|L|, |C|, changed). */
+int dc_trace_get(dc_graph* g, int64_t* trace, int32_t cap, int32_t* n_phases);
+
+/* Live neighbours of v (multi-edges repeated), for tests and tools. */
+int dc_neighbors(dc_graph* g, int32_t v, int32_t* out, int64_t cap, int64_t* deg);
+
+/* Current LDF priority counter (one +-1 per change line, like
+ * HB/all_structure_undir.cuh:111-119). */
+int dc_prop_get(dc_graph* g, int32_t* prop);
+
+/* Compact tombstoned row slots (keeps spare capacity). */
+int dc_compact(dc_graph* g);
+
+/* Enable per-kernel hipEvent timing of the recolor kernels (fills
+ * dc_stats.ms_recolor_kernel). */
+int dc_set_profiling(dc_graph* g, int32_t on);
+
+int dc_synchronize(dc_graph* g);
+const char* dc_last_error(void);
+int dc_device_count(int32_t* count);
+
+/* ---- synthetic inputs (host, deterministic, splitmix64-seeded) ------------ */
+/* Erdos-Renyi G(n, m): m distinct undirected edges, no self loops. */
+int dc_gen_er(int32_t n, int64_t m, uint64_t seed, int32_t* u, int32_t* v);
+
+/* R-MAT: 2^scale vertices, edge_factor * 2^scale samples with quadrant
+ * probabilities a, b, c (d = 1-a-b-c); self loops and duplicates removed.
+ * Returns the unique edge count through *m_out; u/v need room for all samples
+ * (pass NULL to only count... not supported: capacity must be >= samples). */
+int dc_gen_rmat(int32_t scale, int32_t edge_factor, double a, double b, double c,
+                uint64_t seed, int32_t* u, int32_t* v, int64_t* m_out);
+
+/* Update-batch stream over a live edge set: nb batches of bs lines each;
+ * round(bs * ins_frac) insertions of uniform non-edges (distinct, no self
+ * loops) and the rest deletions of distinct existing edges, each batch drawn
+ * from the graph as left by the previous ones.  Outputs nb*bs lines. */
+int dc_gen_batches(int32_t n, int64_t m, const int32_t* u, const int32_t* v,
+                   int32_t nb, int64_t bs, double ins_frac, uint64_t seed,
+                   int32_t* bu, int32_t* bv, uint8_t* bins);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
