@@ -41,6 +41,8 @@ def main():
     ap.add_argument("--edge-factor", type=int, default=16)
     ap.add_argument("--batch", type=int, default=100000)
     ap.add_argument("--rule", choices=["independent", "ldf"], default="independent")
+    ap.add_argument("--driver", choices=["plain", "fused"], default="plain")
+    ap.add_argument("--blocks-per-cu", type=int, default=0)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -74,6 +76,8 @@ def main():
         f"colors {maxc0 + 1}, monochromatic {mono0}")
     dev_batches = [g.upload_batch(*b) for b in batches]
     g.set_profiling(True)
+    g.set_driver(d.DRIVER_FUSED if args.driver == "fused" else d.DRIVER_PLAIN,
+                 args.blocks_per_cu)
 
     colors_after_first = None
     for i in range(args.warmup):
@@ -135,7 +139,7 @@ def main():
         "config": {
             "workload": f"rmat{args.scale}_ef{args.edge_factor}_{args.steps}x{args.batch}_mixed50",
             "scale": args.scale, "edge_factor": args.edge_factor, "edges": int(u.size),
-            "batch": args.batch, "rule": args.rule, "driver": "plain",
+            "batch": args.batch, "rule": args.rule, "driver": args.driver,
             "colors": int(maxc + 1), "monochromatic": int(mono),
         },
         "latency_ms": {"p50": float(np.percentile(lat_ms, 50)),
@@ -143,7 +147,7 @@ def main():
                        "max": float(lat_ms.max())},
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_recolor<M_LOOP>",
+            "kernel": "k_recolor<M_LOOP>" if args.driver == "plain" else "k_fused",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",

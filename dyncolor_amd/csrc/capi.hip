@@ -336,7 +336,8 @@ int dc_recolor(dc_graph* h, int32_t rule, dc_stats* st) {
     std::memset(st, 0, sizeof(*st));
     st->updates = g->staged_k;
   }
-  int rc = dc::recolor_plain(g, rule, st);
+  int rc = g->opt.driver == DC_DRIVER_FUSED ? dc::recolor_fused(g, rule, st)
+                                            : dc::recolor_plain(g, rule, st);
   if (st) {
     st->insertions = g->staged_ins;
     st->deletions = g->staged_del;
@@ -413,6 +414,18 @@ int dc_compact(dc_graph* h) {
   }
   DC_HIP(hipSetDevice(g->device));
   return dc::csr_compact(g);
+}
+
+int dc_set_driver(dc_graph* h, int32_t driver, int32_t blocks_per_cu) {
+  if (!h || (driver != DC_DRIVER_PLAIN && driver != DC_DRIVER_FUSED) || blocks_per_cu < 0)
+    return DC_ERR_INVALID;
+  Graph* g = G_(h);
+  g->opt.driver = driver;
+  if (blocks_per_cu > 0 && blocks_per_cu != g->fused_blocks_per_cu) {
+    g->fused_blocks_per_cu = blocks_per_cu;
+    dc::fused_free(g);  // grid size is recomputed on the next fused launch
+  }
+  return DC_OK;
 }
 
 int dc_set_profiling(dc_graph* h, int32_t on) {

@@ -272,6 +272,9 @@ void free_state(Graph* g) {
   auto f = [](void* p) {
     if (p) (void)hipFree(p);
   };
+  fused_free(g);
+  for (hipEvent_t e : g->evpool) (void)hipEventDestroy(e);
+  g->evpool.clear();
   f(g->row); f(g->cap); f(g->col); f(g->prop);
   f(g->color[0]); f(g->color[1]); f(g->prev);
   f(g->tmark); f(g->mmark); f(g->pending);

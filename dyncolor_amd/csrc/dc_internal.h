@@ -135,6 +135,10 @@ struct Graph {
   std::vector<hipEvent_t> evpool;   // start/stop pairs of profiled launches
   size_t evused = 0;
 
+  // persistent driver state (fused.hip)
+  void* fused = nullptr;
+  int fused_blocks_per_cu = 4;
+
   GraphView view() const { return GraphView{n, n_owned, row, cap, col, prop}; }
 };
 
@@ -158,6 +162,9 @@ int exclusive_scan_u32_to_u64(Graph* g, const uint32_t* in, uint64_t* out, int32
 
 // recolor.hip
 int recolor_plain(Graph* g, int rule, dc_stats* st);
+// fused.hip
+int recolor_fused(Graph* g, int rule, dc_stats* st);
+void fused_free(Graph* g);
 int static_color(Graph* g, int rule);
 int validate(Graph* g, int64_t* mono, int32_t* maxc);
 

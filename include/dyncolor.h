@@ -173,6 +173,13 @@ int dc_compact(dc_graph* g);
  * dc_stats.ms_recolor_kernel). */
 int dc_set_profiling(dc_graph* g, int32_t on);
 
+/* Select the recolor driver (DC_DRIVER_PLAIN / DC_DRIVER_FUSED) and, for the
+ * fused driver, the resident workgroups per compute unit (0 = keep).  The
+ * reference has one fixed driver (its while(*change) loop,
+ * HI/Color_main.cu:181-202); the fused driver runs the same rounds inside one
+ * cooperative launch. */
+int dc_set_driver(dc_graph* g, int32_t driver, int32_t blocks_per_cu);
+
 int dc_synchronize(dc_graph* g);
 const char* dc_last_error(void);
 int dc_device_count(int32_t* count);

@@ -19,7 +19,7 @@ def say(*a):
 
 
 def run_both(oracle_mod, n, u, v, batches, rule, colors=None, opts=None, n_owned=None,
-             ghost_colors=None, device_batches=False):
+             ghost_colors=None, device_batches=False, driver=d.DRIVER_PLAIN):
     """Replay batches on GPU and oracle; assert identical colors + traces."""
     u = np.asarray(u, np.int32)
     v = np.asarray(v, np.int32)
@@ -29,6 +29,7 @@ def run_both(oracle_mod, n, u, v, batches, rule, colors=None, opts=None, n_owned
     else:
         g = d.Graph(n_owned, u, v, opts, n_ghost=n - n_owned, ghost_colors=ghost_colors)
         og = oracle_mod.OracleGraph(n, u, v, n_owned=n_owned)
+    g.set_driver(driver)
     if colors is None:
         if n_owned is not None:
             init = np.zeros(n, np.int32)
