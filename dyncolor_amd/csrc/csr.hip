@@ -194,15 +194,16 @@ __global__ void k_hub_capacity(int32_t n, const uint32_t* cap, uint32_t* out) {
   wave_add_u32(chunks, &out[1]);
 }
 
+// (the staged batch's counters in g->ctr must survive this: it runs inside
+// the apply when a row overflows)
 static int ensure_hub_buffers(Graph* g) {
-  DC_HIP(hipMemsetAsync(g->ctr, 0, sizeof(Counters), g->stream));
-  uint32_t* tmp = reinterpret_cast<uint32_t*>(g->ctr);  // borrow 2 words, cleared again below
+  uint32_t* tmp = g->aux;
+  DC_HIP(hipMemsetAsync(tmp, 0, sizeof(uint32_t) * 2, g->stream));
   hipLaunchKernelGGL(k_hub_capacity, dim3(grid_for(g->n)), dim3(kBlock), 0, g->stream, g->n, g->cap,
                      tmp);
   uint32_t h[2];
   DC_HIP(hipMemcpyAsync(h, tmp, sizeof(h), hipMemcpyDeviceToHost, g->stream));
   DC_HIP(hipStreamSynchronize(g->stream));
-  DC_HIP(hipMemsetAsync(g->ctr, 0, sizeof(Counters), g->stream));
   uint32_t need_hubs = h[0] + 1, need_chunks = h[1] + 1;
   if (need_hubs > g->hub_cap) {
     if (g->hubmask) DC_HIP(hipFree(g->hubmask));
@@ -252,6 +253,7 @@ int alloc_state(Graph* g) {
   g->T.m = g->fbuf + n * 7;
   g->T.cnt = g->fcnt + 12;
   DC_HIP(hipMalloc(&g->ctr, sizeof(Counters)));
+  DC_HIP(hipMalloc(&g->aux, sizeof(uint32_t) * 16));
   DC_HIP(hipHostMalloc(&g->hctr, sizeof(Counters), hipHostMallocDefault));
   DC_HIP(hipMemsetAsync(g->row, 0, sizeof(uint2) * n, g->stream));
   DC_HIP(hipMemsetAsync(g->cap, 0, sizeof(uint32_t) * n, g->stream));
@@ -280,7 +282,7 @@ void free_state(Graph* g) {
   f(g->tmark); f(g->mmark); f(g->pending);
   f(g->fbuf); f(g->fcnt); f(g->hbuf); f(g->hubmask); f(g->hubdone);
   f(g->del); f(g->ins); f(g->xdel); f(g->xins); f(g->ovf);
-  f(g->ctr); f(g->scan_tmp);
+  f(g->ctr); f(g->aux); f(g->scan_tmp);
   f(g->scratch_batch.u); f(g->scratch_batch.v); f(g->scratch_batch.ins);
   if (g->hctr) (void)hipHostFree(g->hctr);
   for (int i = 0; i < 4; ++i)

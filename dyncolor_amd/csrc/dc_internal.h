@@ -116,6 +116,7 @@ struct Graph {
 
   Counters* ctr = nullptr;    // device
   Counters* hctr = nullptr;   // pinned host mirror
+  uint32_t* aux = nullptr;    // small device scratch (hub capacity count)
 
   // upload staging for host-array batches
   Batch scratch_batch;
