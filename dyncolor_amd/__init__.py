@@ -36,6 +36,7 @@ EXPORTS = [
     "dc_process_batch", "dc_validate", "dc_trace_get", "dc_neighbors", "dc_prop_get",
     "dc_compact", "dc_set_profiling", "dc_set_driver", "dc_synchronize", "dc_last_error", "dc_device_count",
     "dc_gen_er", "dc_gen_rmat", "dc_gen_batches",
+    "dc_server_create", "dc_server_destroy", "dc_server_run",
 ]
 
 
@@ -56,6 +57,15 @@ class Stats(C.Structure):
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
+class ServeStats(C.Structure):
+    _fields_ = [("steps", C.c_int32), ("reserved0", C.c_int32), ("served", C.c_int64),
+                ("rejected", C.c_int64), ("lines", C.c_int64), ("max_step_lines", C.c_int64),
+                ("elapsed_s", C.c_double), ("busy_s", C.c_double), ("reserved", C.c_int64 * 4)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if not k.startswith("reserved")}
 
 
 class DynColorError(RuntimeError):
@@ -111,6 +121,12 @@ def lib():
                                   C.c_uint64, _i32p, _i32p, C.POINTER(C.c_int64)]),
         "dc_gen_batches": (C.c_int, [C.c_int32, C.c_int64, _vp, _vp, C.c_int32, C.c_int64,
                                      C.c_double, C.c_uint64, _i32p, _i32p, _u8p]),
+        "dc_server_create": (C.c_int, [_vp, C.c_int32, _i32p, C.c_int64, C.c_int32, C.POINTER(_vp)]),
+        "dc_server_destroy": (C.c_int, [_vp]),
+        "dc_server_run": (C.c_int, [_vp, C.c_int64, _i32p, _i64p, _vp, _vp, _vp,
+                                    np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS"),
+                                    np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS"),
+                                    _i32p, C.POINTER(ServeStats)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -312,3 +328,62 @@ class DeviceBatch:
             self.close()
         except Exception:
             pass
+
+
+# ------------------------------------------------------------ many graphs ---
+def union_edges(graphs):
+    """Disjoint union of [(n, u, v), ...]: returns (n_total, u, v, offsets)."""
+    offs = np.zeros(len(graphs) + 1, np.int64)
+    for i, (n, _, _) in enumerate(graphs):
+        offs[i + 1] = offs[i] + n
+    if offs[-1] > 0x7ffffffe:
+        raise ValueError("union exceeds int32 vertex IDs")
+    u = np.concatenate([np.asarray(gu, np.int64) + offs[i] for i, (_, gu, _) in enumerate(graphs)])
+    v = np.concatenate([np.asarray(gv, np.int64) + offs[i] for i, (_, _, gv) in enumerate(graphs)])
+    return int(offs[-1]), u.astype(np.int32), v.astype(np.int32), offs.astype(np.int32)
+
+
+def union_batch(batches, offsets):
+    """Concatenate per-graph batches [(u, v, ins), ...] into union IDs."""
+    u = np.concatenate([np.asarray(b[0], np.int64) + offsets[i] for i, b in enumerate(batches)])
+    v = np.concatenate([np.asarray(b[1], np.int64) + offsets[i] for i, b in enumerate(batches)])
+    ins = np.concatenate([np.asarray(b[2], np.uint8) for b in batches])
+    return u.astype(np.int32), v.astype(np.int32), ins
+
+
+class Server:
+    """Continuous-batching loop over the graphs resident in one arena Graph."""
+
+    def __init__(self, arena: Graph, offsets, budget_lines, rule=RULE_INDEPENDENT):
+        h = C.c_void_p()
+        self._off = _arr(offsets, np.int32)
+        _check(lib().dc_server_create(arena.handle, self._off.size - 1, self._off,
+                                      int(budget_lines), rule, C.byref(h)))
+        self._h = h
+        self.arena = arena
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().dc_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def run(self, gid, roff, u, v, ins, arrival):
+        gid = _arr(gid, np.int32)
+        roff = _arr(roff, np.int64)
+        u = _arr(u, np.int32)
+        v = _arr(v, np.int32)
+        ins = _arr(ins, np.uint8)
+        arrival = _arr(arrival, np.float64)
+        nreq = gid.size
+        lat = np.zeros(max(nreq, 1), np.float64)
+        step = np.zeros(max(nreq, 1), np.int32)
+        st = ServeStats()
+        _check(lib().dc_server_run(self._h, nreq, gid, roff, _ptr(u), _ptr(v), _ptr(ins),
+                                   arrival, lat, step, C.byref(st)))
+        return lat[:nreq], step[:nreq], st

@@ -184,6 +184,36 @@ int dc_synchronize(dc_graph* g);
 const char* dc_last_error(void);
 int dc_device_count(int32_t* count);
 
+/* ---- serving: continuous batching over resident graphs --------------------
+ * No counterpart in the reference (it recolors one batch per process run,
+ * HI/Color_main.cu:31-247).  The resident graphs are one arena dc_graph
+ * holding their disjoint union: graph g owns vertex IDs [offsets[g],
+ * offsets[g+1]).  Requests are replayed against the wall clock: request i
+ * targets graph gid[i], carries lines [roff[i], roff[i+1]) of (u, v,
+ * is_insert) in that graph's local IDs and arrives arrival[i] seconds after
+ * the start (non-decreasing).  Arrived requests are admitted in arrival order
+ * while the lines in flight fit budget_lines (a larger request is rejected,
+ * latency -1); each step coalesces every graph's admitted requests into one
+ * batch per graph and recolors all of them in one arena launch.
+ * latency[i] = completion - arrival (seconds); step_of_req may be NULL. */
+typedef struct dc_server dc_server;
+
+typedef struct {
+  int32_t steps;
+  int32_t reserved0;
+  int64_t served, rejected, lines, max_step_lines;
+  double elapsed_s, busy_s;
+  int64_t reserved[4];
+} dc_serve_stats;
+
+int dc_server_create(dc_graph* arena, int32_t ngraphs, const int32_t* offsets,
+                     int64_t budget_lines, int32_t rule, dc_server** out);
+int dc_server_destroy(dc_server* s);
+int dc_server_run(dc_server* s, int64_t nreq, const int32_t* gid, const int64_t* roff,
+                  const int32_t* u, const int32_t* v, const uint8_t* is_insert,
+                  const double* arrival, double* latency, int32_t* step_of_req,
+                  dc_serve_stats* st);
+
 /* ---- synthetic inputs (host, deterministic, splitmix64-seeded) ------------ */
 /* Erdos-Renyi G(n, m): m distinct undirected edges, no self loops. */
 int dc_gen_er(int32_t n, int64_t m, uint64_t seed, int32_t* u, int32_t* v);

@@ -1,0 +1,111 @@
+"""Many graphs at once: the arena (disjoint union of resident graphs) recolored
+in one launch gives every graph exactly the colors it gets alone, both for a
+batched step (one batch per graph) and under the serving loop, where each
+graph receives the coalesced batches the loop formed for it."""
+import numpy as np
+import pytest
+
+import dyncolor_amd as d
+from test_gpu_parity import say
+
+pytestmark = pytest.mark.gpu
+
+
+def _graphs():
+    out = []
+    for i, s in enumerate((9, 10, 11, 10, 8)):
+        if i % 2 == 0:
+            u, v = d.gen_rmat(s, 8, seed=40 + i)
+            out.append((1 << s, u, v))
+        else:
+            n = 700 * (i + 1)
+            u, v = d.gen_er(n, n * 6, 50 + i)
+            out.append((n, u, v))
+    return out
+
+
+@pytest.mark.parametrize("driver", [d.DRIVER_PLAIN, d.DRIVER_FUSED])
+def test_union_step_equals_alone(oracle_mod, driver):
+    say("union step", driver)
+    graphs = _graphs()
+    n, U, V, off = d.union_edges(graphs)
+    arena = d.Graph(n, U, V)
+    arena.set_driver(driver)
+    arena.static_color(0)
+    per = [d.gen_batches(gn, gu, gv, 2, max(20, gu.size // 15), 0.5, 60 + i)
+           for i, (gn, gu, gv) in enumerate(graphs)]
+    for step in range(2):
+        bu, bv, bi = d.union_batch([p[step] for p in per], off)
+        arena.process_batch(bu, bv, bi, 0)
+    got = arena.colors()
+    assert arena.validate()[0] == 0
+    for i, (gn, gu, gv) in enumerate(graphs):
+        og = oracle_mod.OracleGraph(gn, gu, gv)
+        og.static_color(0)
+        for step in range(2):
+            og.apply_batch(*per[i][step])
+            og.recolor(0)
+        assert (got[off[i]:off[i + 1]] == og.color).all(), f"graph {i}"
+
+
+def test_serving_isolation(oracle_mod):
+    say("serving isolation")
+    graphs = _graphs()
+    n, U, V, off = d.union_edges(graphs)
+    arena = d.Graph(n, U, V)
+    arena.set_driver(d.DRIVER_FUSED)
+    arena.static_color(0)
+    rng = np.random.default_rng(5)
+    nreq = 80
+    gid = rng.integers(0, len(graphs), nreq).astype(np.int32)
+    sizes = np.clip(np.exp(rng.normal(np.log(40), 1.0, nreq)), 1, 400).astype(np.int64)
+    roff = np.zeros(nreq + 1, np.int64)
+    roff[1:] = np.cumsum(sizes)
+    L = int(roff[-1])
+    u = np.zeros(L, np.int32)
+    v = np.zeros(L, np.int32)
+    ins = np.zeros(L, np.uint8)
+    for r in range(nreq):
+        gn, gu, gv = graphs[gid[r]]
+        for j in range(roff[r], roff[r + 1]):
+            if rng.random() < 0.6:
+                a, b = rng.choice(gn, 2, replace=False)
+                u[j], v[j], ins[j] = a, b, 1
+            else:
+                e = rng.integers(0, gu.size)
+                u[j], v[j], ins[j] = gu[e], gv[e], 0
+    arrival = np.cumsum(rng.exponential(1.0 / 2000.0, nreq))
+    srv = d.Server(arena, off, budget_lines=3000)
+    lat, step, st = srv.run(gid, roff, u, v, ins, arrival)
+    assert st.served == nreq and st.rejected == 0
+    assert (lat >= 0).all()
+    assert st.steps >= 2
+    got = arena.colors()
+    assert arena.validate()[0] == 0
+    for g, (gn, gu, gv) in enumerate(graphs):
+        og = oracle_mod.OracleGraph(gn, gu, gv)
+        og.static_color(0)
+        mine = np.nonzero(gid == g)[0]
+        for s_ in sorted(set(step[mine].tolist())):
+            rs = [r for r in mine if step[r] == s_]
+            idx = np.concatenate([np.arange(roff[r], roff[r + 1]) for r in rs])
+            og.apply_batch(u[idx], v[idx], ins[idx])
+            og.recolor(0)
+        assert (got[off[g]:off[g + 1]] == og.color).all(), f"graph {g}"
+
+
+def test_serving_rejects_oversized_request():
+    say("serving admission")
+    graphs = _graphs()[:2]
+    n, U, V, off = d.union_edges(graphs)
+    arena = d.Graph(n, U, V)
+    arena.static_color(0)
+    srv = d.Server(arena, off, budget_lines=10)
+    gid = np.array([0, 1], np.int32)
+    roff = np.array([0, 5, 25], np.int64)
+    u = np.array([0, 1, 2, 3, 4] + list(range(20)), np.int32)
+    v = np.array([5, 6, 7, 8, 9] + list(range(30, 50)), np.int32)
+    ins = np.ones(25, np.uint8)
+    lat, step, st = srv.run(gid, roff, u, v, ins, np.array([0.0, 0.0]))
+    assert st.served == 1 and st.rejected == 1
+    assert lat[1] == -1.0 and lat[0] >= 0
