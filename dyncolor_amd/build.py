@@ -13,6 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libdyncolor.so")
+CLI = os.path.join(HERE, "dyncolor_cli")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 ARCH = "gfx950"
@@ -62,6 +63,14 @@ def build(force: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}{r.stderr}")
     os.replace(LIB + ".tmp", LIB)
+    # reference-format command-line drop-in, linked against the library
+    cli_src = os.path.join(CSRC, "cli", "dyncolor_cli.cpp")
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-o", CLI + ".tmp", cli_src,
+           "-L" + HERE, "-ldyncolor", "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"cli build failed: {' '.join(cmd)}\n{r.stdout}{r.stderr}")
+    os.replace(CLI + ".tmp", CLI)
     return LIB
 
 

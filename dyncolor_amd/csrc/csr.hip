@@ -180,12 +180,12 @@ __global__ void k_fill(int64_t m, const int32_t* u, const int32_t* v, int32_t n_
   }
 }
 
-// count vertices / chunks that can need the hub path: rows with cap > kMidLen
+// count vertices / chunks that can need the hub path: rows with cap >= kMidLen
 __global__ void k_hub_capacity(int32_t n, const uint32_t* cap, uint32_t* out) {
   uint32_t hubs = 0, chunks = 0;
   for (int32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
     uint32_t c = cap[v];
-    if (c > kMidLen) {
+    if (c >= kMidLen) {
       hubs += 1;
       chunks += (c + kChunk - 1) / kChunk;
     }
@@ -205,24 +205,15 @@ static int ensure_hub_buffers(Graph* g) {
   DC_HIP(hipMemcpyAsync(h, tmp, sizeof(h), hipMemcpyDeviceToHost, g->stream));
   DC_HIP(hipStreamSynchronize(g->stream));
   uint32_t need_hubs = h[0] + 1, need_chunks = h[1] + 1;
-  if (need_hubs > g->hub_cap) {
-    if (g->hubmask) DC_HIP(hipFree(g->hubmask));
-    if (g->hubdone) DC_HIP(hipFree(g->hubdone));
-    g->hubmask = nullptr;
-    g->hubdone = nullptr;
-    uint32_t c = need_hubs + need_hubs / 4 + 16;
-    DC_HIP(hipMalloc(&g->hubmask, sizeof(unsigned long long) * kWinWords * (size_t)c));
-    DC_HIP(hipMalloc(&g->hubdone, sizeof(uint32_t) * (size_t)c));
-    DC_HIP(hipMemsetAsync(g->hubmask, 0, sizeof(unsigned long long) * kWinWords * (size_t)c,
-                          g->stream));
-    DC_HIP(hipMemsetAsync(g->hubdone, 0, sizeof(uint32_t) * (size_t)c, g->stream));
-    g->hub_cap = c;
-  }
+  if (need_hubs > g->hub_cap) g->hub_cap = need_hubs + need_hubs / 4 + 16;
   if (need_chunks > g->hchunk_cap) {
     if (g->hbuf) DC_HIP(hipFree(g->hbuf));
+    if (g->partial) DC_HIP(hipFree(g->partial));
     g->hbuf = nullptr;
+    g->partial = nullptr;
     uint32_t c = need_chunks + need_chunks / 4 + 16;
     DC_HIP(hipMalloc(&g->hbuf, sizeof(int4) * (size_t)c * 4));
+    DC_HIP(hipMalloc(&g->partial, sizeof(unsigned long long) * 16 * (size_t)c));
     g->hchunk_cap = c;
     for (int i = 0; i < 3; ++i) g->F[i].h = g->hbuf + (size_t)c * i;
     g->T.h = g->hbuf + (size_t)c * 3;
@@ -280,7 +271,7 @@ void free_state(Graph* g) {
   f(g->row); f(g->cap); f(g->col); f(g->prop);
   f(g->color[0]); f(g->color[1]); f(g->prev);
   f(g->tmark); f(g->mmark); f(g->pending);
-  f(g->fbuf); f(g->fcnt); f(g->hbuf); f(g->hubmask); f(g->hubdone);
+  f(g->fbuf); f(g->fcnt); f(g->hbuf); f(g->partial);
   f(g->del); f(g->ins); f(g->xdel); f(g->xins); f(g->ovf);
   f(g->ctr); f(g->aux); f(g->scan_tmp);
   f(g->scratch_batch.u); f(g->scratch_batch.v); f(g->scratch_batch.ins);

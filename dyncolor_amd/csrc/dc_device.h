@@ -69,8 +69,8 @@ __device__ __forceinline__ void frontier_push(bool p, int32_t v, const GraphView
   uint32_t len = 0;
   if (p) len = G.row[v].y;
   bool is_s = p && len < kSmallLen;
-  bool is_m = p && len >= kSmallLen && len <= kMidLen;
-  bool is_h = p && len > kMidLen;
+  bool is_m = p && len >= kSmallLen && len < kMidLen;
+  bool is_h = p && len >= kMidLen;
   wave_append(is_s, v, F.s, &F.cnt[0]);
   wave_append(is_m, v, F.m, &F.cnt[1]);
   if (is_h) {

@@ -15,14 +15,15 @@ namespace dc {
 
 // Degree classes of a frontier.  A visited row is read by
 //   S: an 8-lane group          (used slots < kSmallLen, mex fits one u64),
-//   M: one wave64               (kSmallLen <= slots <= kMidLen, 8192-bit LDS window),
-//   H: whole workgroups, one per kChunk-slot chunk (slots > kMidLen), partial
+//   M: one wave64               (kSmallLen <= slots < kMidLen, mex < 512),
+//   H: whole workgroups, one per kChunk-slot chunk (slots >= kMidLen), partial
 //      masks merged in a per-hub global window.
+// With 4 loads in flight per lane no lane walks more than 8 slots of a row.
 constexpr int kBlock = 256;
 constexpr int kGroup = 8;
 constexpr uint32_t kSmallLen = 64;
-constexpr uint32_t kMidLen = 4096;
-constexpr uint32_t kChunk = 4096;
+constexpr uint32_t kMidLen = 512;
+constexpr uint32_t kChunk = 1024;
 constexpr int kWinBits = 8192;               // LDS mex window (bits)
 constexpr int kWinWords = kWinBits / 64;     // u64 words per window
 
@@ -100,8 +101,7 @@ struct Graph {
   uint32_t* fcnt = nullptr;   // counters of all frontiers
   uint32_t hub_cap = 0;       // max hub slots per round
   uint32_t hchunk_cap = 0;    // max hub chunks per frontier
-  unsigned long long* hubmask = nullptr;  // hub_cap x kWinWords
-  uint32_t* hubdone = nullptr;            // hub_cap
+  unsigned long long* partial = nullptr;  // per hub chunk: partial mex window (16 u64)
 
   // staged batch lists: int2 pairs
   int2* del = nullptr;

@@ -2,6 +2,12 @@
 // one-launch-per-phase kernels (recolor.hip) and the persistent cooperative
 // kernel (fused.hip).  Each body walks its whole input with grid-stride loops
 // over the launch's workgroups and reads its item counts from device memory.
+//
+// Row visits are split by slot count so that no lane walks more than 8 slots
+// of one row per phase: S rows (< 64 slots) by an 8-lane group, M rows (64..
+// 511) by one wave64, H rows (>= 512) by one workgroup per 1024-slot chunk.
+// Every lane issues 4 independent slot loads (and then 4 independent color
+// gathers) per step to keep enough memory requests in flight.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -28,6 +34,13 @@ struct ModeTraits {
   static constexpr bool marks = MODE == M_DEL || MODE == M_INS || MODE == M_LDF_P;
 };
 
+constexpr int kUnroll = 4;
+
+// A hub chunk's partial window: colors 0..511 of the chunk's included
+// neighbours (8 words) and whether any included colour was >= 512.
+constexpr int kPartWords = 8;
+constexpr int kPartStride = 16;  // u64 per chunk entry (word 8 = overflow flag)
+
 struct RecolorArgs {
   GraphView G;
   const int32_t* cur;
@@ -38,8 +51,7 @@ struct RecolorArgs {
   uint32_t* mmark;
   uint32_t mepoch;
   int32_t* pending;
-  unsigned long long* hubmask;
-  uint32_t* hubdone;
+  unsigned long long* partial;   // kPartStride u64 per hub chunk of `in`
   uint32_t hub_cap;
   Counters* ctr;                 // window_full flag
   uint32_t* changed;             // color changes of this launch
@@ -100,19 +112,105 @@ template <int MODE>
 __device__ __forceinline__ bool release_nbr(const RecolorArgs& A, int32_t v, int32_t pv, int32_t w,
                                             int32_t pw) {
   if constexpr (ModeTraits<MODE>::stat) {
-    if (w < A.G.n_owned && !include_nbr<MODE>(v, pv, w, pw, A.G.n_owned))
+    if (w >= 0 && w < A.G.n_owned && !include_nbr<MODE>(v, pv, w, pw, A.G.n_owned))
       return atomicSub(&A.pending[w], 1) == 1;
   }
   return false;
 }
 
+// Colors 0..255 of a row visit are collected in four per-lane register words
+// (combined across lanes with xor-shuffles); colors in [256, lim) go straight
+// to an LDS window `hiw` indexed by word (c >> 6).
+struct RegMask {
+  unsigned long long m0, m1, m2, m3;
+  bool over;  // a colour >= lim was seen
+  __device__ __forceinline__ void clear() {
+    m0 = m1 = m2 = m3 = 0ull;
+    over = false;
+  }
+  __device__ __forceinline__ void set(int32_t c, unsigned long long* hiw, int32_t lim) {
+    if (c < 0) return;
+    if (c < 256) {
+      const unsigned long long b = 1ull << (c & 63);
+      const int wi = c >> 6;
+      m0 |= wi == 0 ? b : 0ull;
+      m1 |= wi == 1 ? b : 0ull;
+      m2 |= wi == 2 ? b : 0ull;
+      m3 |= wi == 3 ? b : 0ull;
+    } else if (c < lim) {
+      atomicOr(&hiw[c >> 6], 1ull << (c & 63));
+    } else {
+      over = true;
+    }
+  }
+  // OR across aligned groups of `width` lanes (width a power of two <= 64)
+  __device__ __forceinline__ void reduce(int width) {
+    for (int off = width >> 1; off > 0; off >>= 1) {
+      m0 |= __shfl_xor(m0, off);
+      m1 |= __shfl_xor(m1, off);
+      m2 |= __shfl_xor(m2, off);
+      m3 |= __shfl_xor(m3, off);
+    }
+  }
+  // least color < 256 absent, or -1 if 0..255 are all present
+  __device__ __forceinline__ int32_t first_zero() const {
+    if (m0 != ~0ull) return __ffsll((long long)~m0) - 1;
+    if (m1 != ~0ull) return 64 + __ffsll((long long)~m1) - 1;
+    if (m2 != ~0ull) return 128 + __ffsll((long long)~m2) - 1;
+    if (m3 != ~0ull) return 192 + __ffsll((long long)~m3) - 1;
+    return -1;
+  }
+};
+
+// One lane's share of a row visit: slots lo, lo+stride, ... below hi, four
+// at a time.  Collects the mex set into mk; in the static modes also releases
+// successors and schedules the ones that become ready.
+template <int MODE>
+__device__ __forceinline__ void mex_slots(const RecolorArgs& A, int32_t v, int32_t pv, uint64_t rbase,
+                                          uint32_t lo, uint32_t hi, uint32_t stride, RegMask& mk,
+                                          unsigned long long* hiw, int32_t lim,
+                                          unsigned long long& slots, unsigned long long& live) {
+  const GraphView& G = A.G;
+  for (uint32_t j = lo; j < hi; j += kUnroll * stride) {
+    int32_t w[kUnroll], c[kUnroll], p[kUnroll];
+#pragma unroll
+    for (int k = 0; k < kUnroll; ++k) {
+      const uint32_t jj = j + k * stride;
+      w[k] = jj < hi ? G.col[rbase + jj] : -2;
+    }
+#pragma unroll
+    for (int k = 0; k < kUnroll; ++k) {
+      c[k] = -1;
+      p[k] = 0;
+      if (w[k] >= 0) {
+        c[k] = A.cur[w[k]];
+        if constexpr (ModeTraits<MODE>::prop) p[k] = G.prop[w[k]];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < kUnroll; ++k) {
+      slots += w[k] != -2 ? 1ull : 0ull;
+      if (w[k] >= 0) {
+        ++live;
+        if (include_nbr<MODE>(v, pv, w[k], p[k], G.n_owned)) mk.set(c[k], hiw, lim);
+      }
+    }
+    if constexpr (ModeTraits<MODE>::stat) {
+#pragma unroll
+      for (int k = 0; k < kUnroll; ++k) {
+        const bool ready = release_nbr<MODE>(A, v, pv, w[k], p[k]);
+        frontier_push(ready, w[k], G, A.out, A.hub_cap);
+      }
+    }
+  }
+}
+
 // ----------------------------------------------------------------------------
-// recolor kernel: evaluate every vertex of A.in
+// recolor: evaluate every vertex of A.in
 template <int MODE>
 __device__ __forceinline__ void recolor_body(const RecolorArgs& A) {
-  __shared__ unsigned long long win[kBlock / 64][kWinWords];
-  __shared__ unsigned long long bwin[kWinWords];
-  __shared__ int sh_last;
+  __shared__ unsigned long long wwin[kBlock / 64][8];   // M rows: colors 256..511
+  __shared__ unsigned long long bwin[kPartWords + 1];   // H chunk: colors 0..511 + overflow
   const int lane = lane_id();
   const int wib = threadIdx.x >> 6;
   const int64_t gwave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
@@ -122,141 +220,66 @@ __device__ __forceinline__ void recolor_body(const RecolorArgs& A) {
   uint32_t changed = 0;
   unsigned long long slots = 0, live = 0;
 
-  // ---- class H: one workgroup per 4096-slot chunk, partial masks merged in a
-  //      per-hub global window; the last chunk to finish computes the mex.
+  // ---- H rows: one workgroup per kChunk-slot chunk.  The chunk stores its
+  //      partial window; hub_finalize_body (a later launch or phase) merges a
+  //      hub's chunks and takes the mex, so no workgroup waits on another.
   for (uint32_t ci = blockIdx.x; ci < nH; ci += gridDim.x) {
-    int4 it = A.in.h[ci];
+    const int4 it = A.in.h[ci];
     const int32_t v = it.x;
-    const uint32_t c = (uint32_t)it.y, slot = (uint32_t)it.z, nch = (uint32_t)it.w;
+    const uint32_t c = (uint32_t)it.y;
     const uint2 r = G.row[v];
     const int32_t pv = ModeTraits<MODE>::prop ? G.prop[v] : 0;
-    for (int i = threadIdx.x; i < kWinWords; i += kBlock) bwin[i] = 0ull;
+    if (threadIdx.x <= kPartWords) bwin[threadIdx.x] = 0ull;
     __syncthreads();
     const uint32_t lo = c * kChunk;
     const uint32_t hi = min(r.y, lo + kChunk);
-    for (uint32_t j = lo + threadIdx.x; j < hi; j += kBlock) {
-      int32_t w = G.col[(uint64_t)r.x + j];
-      ++slots;
-      if (w < 0) continue;
-      ++live;
-      int32_t pw = ModeTraits<MODE>::prop ? G.prop[w] : 0;
-      if (include_nbr<MODE>(v, pv, w, pw, G.n_owned)) {
-        int32_t cw = A.cur[w];
-        if (cw >= 0 && cw < kWinBits) atomicOr(&bwin[cw >> 6], 1ull << (cw & 63));
-      }
-      if constexpr (ModeTraits<MODE>::stat) {
-        bool ready = release_nbr<MODE>(A, v, pv, w, pw);
-        if (ready) {
-          // scheduled from inside a block loop: append lane by lane
-          frontier_push(true, w, G, A.out, A.hub_cap);
-        }
-      }
+    RegMask mk;
+    mk.clear();
+    // colors >= 512 land in bwin[8] (any bit there = overflow)
+    mex_slots<MODE>(A, v, pv, r.x, lo + threadIdx.x, hi, kBlock, mk, bwin, 512, slots, live);
+    const bool over = __ballot(mk.over) != 0ull;
+    mk.reduce(64);
+    if (lane == 0) {
+      if (mk.m0) atomicOr(&bwin[0], mk.m0);
+      if (mk.m1) atomicOr(&bwin[1], mk.m1);
+      if (mk.m2) atomicOr(&bwin[2], mk.m2);
+      if (mk.m3) atomicOr(&bwin[3], mk.m3);
+      if (over) atomicOr(&bwin[kPartWords], 1ull);
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < kWinWords; i += kBlock) {
-      unsigned long long x = bwin[i];
-      if (x) atomicOr(&A.hubmask[(uint64_t)slot * kWinWords + i], x);
-    }
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) sh_last = atomicAdd(&A.hubdone[slot], 1u) == nch - 1;
-    __syncthreads();
-    if (sh_last) {
-      __threadfence();
-      // read-and-clear the merged window
-      for (int i = threadIdx.x; i < kWinWords; i += kBlock)
-        bwin[i] = atomicExch(&A.hubmask[(uint64_t)slot * kWinWords + i], 0ull);
-      __syncthreads();
-      __shared__ int sh_mex;
-      if (threadIdx.x == 0) {
-        int m = -1;
-        for (int i = 0; i < kWinWords; ++i)
-          if (bwin[i] != ~0ull) {
-            m = i * 64 + __ffsll((long long)~bwin[i]) - 1;
-            break;
-          }
-        sh_mex = m;
-        A.hubdone[slot] = 0;
-      }
-      __syncthreads();
-      // window full: rescan the whole row, one 8192-color window at a time
-      int32_t base = kWinBits;
-      while (sh_mex < 0) {
-        if (threadIdx.x == 0) A.ctr->window_full = 1;
-        for (int i = threadIdx.x; i < kWinWords; i += kBlock) bwin[i] = 0ull;
-        __syncthreads();
-        for (uint32_t j = threadIdx.x; j < r.y; j += kBlock) {
-          int32_t w = G.col[(uint64_t)r.x + j];
-          if (w < 0) continue;
-          int32_t pw = ModeTraits<MODE>::prop ? G.prop[w] : 0;
-          if (!include_nbr<MODE>(v, pv, w, pw, G.n_owned)) continue;
-          int32_t cw = A.cur[w] - base;
-          if (cw >= 0 && cw < kWinBits) atomicOr(&bwin[cw >> 6], 1ull << (cw & 63));
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-          for (int i = 0; i < kWinWords; ++i)
-            if (bwin[i] != ~0ull) {
-              sh_mex = base + i * 64 + __ffsll((long long)~bwin[i]) - 1;
-              break;
-            }
-        }
-        __syncthreads();
-        base += kWinBits;
-      }
-      bool push = false;
-      if (threadIdx.x == 0) push = finish_vertex<MODE>(A, v, A.cur[v], sh_mex, changed);
-      if (wib == 0) frontier_push(push, v, G, A.out, A.hub_cap);
-    }
+    if (threadIdx.x <= kPartWords) A.partial[(uint64_t)ci * kPartStride + threadIdx.x] = bwin[threadIdx.x];
     __syncthreads();
   }
 
-  // ---- class M: one wave per vertex, 8192-bit LDS window (mex <= slots <= 4096)
-  unsigned long long* ww = win[wib];
+  // ---- M rows: one wave per vertex (mex <= slots <= 511)
+  unsigned long long* hw = wwin[wib];
   for (int64_t i = gwave; i < nM; i += nwaves) {
     const int32_t v = A.in.m[i];
     const uint2 r = G.row[v];
     const int32_t pv = ModeTraits<MODE>::prop ? G.prop[v] : 0;
-    ww[lane] = 0ull;
-    ww[lane + 64] = 0ull;
+    if (lane < 8) hw[lane] = 0ull;
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
-    for (uint32_t j = lane; j < r.y; j += 64) {
-      int32_t w = G.col[(uint64_t)r.x + j];
-      ++slots;
-      bool ready = false;
-      if (w >= 0) {
-        ++live;
-        int32_t pw = ModeTraits<MODE>::prop ? G.prop[w] : 0;
-        if (include_nbr<MODE>(v, pv, w, pw, G.n_owned)) {
-          int32_t cw = A.cur[w];
-          if (cw >= 0 && cw < kWinBits) atomicOr(&ww[cw >> 6], 1ull << (cw & 63));
-        }
-        ready = release_nbr<MODE>(A, v, pv, w, pw);
-      }
-      if constexpr (ModeTraits<MODE>::stat) frontier_push(ready, w, G, A.out, A.hub_cap);
-    }
+    RegMask mk;
+    mk.clear();
+    mex_slots<MODE>(A, v, pv, r.x, lane, r.y, 64, mk, hw, 512, slots, live);
+    mk.reduce(64);
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
-    unsigned long long w0 = ww[lane], w1 = ww[lane + 64];
-    unsigned long long b0 = __ballot(w0 != ~0ull);
-    int32_t m;
-    if (b0) {
-      int wi = __ffsll((long long)b0) - 1;
-      unsigned long long word = __shfl(w0, wi);
+    int32_t m = mk.first_zero();
+    if (m < 0) {
+      const unsigned long long x = lane >= 4 && lane < 8 ? hw[lane] : 0ull;
+      const unsigned long long b = __ballot(lane >= 4 && lane < 8 && x != ~0ull);
+      const int wi = __ffsll((long long)b) - 1;
+      const unsigned long long word = __shfl(x, wi < 0 ? 0 : wi);
       m = wi * 64 + __ffsll((long long)~word) - 1;
-    } else {
-      unsigned long long b1 = __ballot(w1 != ~0ull);
-      int wi = __ffsll((long long)b1) - 1;
-      unsigned long long word = __shfl(w1, wi);
-      m = (64 + wi) * 64 + __ffsll((long long)~word) - 1;
     }
     bool push = false;
     if (lane == 0) push = finish_vertex<MODE>(A, v, A.cur[v], m, changed);
     frontier_push(push, v, G, A.out, A.hub_cap);
   }
 
-  // ---- class S: 8-lane groups, one u64 mask (mex <= slots < 64)
+  // ---- S rows: 8-lane groups, one u64 mask (mex <= slots < 64)
   {
     const int grp = lane >> 3, sub = lane & 7;
     for (int64_t b = gwave * 8; b < nS; b += nwaves * 8) {
@@ -265,28 +288,16 @@ __device__ __forceinline__ void recolor_body(const RecolorArgs& A) {
       const int32_t v = act ? A.in.s[idx] : 0;
       const uint2 r = act ? G.row[v] : make_uint2(0u, 0u);
       const int32_t pv = (act && ModeTraits<MODE>::prop) ? G.prop[v] : 0;
-      unsigned long long mask = 0ull;
-      for (uint32_t j = sub; j < r.y; j += kGroup) {
-        int32_t w = G.col[(uint64_t)r.x + j];
-        ++slots;
-        bool ready = false;
-        if (w >= 0) {
-          ++live;
-          int32_t pw = ModeTraits<MODE>::prop ? G.prop[w] : 0;
-          if (include_nbr<MODE>(v, pv, w, pw, G.n_owned)) {
-            int32_t cw = A.cur[w];
-            if (cw >= 0 && cw < 64) mask |= 1ull << cw;
-          }
-          ready = release_nbr<MODE>(A, v, pv, w, pw);
-        }
-        if constexpr (ModeTraits<MODE>::stat) frontier_push(ready, w, G, A.out, A.hub_cap);
-      }
+      RegMask mk;
+      mk.clear();
+      mex_slots<MODE>(A, v, pv, r.x, sub, r.y, kGroup, mk, nullptr, 0, slots, live);
+      unsigned long long mask = mk.m0;
       mask |= __shfl_xor(mask, 1);
       mask |= __shfl_xor(mask, 2);
       mask |= __shfl_xor(mask, 4);
       bool push = false;
       if (act && sub == 0) {
-        int32_t m = __ffsll((long long)~mask) - 1;
+        const int32_t m = __ffsll((long long)~mask) - 1;
         push = finish_vertex<MODE>(A, v, A.cur[v], m, changed);
       }
       frontier_push(push, v, G, A.out, A.hub_cap);
@@ -297,8 +308,90 @@ __device__ __forceinline__ void recolor_body(const RecolorArgs& A) {
   wave_add_u64(live, A.rlive);
 }
 
+// Exact mex of v's included neighbours by one wave, scanning the whole row in
+// windows of kWinBits colours (LDS); used when a hub's colours 0..511 are all
+// taken and some neighbour colour is larger.
+template <int MODE>
+__device__ inline int32_t wave_row_mex(const RecolorArgs& A, int32_t v, unsigned long long* ww) {
+  const int lane = lane_id();
+  const GraphView& G = A.G;
+  const uint2 r = G.row[v];
+  const int32_t pv = ModeTraits<MODE>::prop ? G.prop[v] : 0;
+  for (int32_t base = 0;; base += kWinBits) {
+    ww[lane] = 0ull;
+    ww[lane + 64] = 0ull;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    for (uint32_t j = lane; j < r.y; j += 64) {
+      const int32_t w = G.col[(uint64_t)r.x + j];
+      if (w < 0) continue;
+      const int32_t pw = ModeTraits<MODE>::prop ? G.prop[w] : 0;
+      if (!include_nbr<MODE>(v, pv, w, pw, G.n_owned)) continue;
+      const int32_t cw = A.cur[w] - base;
+      if (cw >= 0 && cw < kWinBits) atomicOr(&ww[cw >> 6], 1ull << (cw & 63));
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    const unsigned long long x0 = ww[lane], x1 = ww[lane + 64];
+    const unsigned long long b0 = __ballot(x0 != ~0ull);
+    if (b0) {
+      const int wi = __ffsll((long long)b0) - 1;
+      return base + wi * 64 + __ffsll((long long)~__shfl(x0, wi)) - 1;
+    }
+    const unsigned long long b1 = __ballot(x1 != ~0ull);
+    if (b1) {
+      const int wi = __ffsll((long long)b1) - 1;
+      return base + (64 + wi) * 64 + __ffsll((long long)~__shfl(x1, wi)) - 1;
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+}
+
+// Second half of a recolor launch for H rows: one wave per hub ORs the
+// partial windows of the hub's chunks (stored contiguously in A.in.h) and
+// commits the mex.  Runs after every chunk of the launch finished (next kernel
+// or after a grid barrier), so it needs no cross-workgroup signalling.
+template <int MODE>
+__device__ __forceinline__ void hub_finalize_body(const RecolorArgs& A) {
+  __shared__ unsigned long long fwin[kBlock / 64][kWinWords];
+  const int lane = lane_id();
+  const int64_t gwave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  const uint32_t nH = A.in.cnt[2];
+  uint32_t changed = 0;
+  for (int64_t ci = gwave; ci < nH; ci += nwaves) {
+    const int4 it = A.in.h[ci];
+    if (it.y != 0) continue;  // one wave per hub: its first chunk entry
+    const int32_t v = it.x;
+    const uint32_t nch = (uint32_t)it.w;
+    unsigned long long w[kPartWords], ov = 0ull;
+#pragma unroll
+    for (int i = 0; i < kPartWords; ++i) w[i] = 0ull;
+    for (uint32_t k = lane; k < nch; k += 64) {
+      const unsigned long long* p = A.partial + (uint64_t)(ci + k) * kPartStride;
+#pragma unroll
+      for (int i = 0; i < kPartWords; ++i) w[i] |= p[i];
+      ov |= p[kPartWords];
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+      for (int i = 0; i < kPartWords; ++i) w[i] |= __shfl_xor(w[i], off);
+      ov |= __shfl_xor(ov, off);
+    }
+    int32_t m = -1;
+#pragma unroll
+    for (int i = kPartWords - 1; i >= 0; --i)
+      if (w[i] != ~0ull) m = i * 64 + __ffsll((long long)~w[i]) - 1;
+    if (m < 0) m = ov ? wave_row_mex<MODE>(A, v, fwin[threadIdx.x >> 6]) : 64 * kPartWords;
+    bool push = false;
+    if (lane == 0) push = finish_vertex<MODE>(A, v, A.cur[v], m, changed);
+    frontier_push(push, v, A.G, A.out, A.hub_cap);
+  }
+  wave_add_u32(changed, A.changed);
+}
+
 // ----------------------------------------------------------------------------
-// find kernel: for v in L, mark the candidates of the next recolor
+// find: for v in L, mark the candidates of the next recolor
 // (findEligibleNeighbors: HI/gpuFunctions_undir.cuh:229-260 for RULE 0,
 // HB/gpuFunctions_undir.cuh:259-291 for RULE 1), and copy v's color forward
 // into the write buffer.
@@ -315,22 +408,52 @@ struct FindArgs {
   Counters* ctr;
 };
 
+// One lane's share of a find visit of row v (slots lo, lo+stride, ... < hi).
 template <int RULE>
-__device__ __forceinline__ void find_visit(const FindArgs& A, int32_t v, int32_t cv, int32_t prv,
-                                           int32_t pv, int32_t w, bool& pw_out, bool& pv_out) {
-  // pw_out: w becomes a candidate; pv_out: v becomes a candidate
+__device__ __forceinline__ void find_slots(const FindArgs& A, int32_t v, int32_t cv, int32_t prv,
+                                           int32_t pv, uint64_t rbase, uint32_t lo, uint32_t hi,
+                                           uint32_t stride, unsigned long long& slots,
+                                           unsigned long long& live) {
   const GraphView& G = A.G;
-  if constexpr (RULE == 0) {
-    if (w < G.n_owned) {
-      int32_t cw = A.cur[w];
-      pw_out = (cv == cw && v < w) || prv < cw;
+  for (uint32_t j = lo; j < hi; j += kUnroll * stride) {
+    int32_t w[kUnroll], c[kUnroll], p[kUnroll];
+#pragma unroll
+    for (int k = 0; k < kUnroll; ++k) {
+      const uint32_t jj = j + k * stride;
+      w[k] = jj < hi ? G.col[rbase + jj] : -2;
     }
-  } else {
-    int32_t cw = A.cur[w];
-    int32_t pw = G.prop[w];
-    bool vfirst = precedes(nullptr, v, pv, w, pw);
-    pw_out = vfirst;                     // v precedes w: mark w (covers the conflict case too)
-    pv_out = (cv == cw) && !vfirst;      // conflict with a predecessor: mark v
+#pragma unroll
+    for (int k = 0; k < kUnroll; ++k) {
+      c[k] = 0;
+      p[k] = 0;
+      const bool look = RULE == 0 ? (w[k] >= 0 && w[k] < G.n_owned) : w[k] >= 0;
+      if (look) {
+        c[k] = A.cur[w[k]];
+        if constexpr (RULE == 1) p[k] = G.prop[w[k]];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < kUnroll; ++k) {
+      slots += w[k] != -2 ? 1ull : 0ull;
+      live += w[k] >= 0 ? 1ull : 0ull;
+      bool mw = false, mv = false;
+      if constexpr (RULE == 0) {
+        // conflict with the larger-ID endpoint, or v's previous color below w's
+        if (w[k] >= 0 && w[k] < G.n_owned) mw = (cv == c[k] && v < w[k]) || prv < c[k];
+      } else {
+        if (w[k] >= 0) {
+          const bool vfirst = precedes(nullptr, v, pv, w[k], p[k]);
+          mw = vfirst;                       // v precedes w: mark w (covers the conflict case)
+          mv = (cv == c[k]) && !vfirst;      // conflict with a predecessor: mark v
+        }
+      }
+      const bool pushw = mw && claim(A.tmark, w[k], A.epoch);
+      frontier_push(pushw, w[k], G, A.out, A.hub_cap);
+      if constexpr (RULE == 1) {
+        const bool pushv = mv && claim(A.tmark, v, A.epoch);
+        frontier_push(pushv, v, G, A.out, A.hub_cap);
+      }
+    }
   }
 }
 
@@ -343,9 +466,9 @@ __device__ __forceinline__ void find_body(const FindArgs& A) {
   const uint32_t nS = A.in.cnt[0], nM = A.in.cnt[1], nH = A.in.cnt[2];
   unsigned long long slots = 0, live = 0;
 
-  // class H chunks: the whole block walks the chunk
+  // H chunks: the whole workgroup walks the chunk
   for (uint32_t ci = blockIdx.x; ci < nH; ci += gridDim.x) {
-    int4 it = A.in.h[ci];
+    const int4 it = A.in.h[ci];
     const int32_t v = it.x;
     const uint32_t c = (uint32_t)it.y;
     const uint2 r = G.row[v];
@@ -355,28 +478,10 @@ __device__ __forceinline__ void find_body(const FindArgs& A) {
     if (c == 0 && threadIdx.x == 0) A.nxt[v] = cv;
     const uint32_t lo = c * kChunk;
     const uint32_t hi = min(r.y, lo + kChunk);
-    for (uint32_t j0 = lo; j0 < hi; j0 += kBlock) {
-      uint32_t j = j0 + threadIdx.x;
-      bool mw = false, mv = false;
-      int32_t w = -1;
-      if (j < hi) {
-        w = G.col[(uint64_t)r.x + j];
-        ++slots;
-        if (w >= 0) {
-          ++live;
-          find_visit<RULE>(A, v, cv, prv, pv, w, mw, mv);
-        }
-      }
-      bool pushw = mw && claim(A.tmark, w, A.epoch);
-      frontier_push(pushw, w, G, A.out, A.hub_cap);
-      if constexpr (RULE == 1) {
-        bool pushv = mv && claim(A.tmark, v, A.epoch);
-        frontier_push(pushv, v, G, A.out, A.hub_cap);
-      }
-    }
+    find_slots<RULE>(A, v, cv, prv, pv, r.x, lo + threadIdx.x, hi, kBlock, slots, live);
   }
 
-  // class M: one wave per vertex
+  // M rows: one wave per vertex
   for (int64_t i = gwave; i < nM; i += nwaves) {
     const int32_t v = A.in.m[i];
     const uint2 r = G.row[v];
@@ -384,24 +489,10 @@ __device__ __forceinline__ void find_body(const FindArgs& A) {
     const int32_t prv = RULE == 0 ? A.prev[v] : 0;
     const int32_t pv = RULE == 1 ? G.prop[v] : 0;
     if (lane == 0) A.nxt[v] = cv;
-    for (uint32_t j = lane; j < r.y; j += 64) {
-      int32_t w = G.col[(uint64_t)r.x + j];
-      ++slots;
-      bool mw = false, mv = false;
-      if (w >= 0) {
-        ++live;
-        find_visit<RULE>(A, v, cv, prv, pv, w, mw, mv);
-      }
-      bool pushw = mw && claim(A.tmark, w, A.epoch);
-      frontier_push(pushw, w, G, A.out, A.hub_cap);
-      if constexpr (RULE == 1) {
-        bool pushv = mv && claim(A.tmark, v, A.epoch);
-        frontier_push(pushv, v, G, A.out, A.hub_cap);
-      }
-    }
+    find_slots<RULE>(A, v, cv, prv, pv, r.x, lane, r.y, 64, slots, live);
   }
 
-  // class S: 8-lane groups
+  // S rows: 8-lane groups
   {
     const int grp = lane >> 3, sub = lane & 7;
     for (int64_t b = gwave * 8; b < nS; b += nwaves * 8) {
@@ -413,21 +504,7 @@ __device__ __forceinline__ void find_body(const FindArgs& A) {
       const int32_t prv = (act && RULE == 0) ? A.prev[v] : 0;
       const int32_t pv = (act && RULE == 1) ? G.prop[v] : 0;
       if (act && sub == 0) A.nxt[v] = cv;
-      for (uint32_t j = sub; j < r.y; j += kGroup) {
-        int32_t w = G.col[(uint64_t)r.x + j];
-        ++slots;
-        bool mw = false, mv = false;
-        if (w >= 0) {
-          ++live;
-          find_visit<RULE>(A, v, cv, prv, pv, w, mw, mv);
-        }
-        bool pushw = mw && claim(A.tmark, w, A.epoch);
-        frontier_push(pushw, w, G, A.out, A.hub_cap);
-        if constexpr (RULE == 1) {
-          bool pushv = mv && claim(A.tmark, v, A.epoch);
-          frontier_push(pushv, v, G, A.out, A.hub_cap);
-        }
-      }
+      find_slots<RULE>(A, v, cv, prv, pv, r.x, sub, r.y, kGroup, slots, live);
     }
   }
   wave_add_u64(slots, &A.ctr->slots);
@@ -435,20 +512,21 @@ __device__ __forceinline__ void find_body(const FindArgs& A) {
 }
 
 // ----------------------------------------------------------------------------
-// target kernels of the pre-loop rounds
+// target bodies of the pre-loop rounds
 
 // deletion round: target = endpoint with the larger snapshot color, tie ->
 // node2 (HI/gpuFunctions_undir.cuh:153); a frozen ghost target is skipped
 // (MG/gpuFunctions_undir.cuh:181).
-__device__ __forceinline__ void targets_del_body(const int2* del, const uint32_t* ndel_p, GraphView G, const int32_t* cur,
-                              uint32_t* tmark, uint32_t epoch, Frontier T, uint32_t hub_cap) {
+__device__ __forceinline__ void targets_del_body(const int2* del, const uint32_t* ndel_p, GraphView G,
+                                                 const int32_t* cur, uint32_t* tmark, uint32_t epoch,
+                                                 Frontier T, uint32_t hub_cap) {
   const uint32_t nd = *ndel_p;
   for (uint32_t base = blockIdx.x * blockDim.x; base < nd; base += gridDim.x * blockDim.x) {
-    uint32_t i = base + threadIdx.x;
+    const uint32_t i = base + threadIdx.x;
     bool p = false;
     int32_t t = 0;
     if (i < nd) {
-      int2 e = del[i];
+      const int2 e = del[i];
       t = cur[e.x] > cur[e.y] ? e.x : e.y;
       p = t < G.n_owned && claim(tmark, t, epoch);
     }
@@ -458,18 +536,19 @@ __device__ __forceinline__ void targets_del_body(const int2* del, const uint32_t
 
 // LDF change round: both endpoints of every deletion and of every insertion
 // kept by the batch-start color filter (HB/gpuFunctions_undir.cuh:227-247).
-__device__ __forceinline__ void targets_ldf_body(const int2* del, const uint32_t* ndel_p, const int2* ins,
-                              const uint32_t* nins_p, GraphView G, uint32_t* tmark, uint32_t epoch,
-                              Frontier T, uint32_t hub_cap) {
+__device__ __forceinline__ void targets_ldf_body(const int2* del, const uint32_t* ndel_p,
+                                                 const int2* ins, const uint32_t* nins_p, GraphView G,
+                                                 uint32_t* tmark, uint32_t epoch, Frontier T,
+                                                 uint32_t hub_cap) {
   const uint32_t nd = *ndel_p, ni = *nins_p, tot = nd + ni;
   for (uint32_t base = blockIdx.x * blockDim.x; base < tot; base += gridDim.x * blockDim.x) {
-    uint32_t i = base + threadIdx.x;
+    const uint32_t i = base + threadIdx.x;
     int2 e = make_int2(0, 0);
-    bool ok = i < tot;
+    const bool ok = i < tot;
     if (ok) e = i < nd ? del[i] : ins[i - nd];
-    bool pa = ok && claim(tmark, e.x, epoch);
+    const bool pa = ok && claim(tmark, e.x, epoch);
     frontier_push(pa, e.x, G, T, hub_cap);
-    bool pb = ok && claim(tmark, e.y, epoch);
+    const bool pb = ok && claim(tmark, e.y, epoch);
     frontier_push(pb, e.y, G, T, hub_cap);
   }
 }
@@ -478,7 +557,7 @@ __device__ __forceinline__ void targets_ldf_body(const int2* del, const uint32_t
 // popcount of computeSC's mask, HI/gpuFunctions_undir.cuh:203-207), one wave,
 // windows of 8192 colors in LDS.
 __device__ inline uint32_t wave_distinct_colors(const GraphView& G, const int32_t* cur, int32_t v,
-                                         unsigned long long* ww) {
+                                                unsigned long long* ww) {
   const int lane = lane_id();
   const uint2 r = G.row[v];
   uint32_t total = 0;
@@ -490,11 +569,11 @@ __device__ inline uint32_t wave_distinct_colors(const GraphView& G, const int32_
     __builtin_amdgcn_wave_barrier();
     int32_t lmax = 0;
     for (uint32_t j = lane; j < r.y; j += 64) {
-      int32_t w = G.col[(uint64_t)r.x + j];
+      const int32_t w = G.col[(uint64_t)r.x + j];
       if (w < 0) continue;
-      int32_t c = cur[w];
+      const int32_t c = cur[w];
       lmax = max(lmax, c);
-      int32_t cw = c - base;
+      const int32_t cw = c - base;
       if (cw >= 0 && cw < kWinBits) atomicOr(&ww[cw >> 6], 1ull << (cw & 63));
     }
     for (int off = 32; off > 0; off >>= 1) lmax = max(lmax, __shfl_xor(lmax, off));
@@ -514,10 +593,9 @@ __device__ inline uint32_t wave_distinct_colors(const GraphView& G, const int32_
 // HI/gpuFunctions_undir.cuh:186-209); a ghost endpoint counts as +inf (its
 // owner's size is not available on this GPU, MG/gpuFunctions_undir.cuh:
 // 260-263).  One wave per edge.
-__device__ __forceinline__ void targets_ins_body(const int2* ins, const uint32_t* nins_p,
-                                                        GraphView G, const int32_t* cur,
-                                                        uint32_t* tmark, uint32_t epoch, Frontier T,
-                                                        uint32_t hub_cap) {
+__device__ __forceinline__ void targets_ins_body(const int2* ins, const uint32_t* nins_p, GraphView G,
+                                                 const int32_t* cur, uint32_t* tmark, uint32_t epoch,
+                                                 Frontier T, uint32_t hub_cap) {
   __shared__ unsigned long long win[kBlock / 64][kWinWords];
   const int lane = lane_id();
   const int64_t gwave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
@@ -525,12 +603,12 @@ __device__ __forceinline__ void targets_ins_body(const int2* ins, const uint32_t
   unsigned long long* ww = win[threadIdx.x >> 6];
   const uint32_t ni = *nins_p;
   for (int64_t i = gwave; i < ni; i += nwaves) {
-    int2 e = ins[i];
+    const int2 e = ins[i];
     bool push = false;
     int32_t t = 0;
     if (cur[e.x] == cur[e.y]) {
-      uint32_t ka = e.x >= G.n_owned ? 0xffffffffu : wave_distinct_colors(G, cur, e.x, ww);
-      uint32_t kb = e.y >= G.n_owned ? 0xffffffffu : wave_distinct_colors(G, cur, e.y, ww);
+      const uint32_t ka = e.x >= G.n_owned ? 0xffffffffu : wave_distinct_colors(G, cur, e.x, ww);
+      const uint32_t kb = e.y >= G.n_owned ? 0xffffffffu : wave_distinct_colors(G, cur, e.y, ww);
       t = ka < kb ? e.x : e.y;
       if (lane == 0) push = t < G.n_owned && claim(tmark, t, epoch);
     }
@@ -540,6 +618,7 @@ __device__ __forceinline__ void targets_ins_body(const int2* ins, const uint32_t
 
 // copy the marked set's colors forward into the write buffer
 __device__ __forceinline__ void catchup_body(Frontier F, GraphView G, const int32_t* cur, int32_t* nxt) {
+  (void)G;
   const uint32_t nS = F.cnt[0], nM = F.cnt[1], nH = F.cnt[2];
   const uint32_t tot = nS + nM + nH;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < tot; i += gridDim.x * blockDim.x) {
@@ -550,6 +629,5 @@ __device__ __forceinline__ void catchup_body(Frontier F, GraphView G, const int3
     nxt[v] = cur[v];
   }
 }
-
 
 }  // namespace dc

@@ -54,8 +54,7 @@ struct FusedArgs {
   const int2* xdel;
   const int2* xins;
   Counters* ctr;
-  unsigned long long* hubmask;
-  uint32_t* hubdone;
+  unsigned long long* partial;
   uint32_t hub_cap;
   uint32_t* chg;               // [0..3] pre-loop rounds, [4..5] loop ring
   unsigned long long* acc;     // [0] recolor slots [1] recolor live
@@ -75,6 +74,15 @@ struct FusedScratch {
 };
 
 __device__ __forceinline__ uint32_t fcount(const Frontier& F) { return F.cnt[0] + F.cnt[1] + F.cnt[3]; }
+
+// a recolor round: the row pass, then (after every chunk is done) the hub merge
+template <int MODE>
+__device__ __forceinline__ void recolor_phase(cg::grid_group& grid, const RecolorArgs& R) {
+  recolor_body<MODE>(R);
+  grid.sync();
+  hub_finalize_body<MODE>(R);
+  grid.sync();
+}
 
 template <int RULE>
 __global__ __launch_bounds__(kBlock) void k_fused(FusedArgs A) {
@@ -110,8 +118,7 @@ __global__ __launch_bounds__(kBlock) void k_fused(FusedArgs A) {
     R.mmark = A.mmark;
     R.mepoch = mepoch;
     R.pending = nullptr;
-    R.hubmask = A.hubmask;
-    R.hubdone = A.hubdone;
+    R.partial = A.partial;
     R.hub_cap = A.hub_cap;
     R.ctr = A.ctr;
     R.changed = chg;
@@ -126,8 +133,7 @@ __global__ __launch_bounds__(kBlock) void k_fused(FusedArgs A) {
     if (nd > 0) {
       targets_del_body(A.del, &A.ctr->n_del, G, color[cur], A.tmark, ++epoch, A.T0, A.hub_cap);
       grid.sync();
-      recolor_body<M_DEL>(rargs(A.T0, &A.chg[0]));
-      grid.sync();
+      recolor_phase<M_DEL>(grid, rargs(A.T0, &A.chg[0]));
       uint32_t nt = fcount(A.T0), ch = A.chg[0];
       record(1, nt, ch, 0);
       evaluated += nt;
@@ -139,8 +145,7 @@ __global__ __launch_bounds__(kBlock) void k_fused(FusedArgs A) {
       catchup_body(A.Mk, G, color[cur], color[cur ^ 1]);
       targets_del_body(A.xdel, &A.ctr->n_xdel, G, color[cur], A.tmark, ++epoch, A.T1, A.hub_cap);
       grid.sync();
-      recolor_body<M_DEL>(rargs(A.T1, &A.chg[1]));
-      grid.sync();
+      recolor_phase<M_DEL>(grid, rargs(A.T1, &A.chg[1]));
       uint32_t nt = fcount(A.T1), ch = A.chg[1];
       record(4, nt, ch, 0);
       evaluated += nt;
@@ -152,8 +157,7 @@ __global__ __launch_bounds__(kBlock) void k_fused(FusedArgs A) {
       catchup_body(A.Mk, G, color[cur], color[cur ^ 1]);
       targets_ins_body(A.ins, &A.ctr->n_ins, G, color[cur], A.tmark, ++epoch, A.T2, A.hub_cap);
       grid.sync();
-      recolor_body<M_INS>(rargs(A.T2, &A.chg[2]));
-      grid.sync();
+      recolor_phase<M_INS>(grid, rargs(A.T2, &A.chg[2]));
       uint32_t nt = fcount(A.T2), ch = A.chg[2];
       record(2, nt, ch, 0);
       evaluated += nt;
@@ -165,8 +169,7 @@ __global__ __launch_bounds__(kBlock) void k_fused(FusedArgs A) {
       catchup_body(A.Mk, G, color[cur], color[cur ^ 1]);
       targets_ins_body(A.xins, &A.ctr->n_xins, G, color[cur], A.tmark, ++epoch, A.T3, A.hub_cap);
       grid.sync();
-      recolor_body<M_INS>(rargs(A.T3, &A.chg[3]));
-      grid.sync();
+      recolor_phase<M_INS>(grid, rargs(A.T3, &A.chg[3]));
       uint32_t nt = fcount(A.T3), ch = A.chg[3];
       record(5, nt, ch, 0);
       evaluated += nt;
@@ -179,8 +182,7 @@ __global__ __launch_bounds__(kBlock) void k_fused(FusedArgs A) {
       targets_ldf_body(A.del, &A.ctr->n_del, A.ins, &A.ctr->n_ins, G, A.tmark, ++epoch, A.T0,
                        A.hub_cap);
       grid.sync();
-      recolor_body<M_LDF_P>(rargs(A.T0, &A.chg[0]));
-      grid.sync();
+      recolor_phase<M_LDF_P>(grid, rargs(A.T0, &A.chg[0]));
       uint32_t nt = fcount(A.T0), ch = A.chg[0];
       record(3, nt, ch, 0);
       evaluated += nt;
@@ -221,8 +223,7 @@ __global__ __launch_bounds__(kBlock) void k_fused(FusedArgs A) {
         A.chg[4 + ((r + 1) & 1)] = 0;
       }
     }
-    recolor_body<RULE == 0 ? M_LOOP_HI : M_LOOP_LDF>(rargs(C, &A.chg[4 + (r & 1)]));
-    grid.sync();
+    recolor_phase<RULE == 0 ? M_LOOP_HI : M_LOOP_LDF>(grid, rargs(C, &A.chg[4 + (r & 1)]));
     const uint32_t ch = A.chg[4 + (r & 1)];
     record(10, nL, nC, ch);
     evaluated += nC;
@@ -347,8 +348,7 @@ int recolor_fused(Graph* g, int rule, dc_stats* st) {
   A.xdel = g->xdel;
   A.xins = g->xins;
   A.ctr = g->ctr;
-  A.hubmask = g->hubmask;
-  A.hubdone = g->hubdone;
+  A.partial = g->partial;
   A.hub_cap = g->hub_cap;
   A.chg = s->chg;
   A.acc = s->acc;

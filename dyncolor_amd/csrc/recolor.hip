@@ -28,6 +28,11 @@ __global__ __launch_bounds__(kBlock) void k_recolor(RecolorArgs A) {
   recolor_body<MODE>(A);
 }
 
+template <int MODE>
+__global__ __launch_bounds__(kBlock) void k_hub_finalize(RecolorArgs A) {
+  hub_finalize_body<MODE>(A);
+}
+
 template <int RULE>
 __global__ __launch_bounds__(kBlock) void k_find(FindArgs A) {
   find_body<RULE>(A);
@@ -152,8 +157,7 @@ static RecolorArgs make_rargs(Graph* g, Frontier in, Frontier out, uint32_t mepo
   A.mmark = g->mmark;
   A.mepoch = mepoch;
   A.pending = g->pending;
-  A.hubmask = g->hubmask;
-  A.hubdone = g->hubdone;
+  A.partial = g->partial;
   A.hub_cap = g->hub_cap;
   A.ctr = g->ctr;
   A.changed = &g->ctr->changed;
@@ -193,19 +197,26 @@ static int prof_collect(Graph* g, double* ms) {
   return DC_OK;
 }
 
+// One recolor = the row pass over every class, then the hub merge pass.
+template <int MODE>
+static void launch_pair(Graph* g, dim3 grid, const RecolorArgs& A) {
+  hipLaunchKernelGGL(k_recolor<MODE>, grid, dim3(kBlock), 0, g->stream, A);
+  hipLaunchKernelGGL(k_hub_finalize<MODE>, grid, dim3(kBlock), 0, g->stream, A);
+}
+
 static int launch_recolor(Graph* g, int mode, const RecolorArgs& A, float* ms) {
   (void)ms;
-  dim3 grid(frontier_grid(g)), block(kBlock);
+  dim3 grid(frontier_grid(g));
   int prc = prof_begin(g);
   if (prc) return prc;
   switch (mode) {
-    case M_DEL: hipLaunchKernelGGL(k_recolor<M_DEL>, grid, block, 0, g->stream, A); break;
-    case M_INS: hipLaunchKernelGGL(k_recolor<M_INS>, grid, block, 0, g->stream, A); break;
-    case M_LDF_P: hipLaunchKernelGGL(k_recolor<M_LDF_P>, grid, block, 0, g->stream, A); break;
-    case M_LOOP_HI: hipLaunchKernelGGL(k_recolor<M_LOOP_HI>, grid, block, 0, g->stream, A); break;
-    case M_LOOP_LDF: hipLaunchKernelGGL(k_recolor<M_LOOP_LDF>, grid, block, 0, g->stream, A); break;
-    case M_STATIC_ID: hipLaunchKernelGGL(k_recolor<M_STATIC_ID>, grid, block, 0, g->stream, A); break;
-    case M_STATIC_LDF: hipLaunchKernelGGL(k_recolor<M_STATIC_LDF>, grid, block, 0, g->stream, A); break;
+    case M_DEL: launch_pair<M_DEL>(g, grid, A); break;
+    case M_INS: launch_pair<M_INS>(g, grid, A); break;
+    case M_LDF_P: launch_pair<M_LDF_P>(g, grid, A); break;
+    case M_LOOP_HI: launch_pair<M_LOOP_HI>(g, grid, A); break;
+    case M_LOOP_LDF: launch_pair<M_LOOP_LDF>(g, grid, A); break;
+    case M_STATIC_ID: launch_pair<M_STATIC_ID>(g, grid, A); break;
+    case M_STATIC_LDF: launch_pair<M_STATIC_LDF>(g, grid, A); break;
     default: set_error("bad recolor mode"); return DC_ERR_INVALID;
   }
   DC_HIP(hipGetLastError());
