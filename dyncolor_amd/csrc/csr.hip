@@ -232,6 +232,10 @@ int alloc_state(Graph* g) {
   DC_HIP(hipMalloc(&g->tmark, sizeof(uint32_t) * n));
   DC_HIP(hipMalloc(&g->mmark, sizeof(uint32_t) * n));
   DC_HIP(hipMalloc(&g->pending, sizeof(int32_t) * n));
+  g->cbits_words = n / 32 + 2;
+  DC_HIP(hipMalloc(&g->cbits[0], sizeof(uint32_t) * 2 * g->cbits_words));
+  g->cbits[1] = g->cbits[0] + g->cbits_words;
+  DC_HIP(hipMemsetAsync(g->cbits[0], 0, sizeof(uint32_t) * 2 * g->cbits_words, g->stream));
   // 4 frontiers (F0, F1, F2=marked set, T) x 2 int lists (S, M)
   DC_HIP(hipMalloc(&g->fbuf, sizeof(int32_t) * n * 8));
   DC_HIP(hipMalloc(&g->fcnt, sizeof(uint32_t) * 16));
@@ -270,7 +274,7 @@ void free_state(Graph* g) {
   g->evpool.clear();
   f(g->row); f(g->cap); f(g->col); f(g->prop);
   f(g->color[0]); f(g->color[1]); f(g->prev);
-  f(g->tmark); f(g->mmark); f(g->pending);
+  f(g->tmark); f(g->mmark); f(g->pending); f(g->cbits[0]);
   f(g->fbuf); f(g->fcnt); f(g->hbuf); f(g->partial);
   f(g->del); f(g->ins); f(g->xdel); f(g->xins); f(g->ovf);
   f(g->ctr); f(g->aux); f(g->scan_tmp);

@@ -60,6 +60,16 @@ __device__ __forceinline__ bool claim(uint32_t* mark, int32_t v, uint32_t epoch)
   return atomicExch(&mark[v], epoch) != epoch;
 }
 
+// Claim v in a 1-bit-per-vertex bitmap (small enough to stay in L2); true for
+// exactly one caller until the bit is cleared.  The plain load only skips the
+// atomic when the bit is already visible.
+__device__ __forceinline__ bool claim_bit(uint32_t* bits, int32_t v) {
+  uint32_t* p = bits + (v >> 5);
+  const uint32_t b = 1u << (v & 31);
+  if (__atomic_load_n(p, __ATOMIC_RELAXED) & b) return false;
+  return (atomicOr(p, b) & b) == 0u;
+}
+
 // Append a claimed vertex to the class list its row length selects.  Hub rows
 // get one entry per kChunk-slot chunk and a hub slot for the mex reduction.
 // Must be called by whole waves (divergence allowed) so the class appends can

@@ -92,6 +92,8 @@ struct Graph {
   uint32_t* tmark = nullptr;  // per-round target/candidate dedupe
   uint32_t* mmark = nullptr;  // per-batch marked-set dedupe
   uint32_t epoch = 0;
+  uint32_t* cbits[2] = {nullptr, nullptr};  // loop-round claim bitmaps, 1 bit per vertex
+  size_t cbits_words = 0;                   // words per bitmap (both share one allocation)
 
   // frontiers: F[0], F[1] alternate; M = pre-loop marked set; T = round targets
   Frontier F[3] = {};

@@ -395,6 +395,10 @@ __device__ __forceinline__ void hub_finalize_body(const RecolorArgs& A) {
 // (findEligibleNeighbors: HI/gpuFunctions_undir.cuh:229-260 for RULE 0,
 // HB/gpuFunctions_undir.cuh:259-291 for RULE 1), and copy v's color forward
 // into the write buffer.
+// Candidates are claimed in this round's bitmap; the other bitmap holds the
+// claims of the previous round, i.e. exactly the bits of L's vertices, and each
+// visit of v in L zeroes v's word there, so it is clean again for the next
+// round (every set bit in a zeroed word belongs to L as well).
 struct FindArgs {
   GraphView G;
   const int32_t* cur;
@@ -402,8 +406,8 @@ struct FindArgs {
   const int32_t* prev;
   Frontier in;
   Frontier out;
-  uint32_t* tmark;
-  uint32_t epoch;
+  uint32_t* claim_bits;
+  uint32_t* clear_bits;
   uint32_t hub_cap;
   Counters* ctr;
 };
@@ -447,10 +451,10 @@ __device__ __forceinline__ void find_slots(const FindArgs& A, int32_t v, int32_t
           mv = (cv == c[k]) && !vfirst;      // conflict with a predecessor: mark v
         }
       }
-      const bool pushw = mw && claim(A.tmark, w[k], A.epoch);
+      const bool pushw = mw && claim_bit(A.claim_bits, w[k]);
       frontier_push(pushw, w[k], G, A.out, A.hub_cap);
       if constexpr (RULE == 1) {
-        const bool pushv = mv && claim(A.tmark, v, A.epoch);
+        const bool pushv = mv && claim_bit(A.claim_bits, v);
         frontier_push(pushv, v, G, A.out, A.hub_cap);
       }
     }
@@ -475,7 +479,10 @@ __device__ __forceinline__ void find_body(const FindArgs& A) {
     const int32_t cv = A.cur[v];
     const int32_t prv = RULE == 0 ? A.prev[v] : 0;
     const int32_t pv = RULE == 1 ? G.prop[v] : 0;
-    if (c == 0 && threadIdx.x == 0) A.nxt[v] = cv;
+    if (c == 0 && threadIdx.x == 0) {
+      A.nxt[v] = cv;
+      A.clear_bits[v >> 5] = 0u;
+    }
     const uint32_t lo = c * kChunk;
     const uint32_t hi = min(r.y, lo + kChunk);
     find_slots<RULE>(A, v, cv, prv, pv, r.x, lo + threadIdx.x, hi, kBlock, slots, live);
@@ -488,7 +495,10 @@ __device__ __forceinline__ void find_body(const FindArgs& A) {
     const int32_t cv = A.cur[v];
     const int32_t prv = RULE == 0 ? A.prev[v] : 0;
     const int32_t pv = RULE == 1 ? G.prop[v] : 0;
-    if (lane == 0) A.nxt[v] = cv;
+    if (lane == 0) {
+      A.nxt[v] = cv;
+      A.clear_bits[v >> 5] = 0u;
+    }
     find_slots<RULE>(A, v, cv, prv, pv, r.x, lane, r.y, 64, slots, live);
   }
 
@@ -503,7 +513,10 @@ __device__ __forceinline__ void find_body(const FindArgs& A) {
       const int32_t cv = act ? A.cur[v] : 0;
       const int32_t prv = (act && RULE == 0) ? A.prev[v] : 0;
       const int32_t pv = (act && RULE == 1) ? G.prop[v] : 0;
-      if (act && sub == 0) A.nxt[v] = cv;
+      if (act && sub == 0) {
+        A.nxt[v] = cv;
+        A.clear_bits[v >> 5] = 0u;
+      }
       find_slots<RULE>(A, v, cv, prv, pv, r.x, sub, r.y, kGroup, slots, live);
     }
   }

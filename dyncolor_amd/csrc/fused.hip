@@ -55,6 +55,7 @@ struct FusedArgs {
   const int2* xins;
   Counters* ctr;
   unsigned long long* partial;
+  uint32_t* cbits[2];          // loop-round claim bitmaps (clear at launch)
   uint32_t hub_cap;
   uint32_t* chg;               // [0..3] pre-loop rounds, [4..5] loop ring
   unsigned long long* acc;     // [0] recolor slots [1] recolor live
@@ -204,7 +205,8 @@ __global__ __launch_bounds__(kBlock) void k_fused(FusedArgs A) {
       break;
     }
     Frontier C = ci == 0 ? A.F0 : A.F1;
-    FindArgs FA{G, color[cur], color[cur ^ 1], A.prev, L, C, A.tmark, ++epoch, A.hub_cap, A.ctr};
+    FindArgs FA{G, color[cur], color[cur ^ 1], A.prev, L, C, A.cbits[r & 1], A.cbits[(r & 1) ^ 1],
+                A.hub_cap, A.ctr};
     find_body<RULE>(FA);
     grid.sync();
     const uint32_t nC = fcount(C);
@@ -349,6 +351,9 @@ int recolor_fused(Graph* g, int rule, dc_stats* st) {
   A.xins = g->xins;
   A.ctr = g->ctr;
   A.partial = g->partial;
+  A.cbits[0] = g->cbits[0];
+  A.cbits[1] = g->cbits[1];
+  DC_HIP(hipMemsetAsync(g->cbits[0], 0, sizeof(uint32_t) * 2 * g->cbits_words, g->stream));
   A.hub_cap = g->hub_cap;
   A.chg = s->chg;
   A.acc = s->acc;

@@ -377,6 +377,7 @@ int recolor_plain(Graph* g, int rule, dc_stats* st) {
 
   // recolor loop: L = marked set, then L <- C every round
   Frontier L = Mk;
+  DC_HIP(hipMemsetAsync(g->cbits[0], 0, sizeof(uint32_t) * 2 * g->cbits_words, g->stream));
   DC_HIP(hipMemcpyAsync(rb.cnt, Mk.cnt, sizeof(uint32_t) * 4, hipMemcpyDeviceToHost, g->stream));
   DC_HIP(hipStreamSynchronize(g->stream));
   uint64_t nL = fsize(rb.cnt);
@@ -391,7 +392,7 @@ int recolor_plain(Graph* g, int rule, dc_stats* st) {
     if ((rc = zero_frontier(g, C))) return rc;
     if ((rc = reset_round_ctr())) return rc;
     FindArgs FA{g->view(), g->color[g->cur], g->color[g->cur ^ 1], g->prev, L, C,
-                g->tmark, next_epoch(g), g->hub_cap, g->ctr};
+                g->cbits[rounds & 1], g->cbits[(rounds & 1) ^ 1], g->hub_cap, g->ctr};
     if (rule == DC_RULE_INDEPENDENT)
       hipLaunchKernelGGL(k_find<0>, dim3(fg), blk, 0, g->stream, FA);
     else
