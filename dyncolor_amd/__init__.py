@@ -34,7 +34,7 @@ EXPORTS = [
     "dc_graph_info", "dc_colors_set", "dc_colors_get", "dc_static_color", "dc_apply_batch",
     "dc_batch_create", "dc_batch_destroy", "dc_apply_device_batch", "dc_recolor",
     "dc_process_batch", "dc_validate", "dc_trace_get", "dc_neighbors", "dc_prop_get",
-    "dc_compact", "dc_set_profiling", "dc_set_driver", "dc_synchronize", "dc_last_error", "dc_device_count",
+    "dc_compact", "dc_check_consistency", "dc_set_profiling", "dc_set_driver", "dc_synchronize", "dc_last_error", "dc_device_count",
     "dc_gen_er", "dc_gen_rmat", "dc_gen_batches",
     "dc_server_create", "dc_server_destroy", "dc_server_run",
 ]
@@ -111,6 +111,7 @@ def lib():
         "dc_neighbors": (C.c_int, [_vp, C.c_int32, _i32p, C.c_int64, C.POINTER(C.c_int64)]),
         "dc_prop_get": (C.c_int, [_vp, _i32p]),
         "dc_compact": (C.c_int, [_vp]),
+        "dc_check_consistency": (C.c_int, [_vp, C.POINTER(C.c_int64)]),
         "dc_set_profiling": (C.c_int, [_vp, C.c_int32]),
         "dc_set_driver": (C.c_int, [_vp, C.c_int32, C.c_int32]),
         "dc_synchronize": (C.c_int, [_vp]),
@@ -295,6 +296,11 @@ class Graph:
 
     def compact(self):
         _check(lib().dc_compact(self._h))
+
+    def check_consistency(self) -> int:
+        bad = C.c_int64(0)
+        _check(lib().dc_check_consistency(self._h, C.byref(bad)))
+        return int(bad.value)
 
     def set_driver(self, driver, blocks_per_cu=0):
         _check(lib().dc_set_driver(self._h, driver, blocks_per_cu))

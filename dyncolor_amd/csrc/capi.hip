@@ -160,7 +160,13 @@ int dc_partition_create(int32_t n_owned, int32_t n_ghost, int64_t m, const int32
         return dc::hip_fail(e, "upload ghost colors");
       }
     }
+    rc = dc::rebuild_ccol(g);
     (void)hipStreamSynchronize(g->stream);
+    if (rc) {
+      dc_graph_destroy(*out);
+      *out = nullptr;
+      return rc;
+    }
   }
   return DC_OK;
 }
@@ -199,8 +205,21 @@ int dc_colors_set(dc_graph* h, const int32_t* colors) {
   for (int b = 0; b < 2; ++b)
     DC_HIP(hipMemcpyAsync(g->color[b], colors, sizeof(int32_t) * (size_t)g->n,
                           hipMemcpyHostToDevice, g->stream));
+  int rc = dc::rebuild_ccol(g);
+  if (rc) return rc;
   DC_HIP(hipStreamSynchronize(g->stream));
   return DC_OK;
+}
+
+int dc_check_consistency(dc_graph* h, int64_t* bad_slots) {
+  if (!h || !bad_slots) return DC_ERR_INVALID;
+  Graph* g = G_(h);
+  if (g->staged) {
+    dc::set_error("dc_check_consistency: a batch is staged");
+    return DC_ERR_STATE;
+  }
+  DC_HIP(hipSetDevice(g->device));
+  return dc::check_consistency(g, bad_slots);
 }
 
 int dc_colors_get(dc_graph* h, int32_t* colors) {

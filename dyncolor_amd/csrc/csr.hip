@@ -164,20 +164,75 @@ __global__ void k_set_rows(int32_t n, const uint64_t* start, uint2* row, int res
   }
 }
 
-__global__ void k_fill(int64_t m, const int32_t* u, const int32_t* v, int32_t n_owned, uint2* row,
-                       int32_t* col) {
+// Both directions of an edge are appended by the same thread, so each slot
+// learns its twin (the reverse slot's position) here.
+__global__ void k_fill(int64_t m, const int32_t* u, const int32_t* v, GraphView G) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < m;
        i += (int64_t)gridDim.x * blockDim.x) {
-    int32_t a = u[i], b = v[i];
-    if (a < n_owned) {
-      uint32_t p = atomicAdd(&row[a].y, 1u);
-      col[(uint64_t)row[a].x + p] = b;
+    const int32_t a = u[i], b = v[i];
+    const bool oa = a < G.n_owned, ob = b < G.n_owned;
+    const uint32_t pa = oa ? atomicAdd(&G.row[a].y, 1u) : 0u;
+    const uint32_t pb = ob ? atomicAdd(&G.row[b].y, 1u) : 0u;
+    if (oa) {
+      const uint64_t s = (uint64_t)G.row[a].x + pa;
+      G.col[s] = b;
+      G.twin[s] = ob ? (int32_t)pb : -1;
     }
-    if (b < n_owned) {
-      uint32_t p = atomicAdd(&row[b].y, 1u);
-      col[(uint64_t)row[b].x + p] = a;
+    if (ob) {
+      const uint64_t s = (uint64_t)G.row[b].x + pb;
+      G.col[s] = a;
+      G.twin[s] = oa ? (int32_t)pa : -1;
     }
   }
+}
+
+// ccol[j] <- colour of col[j] (tombstones -1) for every used slot, one wave per row
+__global__ void k_rebuild_ccol(GraphView G, const int32_t* color) {
+  const int lane = lane_id();
+  const int64_t wave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  for (int64_t v = wave; v < G.n_owned; v += nwaves) {
+    const uint2 r = G.row[v];
+    for (uint32_t j = lane; j < r.y; j += 64) {
+      const int32_t w = G.col[(uint64_t)r.x + j];
+      G.ccol[(uint64_t)r.x + j] = w >= 0 ? color[w] : -1;
+    }
+  }
+}
+
+// Count slots breaking an invariant: ccol != colour of the neighbour, or a
+// twin that does not point back (tests only).
+__global__ void k_check(GraphView G, const int32_t* color, unsigned long long* bad) {
+  const int lane = lane_id();
+  const int64_t wave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  unsigned long long cnt = 0;
+  for (int64_t v = wave; v < G.n_owned; v += nwaves) {
+    const uint2 r = G.row[v];
+    for (uint32_t j = lane; j < r.y; j += 64) {
+      const uint64_t s = (uint64_t)r.x + j;
+      const int32_t w = G.col[s];
+      if (w < 0) {
+        cnt += G.ccol[s] != -1;
+        continue;
+      }
+      cnt += G.ccol[s] != color[w];
+      const int32_t t = G.twin[s];
+      if (w >= G.n_owned) {
+        cnt += t != -1;
+      } else {
+        const uint2 rw = G.row[w];
+        if (t < 0 || (uint32_t)t >= rw.y) {
+          ++cnt;
+        } else {
+          const uint64_t s2 = (uint64_t)rw.x + (uint32_t)t;
+          cnt += G.col[s2] != (int32_t)v;
+          cnt += G.twin[s2] != (int32_t)j;
+        }
+      }
+    }
+  }
+  wave_add_u64(cnt, bad);
 }
 
 // count vertices / chunks that can need the hub path: rows with cap >= kMidLen
@@ -212,11 +267,12 @@ static int ensure_hub_buffers(Graph* g) {
     g->hbuf = nullptr;
     g->partial = nullptr;
     uint32_t c = need_chunks + need_chunks / 4 + 16;
-    DC_HIP(hipMalloc(&g->hbuf, sizeof(int4) * (size_t)c * 4));
+    DC_HIP(hipMalloc(&g->hbuf, sizeof(int4) * (size_t)c * 5));
     DC_HIP(hipMalloc(&g->partial, sizeof(unsigned long long) * 16 * (size_t)c));
     g->hchunk_cap = c;
     for (int i = 0; i < 3; ++i) g->F[i].h = g->hbuf + (size_t)c * i;
     g->T.h = g->hbuf + (size_t)c * 3;
+    g->X.h = g->hbuf + (size_t)c * 4;
   }
   return DC_OK;
 }
@@ -237,8 +293,8 @@ int alloc_state(Graph* g) {
   g->cbits[1] = g->cbits[0] + g->cbits_words;
   DC_HIP(hipMemsetAsync(g->cbits[0], 0, sizeof(uint32_t) * 2 * g->cbits_words, g->stream));
   // 4 frontiers (F0, F1, F2=marked set, T) x 2 int lists (S, M)
-  DC_HIP(hipMalloc(&g->fbuf, sizeof(int32_t) * n * 8));
-  DC_HIP(hipMalloc(&g->fcnt, sizeof(uint32_t) * 16));
+  DC_HIP(hipMalloc(&g->fbuf, sizeof(int32_t) * n * 10));
+  DC_HIP(hipMalloc(&g->fcnt, sizeof(uint32_t) * 20));
   for (int i = 0; i < 3; ++i) {
     g->F[i].s = g->fbuf + n * (2 * i);
     g->F[i].m = g->fbuf + n * (2 * i + 1);
@@ -247,6 +303,9 @@ int alloc_state(Graph* g) {
   g->T.s = g->fbuf + n * 6;
   g->T.m = g->fbuf + n * 7;
   g->T.cnt = g->fcnt + 12;
+  g->X.s = g->fbuf + n * 8;
+  g->X.m = g->fbuf + n * 9;
+  g->X.cnt = g->fcnt + 16;
   DC_HIP(hipMalloc(&g->ctr, sizeof(Counters)));
   DC_HIP(hipMalloc(&g->aux, sizeof(uint32_t) * 16));
   DC_HIP(hipHostMalloc(&g->hctr, sizeof(Counters), hipHostMallocDefault));
@@ -258,7 +317,7 @@ int alloc_state(Graph* g) {
   DC_HIP(hipMemsetAsync(g->prev, 0xff, sizeof(int32_t) * n, g->stream));
   DC_HIP(hipMemsetAsync(g->tmark, 0, sizeof(uint32_t) * n, g->stream));
   DC_HIP(hipMemsetAsync(g->mmark, 0, sizeof(uint32_t) * n, g->stream));
-  DC_HIP(hipMemsetAsync(g->fcnt, 0, sizeof(uint32_t) * 16, g->stream));
+  DC_HIP(hipMemsetAsync(g->fcnt, 0, sizeof(uint32_t) * 20, g->stream));
   DC_HIP(hipMemsetAsync(g->ctr, 0, sizeof(Counters), g->stream));
   for (int i = 0; i < 4; ++i) DC_HIP(hipEventCreate(&g->ev[i]));
   g->epoch = 0;
@@ -272,7 +331,7 @@ void free_state(Graph* g) {
   fused_free(g);
   for (hipEvent_t e : g->evpool) (void)hipEventDestroy(e);
   g->evpool.clear();
-  f(g->row); f(g->cap); f(g->col); f(g->prop);
+  f(g->row); f(g->cap); f(g->col); f(g->prop); f(g->ccol); f(g->twin);
   f(g->color[0]); f(g->color[1]); f(g->prev);
   f(g->tmark); f(g->mmark); f(g->pending); f(g->cbits[0]);
   f(g->fbuf); f(g->fcnt); f(g->hbuf); f(g->partial);
@@ -284,8 +343,10 @@ void free_state(Graph* g) {
     if (g->ev[i]) (void)hipEventDestroy(g->ev[i]);
 }
 
-// Lay out rows from cap[] (already final) and allocate col[].
-static int layout_rows(Graph* g, int reset_used, int32_t** new_col) {
+// Lay out rows from cap[] (already final) and allocate the slot arrays
+// (col, ccol, twin: three int32 per slot).
+static int layout_rows(Graph* g, int reset_used, int32_t** new_col, int32_t** new_ccol,
+                       int32_t** new_twin) {
   uint64_t* start = nullptr;
   DC_HIP(hipMalloc(&start, sizeof(uint64_t) * ((size_t)g->n + 1)));
   int rc = exclusive_scan_u32_to_u64(g, g->cap, start, g->n);
@@ -301,11 +362,15 @@ static int layout_rows(Graph* g, int reset_used, int32_t** new_col) {
     set_error("graph exceeds 2^32 adjacency slots");
     return DC_ERR_NOMEM;
   }
-  int32_t* col = nullptr;
+  int32_t *col = nullptr, *ccol = nullptr, *twin = nullptr;
   hipError_t e = hipMalloc(&col, sizeof(int32_t) * (size_t)(total + 1));
+  if (e == hipSuccess) e = hipMalloc(&ccol, sizeof(int32_t) * (size_t)(total + 1));
+  if (e == hipSuccess) e = hipMalloc(&twin, sizeof(int32_t) * (size_t)(total + 1));
   if (e != hipSuccess) {
     (void)hipFree(start);
-    return hip_fail(e, "hipMalloc(col)");
+    if (col) (void)hipFree(col);
+    if (ccol) (void)hipFree(ccol);
+    return hip_fail(e, "hipMalloc(slot arrays)");
   }
   hipLaunchKernelGGL(k_set_rows, dim3(grid_for(g->n)), dim3(kBlock), 0, g->stream, g->n, start,
                      g->row, reset_used);
@@ -313,6 +378,28 @@ static int layout_rows(Graph* g, int reset_used, int32_t** new_col) {
   DC_HIP(hipFree(start));
   g->total_cap = total;
   *new_col = col;
+  *new_ccol = ccol;
+  *new_twin = twin;
+  return DC_OK;
+}
+
+int rebuild_ccol(Graph* g) {
+  hipLaunchKernelGGL(k_rebuild_ccol, dim3(grid_for((int64_t)g->n_owned * 64)), dim3(kBlock), 0,
+                     g->stream, g->view(), g->color[g->cur]);
+  DC_HIP(hipGetLastError());
+  return DC_OK;
+}
+
+int check_consistency(Graph* g, int64_t* bad) {
+  DC_HIP(hipMemsetAsync(g->aux, 0, sizeof(unsigned long long), g->stream));
+  unsigned long long* cnt = reinterpret_cast<unsigned long long*>(g->aux);
+  hipLaunchKernelGGL(k_check, dim3(grid_for((int64_t)g->n_owned * 64)), dim3(kBlock), 0, g->stream,
+                     g->view(), g->color[g->cur], cnt);
+  DC_HIP(hipGetLastError());
+  unsigned long long h = 0;
+  DC_HIP(hipMemcpyAsync(&h, cnt, sizeof(h), hipMemcpyDeviceToHost, g->stream));
+  DC_HIP(hipStreamSynchronize(g->stream));
+  if (bad) *bad = (int64_t)h;
   return DC_OK;
 }
 
@@ -325,14 +412,16 @@ int csr_build(Graph* g, int64_t m, const int32_t* du, const int32_t* dv) {
   hipLaunchKernelGGL(k_set_cap, dim3(grid_for(g->n)), dim3(kBlock), 0, g->stream, g->n, g->n_owned,
                      g->cap, g->row, 0, g->opt.slack_frac, g->opt.slack_min);
   DC_HIP(hipGetLastError());
-  int32_t* col = nullptr;
-  int rc = layout_rows(g, 1, &col);
+  int32_t *col = nullptr, *ccol = nullptr, *twin = nullptr;
+  int rc = layout_rows(g, 1, &col, &ccol, &twin);
   if (rc) return rc;
   g->col = col;
+  g->ccol = ccol;
+  g->twin = twin;
   if (m > 0)
-    hipLaunchKernelGGL(k_fill, dim3(grid_for(m)), dim3(kBlock), 0, g->stream, m, du, dv,
-                       g->n_owned, g->row, g->col);
+    hipLaunchKernelGGL(k_fill, dim3(grid_for(m)), dim3(kBlock), 0, g->stream, m, du, dv, g->view());
   DC_HIP(hipGetLastError());
+  if ((rc = rebuild_ccol(g))) return rc;
   return ensure_hub_buffers(g);
 }
 
@@ -369,15 +458,26 @@ __global__ void k_apply(int64_t k, const int32_t* bu, const int32_t* bv, const u
     bool ovf_a = false, ovf_b = false;
     uint32_t pa = 0, pb = 0;
     if (ok && is_ins) {
-      uint2 ra = G.row[a];
+      // reserve both slots first: each one records the other's position
       pa = atomicAdd(&G.row[a].y, 1u);
-      if (pa < G.cap[a]) G.col[(uint64_t)ra.x + pa] = b;
-      else ovf_a = true;
+      if (!cross) pb = atomicAdd(&G.row[b].y, 1u);
+      if (pa < G.cap[a]) {
+        const uint64_t s = (uint64_t)G.row[a].x + pa;
+        G.col[s] = b;
+        G.ccol[s] = color[b];
+        G.twin[s] = cross ? -1 : (int32_t)pb;
+      } else {
+        ovf_a = true;
+      }
       if (!cross) {
-        uint2 rb = G.row[b];
-        pb = atomicAdd(&G.row[b].y, 1u);
-        if (pb < G.cap[b]) G.col[(uint64_t)rb.x + pb] = a;
-        else ovf_b = true;
+        if (pb < G.cap[b]) {
+          const uint64_t s = (uint64_t)G.row[b].x + pb;
+          G.col[s] = a;
+          G.ccol[s] = color[a];
+          G.twin[s] = (int32_t)pa;
+        } else {
+          ovf_b = true;
+        }
       }
       atomicAdd(&G.prop[a], 1);
       atomicAdd(&G.prop[b], 1);
@@ -385,8 +485,9 @@ __global__ void k_apply(int64_t k, const int32_t* bu, const int32_t* bv, const u
       atomicSub(&G.prop[a], 1);
       atomicSub(&G.prop[b], 1);
     }
-    wave_append(ovf_a, make_int4(a, (int)pa, b, 0), ovf, &ctr->overflow);
-    wave_append(ovf_b, make_int4(b, (int)pb, a, 0), ovf, &ctr->overflow);
+    // overflow entry: {row, slot position, neighbour, twin position}
+    wave_append(ovf_a, make_int4(a, (int)pa, b, cross ? -1 : (int)pb), ovf, &ctr->overflow);
+    wave_append(ovf_b, make_int4(b, (int)pb, a, (int)pa), ovf, &ctr->overflow);
     bool eq0 = ok && color[a] == color[b];
     int2 e = make_int2(a, b);
     wave_append(ok && !is_ins && !cross, e, del, &ctr->n_del);
@@ -419,9 +520,10 @@ __global__ void k_tombstone(const int2* del, uint32_t ndel, const int2* xdel, ui
     }
     uint2 r = G.row[x];
     for (uint32_t j = lane; j < r.y; j += 64) {
-      int32_t* p = &G.col[(uint64_t)r.x + j];
-      if (*p == y) {
-        *p = -1;
+      const uint64_t s = (uint64_t)r.x + j;
+      if (G.col[s] == y) {
+        G.col[s] = -1;
+        G.ccol[s] = -1;
         ++tomb;
       }
     }
@@ -431,7 +533,9 @@ __global__ void k_tombstone(const int2* del, uint32_t ndel, const int2* xdel, ui
 
 // regrow: copy every row's slots [0, min(used, old cap)) to the new layout
 __global__ void k_copy_rows(int32_t n, const uint32_t* old_start, const uint32_t* old_cap,
-                            const int32_t* old_col, const uint2* row, int32_t* col) {
+                            const int32_t* old_col, const int32_t* old_ccol,
+                            const int32_t* old_twin, const uint2* row, int32_t* col, int32_t* ccol,
+                            int32_t* twin) {
   const int lane = lane_id();
   const int64_t wave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
   const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
@@ -440,7 +544,11 @@ __global__ void k_copy_rows(int32_t n, const uint32_t* old_start, const uint32_t
     uint32_t c = old_cap[v];
     uint32_t lim = used < c ? used : c;
     uint64_t src = old_start[v], dst = row[v].x;
-    for (uint32_t j = lane; j < lim; j += 64) col[dst + j] = old_col[src + j];
+    for (uint32_t j = lane; j < lim; j += 64) {
+      col[dst + j] = old_col[src + j];
+      ccol[dst + j] = old_ccol[src + j];
+      twin[dst + j] = old_twin[src + j];
+    }
   }
 }
 
@@ -452,10 +560,13 @@ __global__ void k_save_starts(int32_t n, const uint2* row, const uint32_t* cap, 
   }
 }
 
-__global__ void k_fix_overflow(const int4* ovf, uint32_t novf, GraphView G) {
+__global__ void k_fix_overflow(const int4* ovf, uint32_t novf, GraphView G, const int32_t* color) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < novf; i += gridDim.x * blockDim.x) {
-    int4 o = ovf[i];
-    G.col[(uint64_t)G.row[o.x].x + (uint32_t)o.y] = o.z;
+    const int4 o = ovf[i];
+    const uint64_t s = (uint64_t)G.row[o.x].x + (uint32_t)o.y;
+    G.col[s] = o.z;
+    G.ccol[s] = color[o.z];
+    G.twin[s] = o.w;
   }
 }
 
@@ -470,23 +581,27 @@ static int regrow(Graph* g, uint32_t novf) {
   hipLaunchKernelGGL(k_set_cap, dim3(grid_for(g->n)), dim3(kBlock), 0, g->stream, g->n, g->n_owned,
                      g->cap, g->row, 1, g->opt.slack_frac, g->opt.slack_min);
   DC_HIP(hipGetLastError());
-  int32_t* col = nullptr;
-  int rc = layout_rows(g, 0, &col);
+  int32_t *col = nullptr, *ccol = nullptr, *twin = nullptr;
+  int rc = layout_rows(g, 0, &col, &ccol, &twin);
   if (rc) {
     (void)hipFree(old_start);
     (void)hipFree(old_cap);
     return rc;
   }
   hipLaunchKernelGGL(k_copy_rows, dim3(grid_for((int64_t)g->n * 64)), dim3(kBlock), 0, g->stream,
-                     g->n, old_start, old_cap, g->col, g->row, col);
+                     g->n, old_start, old_cap, g->col, g->ccol, g->twin, g->row, col, ccol, twin);
   DC_HIP(hipGetLastError());
   DC_HIP(hipStreamSynchronize(g->stream));
   DC_HIP(hipFree(g->col));
+  DC_HIP(hipFree(g->ccol));
+  DC_HIP(hipFree(g->twin));
   DC_HIP(hipFree(old_start));
   DC_HIP(hipFree(old_cap));
   g->col = col;
+  g->ccol = ccol;
+  g->twin = twin;
   hipLaunchKernelGGL(k_fix_overflow, dim3(grid_for(novf)), dim3(kBlock), 0, g->stream, g->ovf, novf,
-                     g->view());
+                     g->view(), g->color[g->cur]);
   DC_HIP(hipGetLastError());
   return ensure_hub_buffers(g);
 }
@@ -542,7 +657,9 @@ int csr_apply(Graph* g, const Batch& b, int rule) {
 // ------------------------------------------------------------ compact ------
 // In-place compaction of each row's live slots (ballot/mbcnt within the wave;
 // writes never pass the slot being read).
-__global__ void k_compact_rows(GraphView G) {
+// (slots move together with their ccol and twin values; newpos records each
+// old slot's new position so the twins can be re-pointed afterwards)
+__global__ void k_compact_rows(GraphView G, int32_t* newpos) {
   const int lane = lane_id();
   const int64_t wave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
   const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
@@ -550,22 +667,51 @@ __global__ void k_compact_rows(GraphView G) {
     uint2 r = G.row[v];
     uint32_t out = 0;
     for (uint32_t j0 = 0; j0 < r.y; j0 += 64) {
-      uint32_t j = j0 + lane;
-      int32_t w = j < r.y ? G.col[(uint64_t)r.x + j] : -1;
-      bool live = w >= 0;
-      unsigned long long m = __ballot(live);
-      if (live) G.col[(uint64_t)r.x + out + lanes_below(m)] = w;
+      const uint32_t j = j0 + lane;
+      const uint64_t s = (uint64_t)r.x + j;
+      const int32_t w = j < r.y ? G.col[s] : -1;
+      const int32_t c = j < r.y ? G.ccol[s] : -1;
+      const int32_t t = j < r.y ? G.twin[s] : -1;
+      const bool live = w >= 0;
+      const unsigned long long m = __ballot(live);
+      const uint32_t np = out + lanes_below(m);
+      if (j < r.y) newpos[s] = live ? (int32_t)np : -1;
+      if (live) {
+        G.col[(uint64_t)r.x + np] = w;
+        G.ccol[(uint64_t)r.x + np] = c;
+        G.twin[(uint64_t)r.x + np] = t;
+      }
       out += (uint32_t)__popcll(m);
     }
     if (lane == 0) G.row[v].y = out;
   }
 }
 
+__global__ void k_remap_twins(GraphView G, const int32_t* newpos) {
+  const int lane = lane_id();
+  const int64_t wave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  for (int64_t v = wave; v < G.n_owned; v += nwaves) {
+    const uint2 r = G.row[v];
+    for (uint32_t j = lane; j < r.y; j += 64) {
+      const uint64_t s = (uint64_t)r.x + j;
+      const int32_t w = G.col[s];
+      if (w < 0 || w >= G.n_owned) continue;
+      G.twin[s] = newpos[(uint64_t)G.row[w].x + (uint32_t)G.twin[s]];
+    }
+  }
+}
+
 int csr_compact(Graph* g) {
-  hipLaunchKernelGGL(k_compact_rows, dim3(grid_for((int64_t)g->n_owned * 64)), dim3(kBlock), 0,
-                     g->stream, g->view());
-  DC_HIP(hipGetLastError());
-  DC_HIP(hipStreamSynchronize(g->stream));
+  int32_t* newpos = nullptr;
+  DC_HIP(hipMalloc(&newpos, sizeof(int32_t) * (size_t)(g->total_cap + 1)));
+  const dim3 grid(grid_for((int64_t)g->n_owned * 64));
+  hipLaunchKernelGGL(k_compact_rows, grid, dim3(kBlock), 0, g->stream, g->view(), newpos);
+  hipLaunchKernelGGL(k_remap_twins, grid, dim3(kBlock), 0, g->stream, g->view(), newpos);
+  hipError_t e = hipGetLastError();
+  (void)hipStreamSynchronize(g->stream);
+  (void)hipFree(newpos);
+  if (e != hipSuccess) return hip_fail(e, "compact");
   return DC_OK;
 }
 

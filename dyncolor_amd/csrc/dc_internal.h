@@ -54,6 +54,12 @@ struct Counters {
 };
 
 // Everything a kernel needs about one graph, passed by value.
+//
+// Every slot also carries its neighbour's current colour (ccol, -1 for a
+// tombstone) and the position of the reverse slot inside the neighbour's row
+// (twin, -1 when the neighbour is a ghost, whose row is empty).  Row visits
+// then stream ccol instead of gathering colour[col[j]]; when a vertex changes
+// colour, the following propagate pass rewrites ccol at its twins.
 struct GraphView {
   int32_t n;          // total vertices (owned + ghosts)
   int32_t n_owned;    // vertices >= n_owned are frozen ghosts
@@ -61,6 +67,8 @@ struct GraphView {
   uint32_t* cap;      // row capacity
   int32_t* col;       // slot -> neighbour, -1 tombstone
   int32_t* prop;      // LDF priority counter
+  int32_t* ccol;      // slot -> neighbour's colour (snapshot of the current round)
+  int32_t* twin;      // slot -> reverse slot position in the neighbour's row
 };
 
 struct Batch {
@@ -82,6 +90,8 @@ struct Graph {
   uint32_t* cap = nullptr;
   int32_t* col = nullptr;
   int32_t* prop = nullptr;
+  int32_t* ccol = nullptr;
+  int32_t* twin = nullptr;
 
   // colors: double buffer for round-synchronous snapshots
   int32_t* color[2] = {nullptr, nullptr};
@@ -95,9 +105,11 @@ struct Graph {
   uint32_t* cbits[2] = {nullptr, nullptr};  // loop-round claim bitmaps, 1 bit per vertex
   size_t cbits_words = 0;                   // words per bitmap (both share one allocation)
 
-  // frontiers: F[0], F[1] alternate; M = pre-loop marked set; T = round targets
+  // frontiers: F[0], F[1] alternate; M = pre-loop marked set; T = round targets;
+  // X = vertices whose colour changed in the current round (propagate input)
   Frontier F[3] = {};
   Frontier T = {};
+  Frontier X = {};
   int32_t* fbuf = nullptr;    // backing store of the int lists
   int4* hbuf = nullptr;       // backing store of the hub-chunk lists
   uint32_t* fcnt = nullptr;   // counters of all frontiers
@@ -142,7 +154,7 @@ struct Graph {
   void* fused = nullptr;
   int fused_blocks_per_cu = 4;
 
-  GraphView view() const { return GraphView{n, n_owned, row, cap, col, prop}; }
+  GraphView view() const { return GraphView{n, n_owned, row, cap, col, prop, ccol, twin}; }
 };
 
 // error plumbing
@@ -159,6 +171,8 @@ int hip_fail(hipError_t e, const char* where);
 int csr_build(Graph* g, int64_t m, const int32_t* du, const int32_t* dv);
 int csr_apply(Graph* g, const Batch& b, int rule);
 int csr_compact(Graph* g);
+int rebuild_ccol(Graph* g);               // ccol[j] <- colour[col[j]] for every slot
+int check_consistency(Graph* g, int64_t* bad);  // ccol/twin invariants (tests)
 int alloc_state(Graph* g);
 void free_state(Graph* g);
 int exclusive_scan_u32_to_u64(Graph* g, const uint32_t* in, uint64_t* out, int32_t n);

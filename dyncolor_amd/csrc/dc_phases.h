@@ -52,6 +52,7 @@ struct RecolorArgs {
   uint32_t mepoch;
   int32_t* pending;
   unsigned long long* partial;   // kPartStride u64 per hub chunk of `in`
+  Frontier chx;                  // vertices whose colour this launch changed
   uint32_t hub_cap;
   Counters* ctr;                 // window_full flag
   uint32_t* changed;             // color changes of this launch
@@ -73,8 +74,8 @@ __device__ __forceinline__ bool include_nbr(int32_t v, int32_t pv, int32_t w, in
 // Returns whether v joins the output marked set.
 template <int MODE>
 __device__ __forceinline__ bool finish_vertex(const RecolorArgs& A, int32_t v, int32_t cv,
-                                              int32_t m, uint32_t& changed) {
-  bool ch = m != cv;
+                                              int32_t m, uint32_t& changed, bool& ch) {
+  ch = m != cv;
   changed += ch ? 1u : 0u;
   if constexpr (MODE == M_DEL) {
     if (ch) {
@@ -171,20 +172,34 @@ __device__ __forceinline__ void mex_slots(const RecolorArgs& A, int32_t v, int32
                                           unsigned long long* hiw, int32_t lim,
                                           unsigned long long& slots, unsigned long long& live) {
   const GraphView& G = A.G;
+  // independent-rule modes stream ccol only; ldf modes also need the
+  // neighbour's ID for its priority; the static modes run before ccol is
+  // valid and gather colours.
+  constexpr bool kCcolOnly = !ModeTraits<MODE>::prop && !ModeTraits<MODE>::stat;
   for (uint32_t j = lo; j < hi; j += kUnroll * stride) {
     int32_t w[kUnroll], c[kUnroll], p[kUnroll];
 #pragma unroll
     for (int k = 0; k < kUnroll; ++k) {
       const uint32_t jj = j + k * stride;
-      w[k] = jj < hi ? G.col[rbase + jj] : -2;
+      const bool in = jj < hi;
+      if constexpr (kCcolOnly) {
+        c[k] = in ? G.ccol[rbase + jj] : -2;
+        w[k] = c[k] >= 0 ? 0 : c[k];  // live marker only
+      } else if constexpr (ModeTraits<MODE>::stat) {
+        w[k] = in ? G.col[rbase + jj] : -2;
+      } else {
+        w[k] = in ? G.col[rbase + jj] : -2;
+        c[k] = in ? G.ccol[rbase + jj] : -1;
+      }
     }
 #pragma unroll
     for (int k = 0; k < kUnroll; ++k) {
-      c[k] = -1;
       p[k] = 0;
-      if (w[k] >= 0) {
-        c[k] = A.cur[w[k]];
-        if constexpr (ModeTraits<MODE>::prop) p[k] = G.prop[w[k]];
+      if constexpr (ModeTraits<MODE>::stat) {
+        c[k] = w[k] >= 0 ? A.cur[w[k]] : -1;
+      }
+      if constexpr (ModeTraits<MODE>::prop) {
+        if (w[k] >= 0) p[k] = G.prop[w[k]];
       }
     }
 #pragma unroll
@@ -274,9 +289,10 @@ __device__ __forceinline__ void recolor_body(const RecolorArgs& A) {
       const unsigned long long word = __shfl(x, wi < 0 ? 0 : wi);
       m = wi * 64 + __ffsll((long long)~word) - 1;
     }
-    bool push = false;
-    if (lane == 0) push = finish_vertex<MODE>(A, v, A.cur[v], m, changed);
+    bool push = false, chv = false;
+    if (lane == 0) push = finish_vertex<MODE>(A, v, A.cur[v], m, changed, chv);
     frontier_push(push, v, G, A.out, A.hub_cap);
+    if constexpr (!ModeTraits<MODE>::stat) frontier_push(chv, v, G, A.chx, A.hub_cap);
   }
 
   // ---- S rows: 8-lane groups, one u64 mask (mex <= slots < 64)
@@ -295,12 +311,13 @@ __device__ __forceinline__ void recolor_body(const RecolorArgs& A) {
       mask |= __shfl_xor(mask, 1);
       mask |= __shfl_xor(mask, 2);
       mask |= __shfl_xor(mask, 4);
-      bool push = false;
+      bool push = false, chv = false;
       if (act && sub == 0) {
         const int32_t m = __ffsll((long long)~mask) - 1;
-        push = finish_vertex<MODE>(A, v, A.cur[v], m, changed);
+        push = finish_vertex<MODE>(A, v, A.cur[v], m, changed, chv);
       }
       frontier_push(push, v, G, A.out, A.hub_cap);
+      if constexpr (!ModeTraits<MODE>::stat) frontier_push(chv, v, G, A.chx, A.hub_cap);
     }
   }
   wave_add_u32(changed, A.changed);
@@ -383,11 +400,54 @@ __device__ __forceinline__ void hub_finalize_body(const RecolorArgs& A) {
     for (int i = kPartWords - 1; i >= 0; --i)
       if (w[i] != ~0ull) m = i * 64 + __ffsll((long long)~w[i]) - 1;
     if (m < 0) m = ov ? wave_row_mex<MODE>(A, v, fwin[threadIdx.x >> 6]) : 64 * kPartWords;
-    bool push = false;
-    if (lane == 0) push = finish_vertex<MODE>(A, v, A.cur[v], m, changed);
+    bool push = false, chv = false;
+    if (lane == 0) push = finish_vertex<MODE>(A, v, A.cur[v], m, changed, chv);
     frontier_push(push, v, A.G, A.out, A.hub_cap);
+    if constexpr (!ModeTraits<MODE>::stat) frontier_push(chv, v, A.G, A.chx, A.hub_cap);
   }
   wave_add_u32(changed, A.changed);
+}
+
+// ----------------------------------------------------------------------------
+// propagate: every vertex v of X changed colour this round; write its new
+// colour newc[v] into the ccol slot that points back at v in each owned
+// neighbour's row (found through v's twins).  Runs between rounds, when no
+// row visit reads ccol.
+__device__ __forceinline__ void propagate_slots(const GraphView& G, int32_t c, uint64_t rbase,
+                                                uint32_t lo, uint32_t hi, uint32_t stride) {
+  for (uint32_t j = lo; j < hi; j += stride) {
+    const int32_t w = G.col[rbase + j];
+    if (w < 0 || w >= G.n_owned) continue;
+    G.ccol[(uint64_t)G.row[w].x + (uint32_t)G.twin[rbase + j]] = c;
+  }
+}
+
+__device__ __forceinline__ void propagate_body(const GraphView& G, const Frontier& X,
+                                               const int32_t* newc) {
+  const int lane = lane_id();
+  const int64_t gwave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  const uint32_t nS = X.cnt[0], nM = X.cnt[1], nH = X.cnt[2];
+  for (uint32_t ci = blockIdx.x; ci < nH; ci += gridDim.x) {
+    const int4 it = X.h[ci];
+    const uint2 r = G.row[it.x];
+    const uint32_t lo = (uint32_t)it.y * kChunk;
+    propagate_slots(G, newc[it.x], r.x, lo + threadIdx.x, min(r.y, lo + kChunk), kBlock);
+  }
+  for (int64_t i = gwave; i < nM; i += nwaves) {
+    const int32_t v = X.m[i];
+    const uint2 r = G.row[v];
+    propagate_slots(G, newc[v], r.x, lane, r.y, 64);
+  }
+  const int grp = lane >> 3, sub = lane & 7;
+  for (int64_t b = gwave * 8; b < nS; b += nwaves * 8) {
+    const int64_t idx = b + grp;
+    if (idx < nS) {
+      const int32_t v = X.s[idx];
+      const uint2 r = G.row[v];
+      propagate_slots(G, newc[v], r.x, sub, r.y, kGroup);
+    }
+  }
 }
 
 // ----------------------------------------------------------------------------
@@ -424,16 +484,15 @@ __device__ __forceinline__ void find_slots(const FindArgs& A, int32_t v, int32_t
 #pragma unroll
     for (int k = 0; k < kUnroll; ++k) {
       const uint32_t jj = j + k * stride;
-      w[k] = jj < hi ? G.col[rbase + jj] : -2;
+      const bool in = jj < hi;
+      w[k] = in ? G.col[rbase + jj] : -2;
+      c[k] = in ? G.ccol[rbase + jj] : -1;
     }
 #pragma unroll
     for (int k = 0; k < kUnroll; ++k) {
-      c[k] = 0;
       p[k] = 0;
-      const bool look = RULE == 0 ? (w[k] >= 0 && w[k] < G.n_owned) : w[k] >= 0;
-      if (look) {
-        c[k] = A.cur[w[k]];
-        if constexpr (RULE == 1) p[k] = G.prop[w[k]];
+      if constexpr (RULE == 1) {
+        if (w[k] >= 0) p[k] = G.prop[w[k]];
       }
     }
 #pragma unroll
@@ -582,9 +641,8 @@ __device__ inline uint32_t wave_distinct_colors(const GraphView& G, const int32_
     __builtin_amdgcn_wave_barrier();
     int32_t lmax = 0;
     for (uint32_t j = lane; j < r.y; j += 64) {
-      const int32_t w = G.col[(uint64_t)r.x + j];
-      if (w < 0) continue;
-      const int32_t c = cur[w];
+      const int32_t c = G.ccol[(uint64_t)r.x + j];
+      if (c < 0) continue;
       lmax = max(lmax, c);
       const int32_t cw = c - base;
       if (cw >= 0 && cw < kWinBits) atomicOr(&ww[cw >> 6], 1ull << (cw & 63));

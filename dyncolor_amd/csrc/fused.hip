@@ -49,6 +49,7 @@ struct FusedArgs {
   uint32_t epoch0;
   Frontier F0, F1, Mk;
   Frontier T0, T1, T2, T3;
+  Frontier X;
   const int2* del;
   const int2* ins;
   const int2* xdel;
@@ -69,6 +70,7 @@ struct FusedArgs {
 struct FusedScratch {
   uint32_t tcnt[4][4];
   uint32_t fcnt[3][4];
+  uint32_t xcnt[4];
   uint32_t chg[8];
   unsigned long long acc[4];
   FusedResult res;
@@ -76,13 +78,24 @@ struct FusedScratch {
 
 __device__ __forceinline__ uint32_t fcount(const Frontier& F) { return F.cnt[0] + F.cnt[1] + F.cnt[3]; }
 
-// a recolor round: the row pass, then (after every chunk is done) the hub merge
+// a recolor round: the row pass, then (after every chunk is done) the hub
+// merge, then the propagation of the changed colours into ccol.  X's counters
+// are cleared right after the propagation; the next phase's barrier orders
+// that before the next round's pushes.
 template <int MODE>
 __device__ __forceinline__ void recolor_phase(cg::grid_group& grid, const RecolorArgs& R) {
   recolor_body<MODE>(R);
   grid.sync();
   hub_finalize_body<MODE>(R);
   grid.sync();
+  propagate_body(R.G, R.chx, R.nxt);
+  grid.sync();
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    R.chx.cnt[0] = 0;
+    R.chx.cnt[1] = 0;
+    R.chx.cnt[2] = 0;
+    R.chx.cnt[3] = 0;
+  }
 }
 
 template <int RULE>
@@ -120,6 +133,7 @@ __global__ __launch_bounds__(kBlock) void k_fused(FusedArgs A) {
     R.mepoch = mepoch;
     R.pending = nullptr;
     R.partial = A.partial;
+    R.chx = A.X;
     R.hub_cap = A.hub_cap;
     R.ctr = A.ctr;
     R.changed = chg;
@@ -345,6 +359,7 @@ int recolor_fused(Graph* g, int rule, dc_stats* st) {
   A.T1 = frontier_with(g->T, s->tcnt[1]);
   A.T2 = frontier_with(g->T, s->tcnt[2]);
   A.T3 = frontier_with(g->T, s->tcnt[3]);
+  A.X = frontier_with(g->X, s->xcnt);
   A.del = g->del;
   A.ins = g->ins;
   A.xdel = g->xdel;

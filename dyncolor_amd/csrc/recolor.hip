@@ -33,6 +33,10 @@ __global__ __launch_bounds__(kBlock) void k_hub_finalize(RecolorArgs A) {
   hub_finalize_body<MODE>(A);
 }
 
+__global__ __launch_bounds__(kBlock) void k_propagate(GraphView G, Frontier X, const int32_t* newc) {
+  propagate_body(G, X, newc);
+}
+
 template <int RULE>
 __global__ __launch_bounds__(kBlock) void k_find(FindArgs A) {
   find_body<RULE>(A);
@@ -158,6 +162,7 @@ static RecolorArgs make_rargs(Graph* g, Frontier in, Frontier out, uint32_t mepo
   A.mepoch = mepoch;
   A.pending = g->pending;
   A.partial = g->partial;
+  A.chx = g->X;
   A.hub_cap = g->hub_cap;
   A.ctr = g->ctr;
   A.changed = &g->ctr->changed;
@@ -197,16 +202,20 @@ static int prof_collect(Graph* g, double* ms) {
   return DC_OK;
 }
 
-// One recolor = the row pass over every class, then the hub merge pass.
+// One recolor = the row pass over every class, then the hub merge pass, then
+// (independent/ldf modes) the propagation of the changed colours into ccol.
 template <int MODE>
 static void launch_pair(Graph* g, dim3 grid, const RecolorArgs& A) {
   hipLaunchKernelGGL(k_recolor<MODE>, grid, dim3(kBlock), 0, g->stream, A);
   hipLaunchKernelGGL(k_hub_finalize<MODE>, grid, dim3(kBlock), 0, g->stream, A);
+  if constexpr (!ModeTraits<MODE>::stat)
+    hipLaunchKernelGGL(k_propagate, grid, dim3(kBlock), 0, g->stream, A.G, A.chx, A.nxt);
 }
 
 static int launch_recolor(Graph* g, int mode, const RecolorArgs& A, float* ms) {
   (void)ms;
   dim3 grid(frontier_grid(g));
+  DC_HIP(hipMemsetAsync(A.chx.cnt, 0, sizeof(uint32_t) * 4, g->stream));
   int prc = prof_begin(g);
   if (prc) return prc;
   switch (mode) {
@@ -488,6 +497,7 @@ int static_color(Graph* g, int rule) {
   }
   DC_HIP(hipMemcpyAsync(g->color[g->cur ^ 1], g->color[g->cur], sizeof(int32_t) * (size_t)g->n,
                         hipMemcpyDeviceToDevice, g->stream));
+  if ((rc = rebuild_ccol(g))) return rc;
   DC_HIP(hipStreamSynchronize(g->stream));
   return DC_OK;
 }

@@ -169,6 +169,12 @@ int dc_prop_get(dc_graph* g, int32_t* prop);
 /* Compact tombstoned row slots (keeps spare capacity). */
 int dc_compact(dc_graph* g);
 
+/* Count row slots whose cached neighbour colour or reverse-slot link is
+ * stale (0 when the device layout is consistent); for tests.  No counterpart
+ * in the reference, whose rows hold only {neighbour, flag}
+ * (HI/all_structure_undir.cuh:26-29). */
+int dc_check_consistency(dc_graph* g, int64_t* bad_slots);
+
 /* Enable per-kernel hipEvent timing of the recolor kernels (fills
  * dc_stats.ms_recolor_kernel). */
 int dc_set_profiling(dc_graph* g, int32_t on);

@@ -58,6 +58,8 @@ def run_both(oracle_mod, n, u, v, batches, rule, colors=None, opts=None, n_owned
             bad = np.nonzero(got != og.color)[0]
             raise AssertionError(f"batch {bi}: {bad.size} vertices differ, first {bad[:10]}")
         assert tr.tolist() == ph.tolist(), f"batch {bi}: trace differs"
+        # slot colour cache and reverse-slot links stay exact
+        assert g.check_consistency() == 0, f"batch {bi}: stale slot colour / twin"
         stats.append(st)
     return g, og, stats
 
@@ -228,6 +230,7 @@ def test_compaction_keeps_graph(oracle_mod):
     g.compact()
     after = [sorted(g.neighbors(x).tolist()) for x in range(0, 4000, 37)]
     assert before == after
+    assert g.check_consistency() == 0
     # continue both after compaction (lines drawn from the original edge set:
     # some deletions hit absent edges, which is legal)
     for bu, bv, bi in d.gen_batches(4000, u, v, 1, 1000, 0.5, 99):
