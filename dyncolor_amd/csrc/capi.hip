@@ -355,8 +355,13 @@ int dc_recolor(dc_graph* h, int32_t rule, dc_stats* st) {
     std::memset(st, 0, sizeof(*st));
     st->updates = g->staged_k;
   }
-  int rc = g->opt.driver == DC_DRIVER_FUSED ? dc::recolor_fused(g, rule, st)
-                                            : dc::recolor_plain(g, rule, st);
+  // independent rule: the incremental loop (one launch per phase, or one
+  // cooperative launch for the fused driver); ldf rule: the fused driver runs
+  // the whole recolor in one cooperative launch
+  const bool fused = g->opt.driver == DC_DRIVER_FUSED;
+  int rc = rule == DC_RULE_INDEPENDENT ? dc::recolor_plain(g, rule, st, fused)
+           : fused                     ? dc::recolor_fused(g, rule, st)
+                                       : dc::recolor_plain(g, rule, st);
   if (st) {
     st->insertions = g->staged_ins;
     st->deletions = g->staged_del;

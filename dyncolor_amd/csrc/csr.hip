@@ -201,14 +201,46 @@ __global__ void k_rebuild_ccol(GraphView G, const int32_t* color) {
 }
 
 // Count slots breaking an invariant: ccol != colour of the neighbour, or a
-// twin that does not point back (tests only).
-__global__ void k_check(GraphView G, const int32_t* color, unsigned long long* bad) {
+// twin that does not point back; and vertices whose cached mex (below the
+// window size) is not the mex of their neighbours' colours (tests only).
+__global__ __launch_bounds__(kBlock) void k_check(GraphView G, const int32_t* color,
+                                                  unsigned long long* bad) {
+  __shared__ unsigned long long win[kBlock / 64][kWinWords];
   const int lane = lane_id();
   const int64_t wave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
   const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  unsigned long long* ww = win[threadIdx.x >> 6];
   unsigned long long cnt = 0;
   for (int64_t v = wave; v < G.n_owned; v += nwaves) {
     const uint2 r = G.row[v];
+    const int32_t mc = G.mexc[v];
+    if (mc >= 0 && mc < kWinBits) {
+      ww[lane] = 0ull;
+      ww[lane + 64] = 0ull;
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      for (uint32_t j = lane; j < r.y; j += 64) {
+        const int32_t w = G.col[(uint64_t)r.x + j];
+        if (w < 0) continue;
+        const int32_t c = color[w];
+        if (c >= 0 && c < kWinBits) atomicOr(&ww[c >> 6], 1ull << (c & 63));
+      }
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      bool wrong = false;
+      for (int wi = lane; wi < kWinWords; wi += 64) {
+        const int32_t lo = wi * 64;
+        const unsigned long long x = ww[wi];
+        if (lo < mc) {
+          const unsigned long long need = mc - lo >= 64 ? ~0ull : ((1ull << (mc - lo)) - 1ull);
+          wrong |= (x & need) != need;
+        }
+        if (mc >= lo && mc < lo + 64) wrong |= ((x >> (mc - lo)) & 1ull) != 0ull;
+      }
+      const unsigned long long wb = __ballot(wrong);
+      if (lane == 0 && wb) ++cnt;
+      __builtin_amdgcn_wave_barrier();
+    }
     for (uint32_t j = lane; j < r.y; j += 64) {
       const uint64_t s = (uint64_t)r.x + j;
       const int32_t w = G.col[s];
@@ -288,6 +320,10 @@ int alloc_state(Graph* g) {
   DC_HIP(hipMalloc(&g->tmark, sizeof(uint32_t) * n));
   DC_HIP(hipMalloc(&g->mmark, sizeof(uint32_t) * n));
   DC_HIP(hipMalloc(&g->pending, sizeof(int32_t) * n));
+  DC_HIP(hipMalloc(&g->mexc, sizeof(int32_t) * n));
+  DC_HIP(hipMemsetAsync(g->mexc, 0xff, sizeof(int32_t) * n, g->stream));
+  DC_HIP(hipMalloc(&g->hidx, sizeof(int32_t) * n));
+  DC_HIP(hipMemsetAsync(g->hidx, 0xff, sizeof(int32_t) * n, g->stream));
   g->cbits_words = n / 32 + 2;
   DC_HIP(hipMalloc(&g->cbits[0], sizeof(uint32_t) * 2 * g->cbits_words));
   g->cbits[1] = g->cbits[0] + g->cbits_words;
@@ -329,11 +365,13 @@ void free_state(Graph* g) {
     if (p) (void)hipFree(p);
   };
   fused_free(g);
+  loop_free(g);
   for (hipEvent_t e : g->evpool) (void)hipEventDestroy(e);
   g->evpool.clear();
   f(g->row); f(g->cap); f(g->col); f(g->prop); f(g->ccol); f(g->twin);
   f(g->color[0]); f(g->color[1]); f(g->prev);
-  f(g->tmark); f(g->mmark); f(g->pending); f(g->cbits[0]);
+  f(g->tmark); f(g->mmark); f(g->pending); f(g->cbits[0]); f(g->mexc);
+  f(g->hidx); f(g->hist); f(g->hubv); f(g->hch);
   f(g->fbuf); f(g->fcnt); f(g->hbuf); f(g->partial);
   f(g->del); f(g->ins); f(g->xdel); f(g->xins); f(g->ovf);
   f(g->ctr); f(g->aux); f(g->scan_tmp);
@@ -341,6 +379,123 @@ void free_state(Graph* g) {
   if (g->hctr) (void)hipHostFree(g->hctr);
   for (int i = 0; i < 4; ++i)
     if (g->ev[i]) (void)hipEventDestroy(g->ev[i]);
+}
+
+// ------------------------------------------------------- hub histograms ----
+// Hub rows: cap >= kMidLen.  hidx[v] = row of v's histogram or -1.
+__global__ void k_hist_index(int32_t n, int32_t n_owned, const uint32_t* cap, int32_t* hidx,
+                             int32_t* hubv, uint32_t* counter) {
+  for (int32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
+    int32_t h = -1;
+    if (v < n_owned && cap[v] >= kMidLen) {
+      h = (int32_t)atomicAdd(counter, 1u);
+      hubv[h] = v;
+    }
+    hidx[v] = h;
+  }
+}
+
+// One workgroup per hub: count the live slots' colours below kHistW in LDS.
+__global__ __launch_bounds__(kBlock) void k_hist_fill(GraphView G, const int32_t* hubv,
+                                                      uint32_t n_hubs) {
+  __shared__ uint32_t cnt[kHistW];
+  for (uint32_t h = blockIdx.x; h < n_hubs; h += gridDim.x) {
+    for (int i = threadIdx.x; i < kHistW; i += blockDim.x) cnt[i] = 0u;
+    __syncthreads();
+    const uint2 r = G.row[hubv[h]];
+    for (uint32_t j = threadIdx.x; j < r.y; j += blockDim.x) {
+      const int32_t c = G.ccol[(uint64_t)r.x + j];
+      if (c >= 0 && c < kHistW) atomicAdd(&cnt[c], 1u);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kHistW; i += blockDim.x) G.hist[(uint64_t)h * kHistW + i] = cnt[i];
+    __syncthreads();
+  }
+}
+
+// the static hub chunk table: one entry per kChunk slots of a hub's capacity
+__global__ void k_hub_chunks(const int32_t* hubv, uint32_t n_hubs, const uint32_t* cap, int2* hch,
+                             uint32_t* counter) {
+  for (uint32_t h = blockIdx.x * blockDim.x + threadIdx.x; h < n_hubs; h += gridDim.x * blockDim.x) {
+    const int32_t v = hubv[h];
+    const uint32_t nch = (cap[v] + kChunk - 1) / kChunk;
+    const uint32_t base = atomicAdd(counter, nch);
+    for (uint32_t c = 0; c < nch; ++c) hch[base + c] = make_int2(v, (int)c);
+  }
+}
+
+static int hist_fill(Graph* g) {
+  if (g->n_hubs == 0) return DC_OK;
+  const uint32_t blocks = std::min<uint32_t>(g->n_hubs, 8192u);
+  hipLaunchKernelGGL(k_hist_fill, dim3(blocks), dim3(kBlock), 0, g->stream, g->view(), g->hubv,
+                     g->n_hubs);
+  DC_HIP(hipGetLastError());
+  return DC_OK;
+}
+
+int hist_reindex(Graph* g) {
+  uint32_t* tmp = g->aux;
+  DC_HIP(hipMemsetAsync(tmp, 0, sizeof(uint32_t) * 4, g->stream));
+  hipLaunchKernelGGL(k_hub_capacity, dim3(grid_for(g->n)), dim3(kBlock), 0, g->stream, g->n, g->cap,
+                     tmp);
+  uint32_t h[2];
+  DC_HIP(hipMemcpyAsync(h, tmp, sizeof(h), hipMemcpyDeviceToHost, g->stream));
+  DC_HIP(hipStreamSynchronize(g->stream));
+  const uint32_t nh = h[0];
+  if (nh > g->hist_cap) {
+    if (g->hist) DC_HIP(hipFree(g->hist));
+    if (g->hubv) DC_HIP(hipFree(g->hubv));
+    g->hist = nullptr;
+    g->hubv = nullptr;
+    const uint32_t c = nh + nh / 4 + 16;
+    DC_HIP(hipMalloc(&g->hubv, sizeof(int32_t) * (size_t)c));
+    DC_HIP(hipMalloc(&g->hist, sizeof(uint32_t) * kHistW * (size_t)c));
+    g->hist_cap = c;
+  }
+  hipLaunchKernelGGL(k_hist_index, dim3(grid_for(g->n)), dim3(kBlock), 0, g->stream, g->n, g->n_owned,
+                     g->cap, g->hidx, g->hubv, tmp + 2);
+  DC_HIP(hipGetLastError());
+  DC_HIP(hipMemcpyAsync(&g->n_hubs, tmp + 2, sizeof(uint32_t), hipMemcpyDeviceToHost, g->stream));
+  DC_HIP(hipStreamSynchronize(g->stream));
+  // chunk table (h[1] = chunks of every row with cap >= kMidLen)
+  if (h[1] > g->hch_cap) {
+    if (g->hch) DC_HIP(hipFree(g->hch));
+    g->hch = nullptr;
+    const uint32_t c = h[1] + h[1] / 4 + 16;
+    DC_HIP(hipMalloc(&g->hch, sizeof(int2) * (size_t)c));
+    g->hch_cap = c;
+  }
+  DC_HIP(hipMemsetAsync(tmp + 3, 0, sizeof(uint32_t), g->stream));
+  if (g->n_hubs)
+    hipLaunchKernelGGL(k_hub_chunks, dim3(grid_for(g->n_hubs)), dim3(kBlock), 0, g->stream, g->hubv,
+                       g->n_hubs, g->cap, g->hch, tmp + 3);
+  DC_HIP(hipGetLastError());
+  DC_HIP(hipMemcpyAsync(&g->n_hch, tmp + 3, sizeof(uint32_t), hipMemcpyDeviceToHost, g->stream));
+  DC_HIP(hipStreamSynchronize(g->stream));
+  return hist_fill(g);
+}
+
+// tests: histogram entries that differ from a recount of the hub's row
+__global__ __launch_bounds__(kBlock) void k_hist_check(GraphView G, const int32_t* hubv,
+                                                       uint32_t n_hubs, unsigned long long* bad) {
+  __shared__ uint32_t cnt[kHistW];
+  unsigned long long b = 0;
+  for (uint32_t h = blockIdx.x; h < n_hubs; h += gridDim.x) {
+    for (int i = threadIdx.x; i < kHistW; i += blockDim.x) cnt[i] = 0u;
+    __syncthreads();
+    const int32_t v = hubv[h];
+    const uint2 r = G.row[v];
+    for (uint32_t j = threadIdx.x; j < r.y; j += blockDim.x) {
+      const int32_t c = G.ccol[(uint64_t)r.x + j];
+      if (c >= 0 && c < kHistW) atomicAdd(&cnt[c], 1u);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kHistW; i += blockDim.x)
+      b += G.hist[(uint64_t)h * kHistW + i] != cnt[i];
+    if (threadIdx.x == 0) b += G.hidx[v] != (int32_t)h;
+    __syncthreads();
+  }
+  wave_add_u64(b, bad);
 }
 
 // Lay out rows from cap[] (already final) and allocate the slot arrays
@@ -384,10 +539,12 @@ static int layout_rows(Graph* g, int reset_used, int32_t** new_col, int32_t** ne
 }
 
 int rebuild_ccol(Graph* g) {
+  // every cached mex is stale once colours are replaced wholesale
+  DC_HIP(hipMemsetAsync(g->mexc, 0xff, sizeof(int32_t) * ((size_t)g->n + 1), g->stream));
   hipLaunchKernelGGL(k_rebuild_ccol, dim3(grid_for((int64_t)g->n_owned * 64)), dim3(kBlock), 0,
                      g->stream, g->view(), g->color[g->cur]);
   DC_HIP(hipGetLastError());
-  return DC_OK;
+  return hist_fill(g);
 }
 
 int check_consistency(Graph* g, int64_t* bad) {
@@ -395,11 +552,17 @@ int check_consistency(Graph* g, int64_t* bad) {
   unsigned long long* cnt = reinterpret_cast<unsigned long long*>(g->aux);
   hipLaunchKernelGGL(k_check, dim3(grid_for((int64_t)g->n_owned * 64)), dim3(kBlock), 0, g->stream,
                      g->view(), g->color[g->cur], cnt);
+  if (g->n_hubs)
+    hipLaunchKernelGGL(k_hist_check, dim3(std::min<uint32_t>(g->n_hubs, 4096u)), dim3(kBlock), 0,
+                       g->stream, g->view(), g->hubv, g->n_hubs, cnt);
   DC_HIP(hipGetLastError());
   unsigned long long h = 0;
   DC_HIP(hipMemcpyAsync(&h, cnt, sizeof(h), hipMemcpyDeviceToHost, g->stream));
   DC_HIP(hipStreamSynchronize(g->stream));
-  if (bad) *bad = (int64_t)h;
+  int64_t lbad = 0;
+  int rc = loop_check(g, &lbad);
+  if (rc) return rc;
+  if (bad) *bad = (int64_t)h + lbad;
   return DC_OK;
 }
 
@@ -422,6 +585,7 @@ int csr_build(Graph* g, int64_t m, const int32_t* du, const int32_t* dv) {
     hipLaunchKernelGGL(k_fill, dim3(grid_for(m)), dim3(kBlock), 0, g->stream, m, du, dv, g->view());
   DC_HIP(hipGetLastError());
   if ((rc = rebuild_ccol(g))) return rc;
+  if ((rc = hist_reindex(g))) return rc;
   return ensure_hub_buffers(g);
 }
 
@@ -456,6 +620,21 @@ __global__ void k_apply(int64_t k, const int32_t* bu, const int32_t* bv, const u
     }
     bool cross = ok && b >= G.n_owned;
     bool ovf_a = false, ovf_b = false;
+    // both rows change (or may, for a deletion of an absent edge)
+    if (ok) {
+      G.mexc[a] = -1;
+      if (!cross) G.mexc[b] = -1;
+    }
+    // a new slot's colour enters its row's histogram (its slot may land
+    // after a regrow, which recounts the histograms anyway)
+    if (ok && is_ins) {
+      const int32_t ha = G.hidx[a], cb = color[b];
+      if (ha >= 0 && cb < kHistW) atomicAdd(&G.hist[(uint64_t)ha * kHistW + cb], 1u);
+      if (!cross) {
+        const int32_t hb = G.hidx[b], ca = color[a];
+        if (hb >= 0 && ca < kHistW) atomicAdd(&G.hist[(uint64_t)hb * kHistW + ca], 1u);
+      }
+    }
     uint32_t pa = 0, pb = 0;
     if (ok && is_ins) {
       // reserve both slots first: each one records the other's position
@@ -497,34 +676,55 @@ __global__ void k_apply(int64_t k, const int32_t* bu, const int32_t* bv, const u
   }
 }
 
-// One wave per (deletion line, row): tombstone every slot of the row equal to
-// the other endpoint (all copies, including ones inserted by this batch;
-// deleteEdgeFromAdj, HI/gpuFunctions_undir.cuh:118-142).
+// One wave per deletion line: tombstone every slot of the shorter row equal
+// to the other endpoint together with its twin in the other row, i.e. every
+// copy of the edge in both rows, including copies inserted by this batch
+// (deleteEdgeFromAdj, HI/gpuFunctions_undir.cuh:118-142, scans both rows; the
+// twin links make one scan of min(deg) slots enough).  A ghost endpoint has
+// no row: the owned row is scanned.
 __global__ void k_tombstone(const int2* del, uint32_t ndel, const int2* xdel, uint32_t nxdel,
                             GraphView G, Counters* ctr) {
   const int lane = lane_id();
   const int64_t wave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
   const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
-  const int64_t items = 2 * (int64_t)ndel + nxdel;
+  const int64_t items = (int64_t)ndel + nxdel;
   unsigned long long tomb = 0;
   for (int64_t it = wave; it < items; it += nwaves) {
-    int32_t x, y;
-    if (it < 2 * (int64_t)ndel) {
-      int2 e = del[it >> 1];
-      x = (it & 1) ? e.y : e.x;
-      y = (it & 1) ? e.x : e.y;
-    } else {
-      int2 e = xdel[it - 2 * (int64_t)ndel];
-      x = e.x;
-      y = e.y;
-    }
+    const bool cross = it >= (int64_t)ndel;
+    const int2 e = cross ? xdel[it - ndel] : del[it];
+    int32_t x = e.x, y = e.y;   // cross lines are (owned, ghost)
     uint2 r = G.row[x];
+    uint2 ry = make_uint2(0u, 0u);
+    if (!cross) {
+      ry = G.row[y];
+      if (ry.y < r.y) {
+        const int32_t t = x;
+        x = y;
+        y = t;
+        const uint2 u = r;
+        r = ry;
+        ry = u;
+      }
+    }
+    // Each slot is retired by the one thread that swaps its live value out
+    // (the same edge can come twice, in either orientation), which also
+    // takes its colour out of the row's histogram.
+    const int32_t hx = G.hidx[x], hy = cross ? -1 : G.hidx[y];
     for (uint32_t j = lane; j < r.y; j += 64) {
       const uint64_t s = (uint64_t)r.x + j;
-      if (G.col[s] == y) {
-        G.col[s] = -1;
-        G.ccol[s] = -1;
+      if (G.col[s] != y) continue;
+      if (atomicExch(&G.col[s], -1) == y) {
+        const int32_t c = atomicExch(&G.ccol[s], -1);
+        if (hx >= 0 && c >= 0 && c < kHistW) atomicSub(&G.hist[(uint64_t)hx * kHistW + c], 1u);
         ++tomb;
+      }
+      if (!cross) {
+        const uint64_t s2 = (uint64_t)ry.x + (uint32_t)G.twin[s];
+        if (atomicExch(&G.col[s2], -1) == x) {
+          const int32_t c = atomicExch(&G.ccol[s2], -1);
+          if (hy >= 0 && c >= 0 && c < kHistW) atomicSub(&G.hist[(uint64_t)hy * kHistW + c], 1u);
+          ++tomb;
+        }
       }
     }
   }
@@ -603,6 +803,8 @@ static int regrow(Graph* g, uint32_t novf) {
   hipLaunchKernelGGL(k_fix_overflow, dim3(grid_for(novf)), dim3(kBlock), 0, g->stream, g->ovf, novf,
                      g->view(), g->color[g->cur]);
   DC_HIP(hipGetLastError());
+  // capacities changed: hub rows and their histograms are recomputed
+  if ((rc = hist_reindex(g))) return rc;
   return ensure_hub_buffers(g);
 }
 
@@ -643,7 +845,7 @@ int csr_apply(Graph* g, const Batch& b, int rule) {
     if (rc) return rc;
     g->staged_regrows = 1;
   }
-  uint32_t items = 2 * c.n_del + c.n_xdel;
+  uint32_t items = c.n_del + c.n_xdel;
   if (items > 0) {
     int64_t threads = (int64_t)items * 64;
     hipLaunchKernelGGL(k_tombstone, dim3(grid_for(threads)), dim3(kBlock), 0, g->stream, g->del,

@@ -26,6 +26,7 @@ constexpr uint32_t kMidLen = 512;
 constexpr uint32_t kChunk = 1024;
 constexpr int kWinBits = 8192;               // LDS mex window (bits)
 constexpr int kWinWords = kWinBits / 64;     // u64 words per window
+constexpr int kHistW = 512;                  // hub colour histogram width (colours 0..511)
 
 // One frontier = three class lists + counters.
 struct Frontier {
@@ -69,6 +70,9 @@ struct GraphView {
   int32_t* prop;      // LDF priority counter
   int32_t* ccol;      // slot -> neighbour's colour (snapshot of the current round)
   int32_t* twin;      // slot -> reverse slot position in the neighbour's row
+  int32_t* mexc;      // vertex -> mex of all its live neighbours' colours, -1 = stale
+  int32_t* hidx;      // vertex -> hub histogram row, -1 for rows with cap < kMidLen
+  uint32_t* hist;     // hub row -> live neighbours per colour 0..kHistW-1
 };
 
 struct Batch {
@@ -92,6 +96,19 @@ struct Graph {
   int32_t* prop = nullptr;
   int32_t* ccol = nullptr;
   int32_t* twin = nullptr;
+  // Cached mex over all live neighbours (the mex of the deletion, insertion
+  // and independent-rule loop rounds).  A row edit or a neighbour's colour
+  // change sets it to -1; the next evaluation of the vertex rescans its row.
+  int32_t* mexc = nullptr;
+  // Hub rows (cap >= kMidLen) also keep a histogram of their live
+  // neighbours' colours below kHistW, so a hub's mex is one short read.
+  int32_t* hidx = nullptr;
+  uint32_t* hist = nullptr;
+  int32_t* hubv = nullptr;      // hub row -> vertex
+  uint32_t n_hubs = 0, hist_cap = 0;
+  // every kChunk-slot chunk of every hub row's capacity: {vertex, chunk}
+  int2* hch = nullptr;
+  uint32_t n_hch = 0, hch_cap = 0;
 
   // colors: double buffer for round-synchronous snapshots
   int32_t* color[2] = {nullptr, nullptr};
@@ -154,7 +171,12 @@ struct Graph {
   void* fused = nullptr;
   int fused_blocks_per_cu = 4;
 
-  GraphView view() const { return GraphView{n, n_owned, row, cap, col, prop, ccol, twin}; }
+  // incremental independent-rule loop state (loop.hip)
+  void* inc = nullptr;
+
+  GraphView view() const {
+    return GraphView{n, n_owned, row, cap, col, prop, ccol, twin, mexc, hidx, hist};
+  }
 };
 
 // error plumbing
@@ -171,14 +193,28 @@ int hip_fail(hipError_t e, const char* where);
 int csr_build(Graph* g, int64_t m, const int32_t* du, const int32_t* dv);
 int csr_apply(Graph* g, const Batch& b, int rule);
 int csr_compact(Graph* g);
-int rebuild_ccol(Graph* g);               // ccol[j] <- colour[col[j]] for every slot
+int rebuild_ccol(Graph* g);               // ccol[j] <- colour[col[j]] for every slot (+ hub histograms)
+int hist_reindex(Graph* g);               // hub rows from cap[], then rebuild the histograms
 int check_consistency(Graph* g, int64_t* bad);  // ccol/twin invariants (tests)
 int alloc_state(Graph* g);
 void free_state(Graph* g);
 int exclusive_scan_u32_to_u64(Graph* g, const uint32_t* in, uint64_t* out, int32_t n);
 
 // recolor.hip
-int recolor_plain(Graph* g, int rule, dc_stats* st);
+// persistent_loop: run the independent-rule loop as one cooperative launch
+int recolor_plain(Graph* g, int rule, dc_stats* st, bool persistent_loop = false);
+// loop.hip: the independent-rule loop with incrementally maintained
+// candidate counts; L0 = the marked set of the pre-loop rounds
+struct LoopResult {
+  int32_t rounds = 0;
+  uint64_t evaluated = 0, changed = 0, frontier_sum = 0;
+  uint64_t slots = 0, live = 0, rslots = 0, rlive = 0;
+  double kernel_ms = 0.0;
+  int64_t launches = 0;
+};
+int loop_hi(Graph* g, Frontier L0, bool persistent, LoopResult* res);
+void loop_free(Graph* g);
+int loop_check(Graph* g, int64_t* bad);   // candidate counts / bitmaps clean at rest
 // fused.hip
 int recolor_fused(Graph* g, int rule, dc_stats* st);
 void fused_free(Graph* g);

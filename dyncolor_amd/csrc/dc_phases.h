@@ -32,6 +32,8 @@ struct ModeTraits {
   static constexpr bool prop = MODE == M_LDF_P || MODE == M_LOOP_LDF || MODE == M_STATIC_LDF;
   static constexpr bool stat = MODE == M_STATIC_ID || MODE == M_STATIC_LDF;
   static constexpr bool marks = MODE == M_DEL || MODE == M_INS || MODE == M_LDF_P;
+  // the mex of every live neighbour: may use and refresh the per-vertex cache
+  static constexpr bool cached = MODE == M_DEL || MODE == M_INS || MODE == M_LOOP_HI;
 };
 
 constexpr int kUnroll = 4;
@@ -242,6 +244,8 @@ __device__ __forceinline__ void recolor_body(const RecolorArgs& A) {
     const int4 it = A.in.h[ci];
     const int32_t v = it.x;
     const uint32_t c = (uint32_t)it.y;
+    // cached modes take a hub's mex from its cache or histogram (finalize)
+    if (ModeTraits<MODE>::cached) break;
     const uint2 r = G.row[v];
     const int32_t pv = ModeTraits<MODE>::prop ? G.prop[v] : 0;
     if (threadIdx.x <= kPartWords) bwin[threadIdx.x] = 0ull;
@@ -272,22 +276,26 @@ __device__ __forceinline__ void recolor_body(const RecolorArgs& A) {
     const int32_t v = A.in.m[i];
     const uint2 r = G.row[v];
     const int32_t pv = ModeTraits<MODE>::prop ? G.prop[v] : 0;
-    if (lane < 8) hw[lane] = 0ull;
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    RegMask mk;
-    mk.clear();
-    mex_slots<MODE>(A, v, pv, r.x, lane, r.y, 64, mk, hw, 512, slots, live);
-    mk.reduce(64);
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    int32_t m = mk.first_zero();
+    int32_t m = ModeTraits<MODE>::cached ? G.mexc[v] : -1;
     if (m < 0) {
-      const unsigned long long x = lane >= 4 && lane < 8 ? hw[lane] : 0ull;
-      const unsigned long long b = __ballot(lane >= 4 && lane < 8 && x != ~0ull);
-      const int wi = __ffsll((long long)b) - 1;
-      const unsigned long long word = __shfl(x, wi < 0 ? 0 : wi);
-      m = wi * 64 + __ffsll((long long)~word) - 1;
+      if (lane < 8) hw[lane] = 0ull;
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      RegMask mk;
+      mk.clear();
+      mex_slots<MODE>(A, v, pv, r.x, lane, r.y, 64, mk, hw, 512, slots, live);
+      mk.reduce(64);
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      m = mk.first_zero();
+      if (m < 0) {
+        const unsigned long long x = lane >= 4 && lane < 8 ? hw[lane] : 0ull;
+        const unsigned long long b = __ballot(lane >= 4 && lane < 8 && x != ~0ull);
+        const int wi = __ffsll((long long)b) - 1;
+        const unsigned long long word = __shfl(x, wi < 0 ? 0 : wi);
+        m = wi * 64 + __ffsll((long long)~word) - 1;
+      }
+      if (ModeTraits<MODE>::cached && lane == 0) G.mexc[v] = m;
     }
     bool push = false, chv = false;
     if (lane == 0) push = finish_vertex<MODE>(A, v, A.cur[v], m, changed, chv);
@@ -304,16 +312,18 @@ __device__ __forceinline__ void recolor_body(const RecolorArgs& A) {
       const int32_t v = act ? A.in.s[idx] : 0;
       const uint2 r = act ? G.row[v] : make_uint2(0u, 0u);
       const int32_t pv = (act && ModeTraits<MODE>::prop) ? G.prop[v] : 0;
+      const int32_t cm = (act && ModeTraits<MODE>::cached) ? G.mexc[v] : -1;
       RegMask mk;
       mk.clear();
-      mex_slots<MODE>(A, v, pv, r.x, sub, r.y, kGroup, mk, nullptr, 0, slots, live);
+      mex_slots<MODE>(A, v, pv, r.x, sub, cm >= 0 ? 0u : r.y, kGroup, mk, nullptr, 0, slots, live);
       unsigned long long mask = mk.m0;
       mask |= __shfl_xor(mask, 1);
       mask |= __shfl_xor(mask, 2);
       mask |= __shfl_xor(mask, 4);
       bool push = false, chv = false;
       if (act && sub == 0) {
-        const int32_t m = __ffsll((long long)~mask) - 1;
+        const int32_t m = cm >= 0 ? cm : __ffsll((long long)~mask) - 1;
+        if (ModeTraits<MODE>::cached && cm < 0) G.mexc[v] = m;
         push = finish_vertex<MODE>(A, v, A.cur[v], m, changed, chv);
       }
       frontier_push(push, v, G, A.out, A.hub_cap);
@@ -323,6 +333,20 @@ __device__ __forceinline__ void recolor_body(const RecolorArgs& A) {
   wave_add_u32(changed, A.changed);
   wave_add_u64(slots, A.rslots);
   wave_add_u64(live, A.rlive);
+}
+
+// Mex of a hub from its colour histogram, by one wave (lane l looks at colours
+// 8l..8l+7); -1 when colours 0..kHistW-1 are all present.
+__device__ __forceinline__ int32_t hist_mex(const uint32_t* hrow) {
+  const int lane = lane_id();
+  const uint4* p = reinterpret_cast<const uint4*>(hrow) + 2 * lane;
+  const uint4 a = p[0], b = p[1];
+  const int idx = a.x == 0u ? 0 : a.y == 0u ? 1 : a.z == 0u ? 2 : a.w == 0u ? 3
+                : b.x == 0u ? 4 : b.y == 0u ? 5 : b.z == 0u ? 6 : b.w == 0u ? 7 : 8;
+  const unsigned long long z = __ballot(idx < 8);
+  if (z == 0ull) return -1;
+  const int l = __ffsll((long long)z) - 1;
+  return l * 8 + __shfl(idx, l);
 }
 
 // Exact mex of v's included neighbours by one wave, scanning the whole row in
@@ -344,7 +368,9 @@ __device__ inline int32_t wave_row_mex(const RecolorArgs& A, int32_t v, unsigned
       if (w < 0) continue;
       const int32_t pw = ModeTraits<MODE>::prop ? G.prop[w] : 0;
       if (!include_nbr<MODE>(v, pv, w, pw, G.n_owned)) continue;
-      const int32_t cw = A.cur[w] - base;
+      // the static modes run before ccol is valid; every other mode reads
+      // the round-start colour from the slot
+      const int32_t cw = (ModeTraits<MODE>::stat ? A.cur[w] : G.ccol[(uint64_t)r.x + j]) - base;
       if (cw >= 0 && cw < kWinBits) atomicOr(&ww[cw >> 6], 1ull << (cw & 63));
     }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -381,6 +407,19 @@ __device__ __forceinline__ void hub_finalize_body(const RecolorArgs& A) {
     if (it.y != 0) continue;  // one wave per hub: its first chunk entry
     const int32_t v = it.x;
     const uint32_t nch = (uint32_t)it.w;
+    if constexpr (ModeTraits<MODE>::cached) {
+      int32_t m = A.G.mexc[v];
+      if (m < 0) {
+        m = hist_mex(A.G.hist + (uint64_t)A.G.hidx[v] * kHistW);
+        if (m < 0) m = wave_row_mex<MODE>(A, v, fwin[threadIdx.x >> 6]);
+        if (lane == 0) A.G.mexc[v] = m;
+      }
+      bool push = false, chv = false;
+      if (lane == 0) push = finish_vertex<MODE>(A, v, A.cur[v], m, changed, chv);
+      frontier_push(push, v, A.G, A.out, A.hub_cap);
+      frontier_push(chv, v, A.G, A.chx, A.hub_cap);
+      continue;
+    }
     unsigned long long w[kPartWords], ov = 0ull;
 #pragma unroll
     for (int i = 0; i < kPartWords; ++i) w[i] = 0ull;
@@ -400,6 +439,7 @@ __device__ __forceinline__ void hub_finalize_body(const RecolorArgs& A) {
     for (int i = kPartWords - 1; i >= 0; --i)
       if (w[i] != ~0ull) m = i * 64 + __ffsll((long long)~w[i]) - 1;
     if (m < 0) m = ov ? wave_row_mex<MODE>(A, v, fwin[threadIdx.x >> 6]) : 64 * kPartWords;
+    if (ModeTraits<MODE>::cached && lane == 0) A.G.mexc[v] = m;
     bool push = false, chv = false;
     if (lane == 0) push = finish_vertex<MODE>(A, v, A.cur[v], m, changed, chv);
     frontier_push(push, v, A.G, A.out, A.hub_cap);
@@ -413,17 +453,27 @@ __device__ __forceinline__ void hub_finalize_body(const RecolorArgs& A) {
 // colour newc[v] into the ccol slot that points back at v in each owned
 // neighbour's row (found through v's twins).  Runs between rounds, when no
 // row visit reads ccol.
-__device__ __forceinline__ void propagate_slots(const GraphView& G, int32_t c, uint64_t rbase,
-                                                uint32_t lo, uint32_t hi, uint32_t stride) {
+// (v changed from colour `old` to `c`: each owned neighbour's slot cache,
+// cached mex and, for hubs, colour histogram follow)
+__device__ __forceinline__ void propagate_slots(const GraphView& G, int32_t c, int32_t old,
+                                                uint64_t rbase, uint32_t lo, uint32_t hi,
+                                                uint32_t stride) {
   for (uint32_t j = lo; j < hi; j += stride) {
     const int32_t w = G.col[rbase + j];
     if (w < 0 || w >= G.n_owned) continue;
     G.ccol[(uint64_t)G.row[w].x + (uint32_t)G.twin[rbase + j]] = c;
+    G.mexc[w] = -1;
+    const int32_t h = G.hidx[w];
+    if (h >= 0) {
+      uint32_t* hr = G.hist + (uint64_t)h * kHistW;
+      if (old >= 0 && old < kHistW) atomicSub(&hr[old], 1u);
+      if (c < kHistW) atomicAdd(&hr[c], 1u);
+    }
   }
 }
 
 __device__ __forceinline__ void propagate_body(const GraphView& G, const Frontier& X,
-                                               const int32_t* newc) {
+                                               const int32_t* newc, const int32_t* oldc) {
   const int lane = lane_id();
   const int64_t gwave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
   const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
@@ -432,12 +482,12 @@ __device__ __forceinline__ void propagate_body(const GraphView& G, const Frontie
     const int4 it = X.h[ci];
     const uint2 r = G.row[it.x];
     const uint32_t lo = (uint32_t)it.y * kChunk;
-    propagate_slots(G, newc[it.x], r.x, lo + threadIdx.x, min(r.y, lo + kChunk), kBlock);
+    propagate_slots(G, newc[it.x], oldc[it.x], r.x, lo + threadIdx.x, min(r.y, lo + kChunk), kBlock);
   }
   for (int64_t i = gwave; i < nM; i += nwaves) {
     const int32_t v = X.m[i];
     const uint2 r = G.row[v];
-    propagate_slots(G, newc[v], r.x, lane, r.y, 64);
+    propagate_slots(G, newc[v], oldc[v], r.x, lane, r.y, 64);
   }
   const int grp = lane >> 3, sub = lane & 7;
   for (int64_t b = gwave * 8; b < nS; b += nwaves * 8) {
@@ -445,7 +495,7 @@ __device__ __forceinline__ void propagate_body(const GraphView& G, const Frontie
     if (idx < nS) {
       const int32_t v = X.s[idx];
       const uint2 r = G.row[v];
-      propagate_slots(G, newc[v], r.x, sub, r.y, kGroup);
+      propagate_slots(G, newc[v], oldc[v], r.x, sub, r.y, kGroup);
     }
   }
 }

@@ -88,7 +88,7 @@ __device__ __forceinline__ void recolor_phase(cg::grid_group& grid, const Recolo
   grid.sync();
   hub_finalize_body<MODE>(R);
   grid.sync();
-  propagate_body(R.G, R.chx, R.nxt);
+  propagate_body(R.G, R.chx, R.nxt, R.cur);
   grid.sync();
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     R.chx.cnt[0] = 0;
@@ -386,6 +386,8 @@ int recolor_fused(Graph* g, int rule, dc_stats* st) {
     DC_HIP(hipLaunchCooperativeKernel((const void*)k_fused<1>, dim3(grid), dim3(kBlock), args, 0,
                                       g->stream));
   DC_HIP(hipEventRecord(g->ev[1], g->stream));
+  // the loop can end with claims still set (ldf: "no change" exit)
+  DC_HIP(hipMemsetAsync(g->cbits[0], 0, sizeof(uint32_t) * 2 * g->cbits_words, g->stream));
   FusedScratch hs;
   DC_HIP(hipMemcpyAsync(&hs, s, sizeof(FusedScratch), hipMemcpyDeviceToHost, g->stream));
   DC_HIP(hipStreamSynchronize(g->stream));

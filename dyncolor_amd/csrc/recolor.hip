@@ -33,8 +33,9 @@ __global__ __launch_bounds__(kBlock) void k_hub_finalize(RecolorArgs A) {
   hub_finalize_body<MODE>(A);
 }
 
-__global__ __launch_bounds__(kBlock) void k_propagate(GraphView G, Frontier X, const int32_t* newc) {
-  propagate_body(G, X, newc);
+__global__ __launch_bounds__(kBlock) void k_propagate(GraphView G, Frontier X, const int32_t* newc,
+                                                      const int32_t* oldc) {
+  propagate_body(G, X, newc, oldc);
 }
 
 template <int RULE>
@@ -209,7 +210,7 @@ static void launch_pair(Graph* g, dim3 grid, const RecolorArgs& A) {
   hipLaunchKernelGGL(k_recolor<MODE>, grid, dim3(kBlock), 0, g->stream, A);
   hipLaunchKernelGGL(k_hub_finalize<MODE>, grid, dim3(kBlock), 0, g->stream, A);
   if constexpr (!ModeTraits<MODE>::stat)
-    hipLaunchKernelGGL(k_propagate, grid, dim3(kBlock), 0, g->stream, A.G, A.chx, A.nxt);
+    hipLaunchKernelGGL(k_propagate, grid, dim3(kBlock), 0, g->stream, A.G, A.chx, A.nxt, A.cur);
 }
 
 static int launch_recolor(Graph* g, int mode, const RecolorArgs& A, float* ms) {
@@ -251,7 +252,7 @@ static inline uint64_t fsize(const uint32_t* cnt) { return (uint64_t)cnt[0] + cn
 // size after every round (the structure of HI/Color_main.cu:146-202 and
 // HB/Color_main.cu:160-224, with the O(n) memsets and compactions replaced by
 // epoch claims and wave-aggregated appends).
-int recolor_plain(Graph* g, int rule, dc_stats* st) {
+int recolor_plain(Graph* g, int rule, dc_stats* st, bool persistent_loop) {
   if (!g->staged) {
     set_error("dc_recolor: no staged batch");
     return DC_ERR_STATE;
@@ -387,6 +388,40 @@ int recolor_plain(Graph* g, int rule, dc_stats* st) {
   // recolor loop: L = marked set, then L <- C every round
   Frontier L = Mk;
   DC_HIP(hipMemsetAsync(g->cbits[0], 0, sizeof(uint32_t) * 2 * g->cbits_words, g->stream));
+  if (rule == DC_RULE_INDEPENDENT && change) {
+    // candidate counts kept incrementally (loop.hip); both colour buffers
+    // are written there, so no catch-up follows
+    LoopResult lr;
+    if ((rc = loop_hi(g, Mk, persistent_loop, &lr))) return rc;
+    evaluated += lr.evaluated;
+    changed_total += lr.changed;
+    frontier_sum += lr.frontier_sum;
+    slots_total += lr.slots + lr.rslots;
+    live_total += lr.live + lr.rlive;
+    recolor_bytes += 24ull * lr.evaluated + 4ull * lr.rslots;
+    launches += (int)lr.launches;
+    change = false;
+    DC_HIP(hipEventRecord(g->ev[1], g->stream));
+    DC_HIP(hipEventSynchronize(g->ev[1]));
+    float ms = 0.f;
+    DC_HIP(hipEventElapsedTime(&ms, g->ev[0], g->ev[1]));
+    double kernel_ms = 0.0;
+    if ((rc = prof_collect(g, &kernel_ms))) return rc;
+    if (st) {
+      st->recolor_bytes = (int64_t)recolor_bytes;
+      st->phases = (int32_t)(g->trace.size() / 4);
+      st->loop_rounds = lr.rounds;
+      st->evaluated = (int64_t)evaluated;
+      st->changed = (int64_t)changed_total;
+      st->frontier_sum = (int64_t)frontier_sum;
+      st->slots_scanned = (int64_t)slots_total;
+      st->live_scanned = (int64_t)live_total;
+      st->ms_recolor = ms;
+      st->ms_recolor_kernel = kernel_ms + lr.kernel_ms;
+      st->recolor_launches = launches;
+    }
+    return DC_OK;
+  }
   DC_HIP(hipMemcpyAsync(rb.cnt, Mk.cnt, sizeof(uint32_t) * 4, hipMemcpyDeviceToHost, g->stream));
   DC_HIP(hipStreamSynchronize(g->stream));
   uint64_t nL = fsize(rb.cnt);
@@ -437,6 +472,8 @@ int recolor_plain(Graph* g, int rule, dc_stats* st) {
                        L, g->view(), g->color[g->cur], g->color[g->cur ^ 1]);
     DC_HIP(hipGetLastError());
   }
+  // an ldf loop ends on "no change" with its last claims still set
+  DC_HIP(hipMemsetAsync(g->cbits[0], 0, sizeof(uint32_t) * 2 * g->cbits_words, g->stream));
   DC_HIP(hipEventRecord(g->ev[1], g->stream));
   DC_HIP(hipEventSynchronize(g->ev[1]));
   float ms = 0.f;
