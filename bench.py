@@ -99,7 +99,7 @@ def run_stream(args, d, rule, n, u, v, workload):
 
     lat = []
     tot = {"bytes": 0, "kms": 0.0, "launches": 0, "rounds": 0, "changed": 0, "evaluated": 0,
-           "slots": 0}
+           "slots": 0, "regrows": 0}
     t0 = time.perf_counter()
     for i in range(args.warmup, nb):
         tb = time.perf_counter()
@@ -113,6 +113,7 @@ def run_stream(args, d, rule, n, u, v, workload):
         tot["changed"] += st.changed
         tot["evaluated"] += st.evaluated
         tot["slots"] += st.slots_scanned
+        tot["regrows"] += st.regrows
     g.synchronize()
     elapsed = time.perf_counter() - t0
     mono, maxc = g.validate()
@@ -129,7 +130,8 @@ def run_stream(args, d, rule, n, u, v, workload):
     out["roofline"] = roofline("k_recolor<M_LOOP>" if args.driver == "plain" else "k_fused",
                                tot["bytes"], tot["kms"], tot["launches"])
     out["work"] = {"loop_rounds": tot["rounds"], "evaluated": tot["evaluated"],
-                   "changed": tot["changed"], "row_slots_scanned": tot["slots"]}
+                   "changed": tot["changed"], "row_slots_scanned": tot["slots"],
+                   "batches_with_row_growth": tot["regrows"]}
     out["cpu_baseline"] = None
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(n, u, v, batches[0], rule, colors_after_first)

@@ -63,6 +63,7 @@ static int create_common(int32_t n, int32_t n_owned, int64_t m, const int32_t* u
   if (opt) g->opt = *opt;
   if (g->opt.slack_frac < 0.f) g->opt.slack_frac = 0.f;
   if (g->opt.slack_min < 1) g->opt.slack_min = 1;
+  if (g->opt.spare_slots < 0) g->opt.spare_slots = 0;
   if (g->opt.max_rounds < 1) g->opt.max_rounds = 1;
   g->n = n;
   g->n_owned = n_owned;
@@ -365,7 +366,7 @@ int dc_recolor(dc_graph* h, int32_t rule, dc_stats* st) {
   if (st) {
     st->insertions = g->staged_ins;
     st->deletions = g->staged_del;
-    st->regrows = g->staged_regrows;
+    st->regrows = g->staged_moves;   // batches whose rows outgrew their capacity
   }
   return rc;
 }

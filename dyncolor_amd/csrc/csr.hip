@@ -267,26 +267,31 @@ __global__ __launch_bounds__(kBlock) void k_check(GraphView G, const int32_t* co
   wave_add_u64(cnt, bad);
 }
 
-// count vertices / chunks that can need the hub path: rows with cap >= kMidLen
+// count vertices / chunks that can need the hub path (rows with cap >=
+// kMidLen) and rows that keep a colour histogram (cap >= kSmallLen).  Launched
+// with a small grid: one atomic per wave and counter.
 __global__ void k_hub_capacity(int32_t n, const uint32_t* cap, uint32_t* out) {
-  uint32_t hubs = 0, chunks = 0;
+  uint32_t hubs = 0, chunks = 0, hrows = 0;
   for (int32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
     uint32_t c = cap[v];
     if (c >= kMidLen) {
       hubs += 1;
       chunks += (c + kChunk - 1) / kChunk;
     }
+    hrows += c >= kSmallLen ? 1u : 0u;
   }
   wave_add_u32(hubs, &out[0]);
   wave_add_u32(chunks, &out[1]);
+  wave_add_u32(hrows, &out[2]);
 }
+constexpr int kCountGrid = 256;
 
 // (the staged batch's counters in g->ctr must survive this: it runs inside
 // the apply when a row overflows)
 static int ensure_hub_buffers(Graph* g) {
   uint32_t* tmp = g->aux;
-  DC_HIP(hipMemsetAsync(tmp, 0, sizeof(uint32_t) * 2, g->stream));
-  hipLaunchKernelGGL(k_hub_capacity, dim3(grid_for(g->n)), dim3(kBlock), 0, g->stream, g->n, g->cap,
+  DC_HIP(hipMemsetAsync(tmp, 0, sizeof(uint32_t) * 3, g->stream));
+  hipLaunchKernelGGL(k_hub_capacity, dim3(kCountGrid), dim3(kBlock), 0, g->stream, g->n, g->cap,
                      tmp);
   uint32_t h[2];
   DC_HIP(hipMemcpyAsync(h, tmp, sizeof(h), hipMemcpyDeviceToHost, g->stream));
@@ -344,6 +349,7 @@ int alloc_state(Graph* g) {
   g->X.cnt = g->fcnt + 16;
   DC_HIP(hipMalloc(&g->ctr, sizeof(Counters)));
   DC_HIP(hipMalloc(&g->aux, sizeof(uint32_t) * 16));
+  DC_HIP(hipMalloc(&g->tail, sizeof(unsigned long long)));
   DC_HIP(hipHostMalloc(&g->hctr, sizeof(Counters), hipHostMallocDefault));
   DC_HIP(hipMemsetAsync(g->row, 0, sizeof(uint2) * n, g->stream));
   DC_HIP(hipMemsetAsync(g->cap, 0, sizeof(uint32_t) * n, g->stream));
@@ -371,10 +377,10 @@ void free_state(Graph* g) {
   f(g->row); f(g->cap); f(g->col); f(g->prop); f(g->ccol); f(g->twin);
   f(g->color[0]); f(g->color[1]); f(g->prev);
   f(g->tmark); f(g->mmark); f(g->pending); f(g->cbits[0]); f(g->mexc);
-  f(g->hidx); f(g->hist); f(g->hubv); f(g->hch);
+  f(g->hidx); f(g->hist); f(g->hrowv); f(g->hch);
   f(g->fbuf); f(g->fcnt); f(g->hbuf); f(g->partial);
   f(g->del); f(g->ins); f(g->xdel); f(g->xins); f(g->ovf);
-  f(g->ctr); f(g->aux); f(g->scan_tmp);
+  f(g->ctr); f(g->aux); f(g->tail); f(g->scan_tmp);
   f(g->scratch_batch.u); f(g->scratch_batch.v); f(g->scratch_batch.ins);
   if (g->hctr) (void)hipHostFree(g->hctr);
   for (int i = 0; i < 4; ++i)
@@ -382,27 +388,27 @@ void free_state(Graph* g) {
 }
 
 // ------------------------------------------------------- hub histograms ----
-// Hub rows: cap >= kMidLen.  hidx[v] = row of v's histogram or -1.
+// Histogram rows: cap >= kSmallLen.  hidx[v] = row of v's histogram or -1.
 __global__ void k_hist_index(int32_t n, int32_t n_owned, const uint32_t* cap, int32_t* hidx,
-                             int32_t* hubv, uint32_t* counter) {
+                             int32_t* hrowv, uint32_t* counter) {
   for (int32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
     int32_t h = -1;
-    if (v < n_owned && cap[v] >= kMidLen) {
+    if (v < n_owned && cap[v] >= kSmallLen) {
       h = (int32_t)atomicAdd(counter, 1u);
-      hubv[h] = v;
+      hrowv[h] = v;
     }
     hidx[v] = h;
   }
 }
 
 // One workgroup per hub: count the live slots' colours below kHistW in LDS.
-__global__ __launch_bounds__(kBlock) void k_hist_fill(GraphView G, const int32_t* hubv,
-                                                      uint32_t n_hubs) {
+__global__ __launch_bounds__(kBlock) void k_hist_fill(GraphView G, const int32_t* hrowv,
+                                                      uint32_t n_hrows) {
   __shared__ uint32_t cnt[kHistW];
-  for (uint32_t h = blockIdx.x; h < n_hubs; h += gridDim.x) {
+  for (uint32_t h = blockIdx.x; h < n_hrows; h += gridDim.x) {
     for (int i = threadIdx.x; i < kHistW; i += blockDim.x) cnt[i] = 0u;
     __syncthreads();
-    const uint2 r = G.row[hubv[h]];
+    const uint2 r = G.row[hrowv[h]];
     for (uint32_t j = threadIdx.x; j < r.y; j += blockDim.x) {
       const int32_t c = G.ccol[(uint64_t)r.x + j];
       if (c >= 0 && c < kHistW) atomicAdd(&cnt[c], 1u);
@@ -414,21 +420,24 @@ __global__ __launch_bounds__(kBlock) void k_hist_fill(GraphView G, const int32_t
 }
 
 // the static hub chunk table: one entry per kChunk slots of a hub's capacity
-__global__ void k_hub_chunks(const int32_t* hubv, uint32_t n_hubs, const uint32_t* cap, int2* hch,
+__global__ void k_hub_chunks(const int32_t* hrowv, uint32_t n_hrows, const uint32_t* cap, int2* hch,
                              uint32_t* counter) {
-  for (uint32_t h = blockIdx.x * blockDim.x + threadIdx.x; h < n_hubs; h += gridDim.x * blockDim.x) {
-    const int32_t v = hubv[h];
-    const uint32_t nch = (cap[v] + kChunk - 1) / kChunk;
+  for (uint32_t h = blockIdx.x * blockDim.x + threadIdx.x; h < n_hrows; h += gridDim.x * blockDim.x) {
+    const int32_t v = hrowv[h];
+    const uint32_t nch = cap[v] >= kMidLen ? (cap[v] + kChunk - 1) / kChunk : 0u;
+    if (nch == 0) continue;
     const uint32_t base = atomicAdd(counter, nch);
     for (uint32_t c = 0; c < nch; ++c) hch[base + c] = make_int2(v, (int)c);
   }
 }
 
+static int hub_chunk_table(Graph* g, uint32_t chunks);
+
 static int hist_fill(Graph* g) {
-  if (g->n_hubs == 0) return DC_OK;
-  const uint32_t blocks = std::min<uint32_t>(g->n_hubs, 8192u);
-  hipLaunchKernelGGL(k_hist_fill, dim3(blocks), dim3(kBlock), 0, g->stream, g->view(), g->hubv,
-                     g->n_hubs);
+  if (g->n_hrows == 0) return DC_OK;
+  const uint32_t blocks = std::min<uint32_t>(g->n_hrows, 8192u);
+  hipLaunchKernelGGL(k_hist_fill, dim3(blocks), dim3(kBlock), 0, g->stream, g->view(), g->hrowv,
+                     g->n_hrows);
   DC_HIP(hipGetLastError());
   return DC_OK;
 }
@@ -436,54 +445,74 @@ static int hist_fill(Graph* g) {
 int hist_reindex(Graph* g) {
   uint32_t* tmp = g->aux;
   DC_HIP(hipMemsetAsync(tmp, 0, sizeof(uint32_t) * 4, g->stream));
-  hipLaunchKernelGGL(k_hub_capacity, dim3(grid_for(g->n)), dim3(kBlock), 0, g->stream, g->n, g->cap,
+  hipLaunchKernelGGL(k_hub_capacity, dim3(kCountGrid), dim3(kBlock), 0, g->stream, g->n, g->cap,
                      tmp);
-  uint32_t h[2];
+  uint32_t h[3];
   DC_HIP(hipMemcpyAsync(h, tmp, sizeof(h), hipMemcpyDeviceToHost, g->stream));
   DC_HIP(hipStreamSynchronize(g->stream));
-  const uint32_t nh = h[0];
+  const uint32_t nh = h[2];
   if (nh > g->hist_cap) {
     if (g->hist) DC_HIP(hipFree(g->hist));
-    if (g->hubv) DC_HIP(hipFree(g->hubv));
+    if (g->hrowv) DC_HIP(hipFree(g->hrowv));
     g->hist = nullptr;
-    g->hubv = nullptr;
+    g->hrowv = nullptr;
     const uint32_t c = nh + nh / 4 + 16;
-    DC_HIP(hipMalloc(&g->hubv, sizeof(int32_t) * (size_t)c));
+    DC_HIP(hipMalloc(&g->hrowv, sizeof(int32_t) * (size_t)c));
     DC_HIP(hipMalloc(&g->hist, sizeof(uint32_t) * kHistW * (size_t)c));
     g->hist_cap = c;
   }
+  DC_HIP(hipMemsetAsync(tmp + 4, 0, sizeof(uint32_t), g->stream));
   hipLaunchKernelGGL(k_hist_index, dim3(grid_for(g->n)), dim3(kBlock), 0, g->stream, g->n, g->n_owned,
-                     g->cap, g->hidx, g->hubv, tmp + 2);
+                     g->cap, g->hidx, g->hrowv, tmp + 4);
   DC_HIP(hipGetLastError());
-  DC_HIP(hipMemcpyAsync(&g->n_hubs, tmp + 2, sizeof(uint32_t), hipMemcpyDeviceToHost, g->stream));
+  DC_HIP(hipMemcpyAsync(&g->n_hrows, tmp + 4, sizeof(uint32_t), hipMemcpyDeviceToHost, g->stream));
   DC_HIP(hipStreamSynchronize(g->stream));
-  // chunk table (h[1] = chunks of every row with cap >= kMidLen)
-  if (h[1] > g->hch_cap) {
+  int rc = hub_chunk_table(g, h[1]);
+  if (rc) return rc;
+  return hist_fill(g);
+}
+
+// the hub chunk table from the current capacities (chunks = an upper bound
+// of the entries, from k_hub_capacity)
+static int hub_chunk_table(Graph* g, uint32_t chunks) {
+  uint32_t* tmp = g->aux;
+  if (chunks > g->hch_cap) {
     if (g->hch) DC_HIP(hipFree(g->hch));
     g->hch = nullptr;
-    const uint32_t c = h[1] + h[1] / 4 + 16;
+    const uint32_t c = chunks + chunks / 4 + 16;
     DC_HIP(hipMalloc(&g->hch, sizeof(int2) * (size_t)c));
     g->hch_cap = c;
   }
   DC_HIP(hipMemsetAsync(tmp + 3, 0, sizeof(uint32_t), g->stream));
-  if (g->n_hubs)
-    hipLaunchKernelGGL(k_hub_chunks, dim3(grid_for(g->n_hubs)), dim3(kBlock), 0, g->stream, g->hubv,
-                       g->n_hubs, g->cap, g->hch, tmp + 3);
+  if (g->n_hrows)
+    hipLaunchKernelGGL(k_hub_chunks, dim3(grid_for(g->n_hrows)), dim3(kBlock), 0, g->stream, g->hrowv,
+                       g->n_hrows, g->cap, g->hch, tmp + 3);
   DC_HIP(hipGetLastError());
   DC_HIP(hipMemcpyAsync(&g->n_hch, tmp + 3, sizeof(uint32_t), hipMemcpyDeviceToHost, g->stream));
   DC_HIP(hipStreamSynchronize(g->stream));
-  return hist_fill(g);
+  return DC_OK;
+}
+
+static int hub_chunk_table(Graph* g) {
+  uint32_t* tmp = g->aux;
+  DC_HIP(hipMemsetAsync(tmp, 0, sizeof(uint32_t) * 3, g->stream));
+  hipLaunchKernelGGL(k_hub_capacity, dim3(kCountGrid), dim3(kBlock), 0, g->stream, g->n, g->cap,
+                     tmp);
+  uint32_t h[3];
+  DC_HIP(hipMemcpyAsync(h, tmp, sizeof(h), hipMemcpyDeviceToHost, g->stream));
+  DC_HIP(hipStreamSynchronize(g->stream));
+  return hub_chunk_table(g, h[1]);
 }
 
 // tests: histogram entries that differ from a recount of the hub's row
-__global__ __launch_bounds__(kBlock) void k_hist_check(GraphView G, const int32_t* hubv,
-                                                       uint32_t n_hubs, unsigned long long* bad) {
+__global__ __launch_bounds__(kBlock) void k_hist_check(GraphView G, const int32_t* hrowv,
+                                                       uint32_t n_hrows, unsigned long long* bad) {
   __shared__ uint32_t cnt[kHistW];
   unsigned long long b = 0;
-  for (uint32_t h = blockIdx.x; h < n_hubs; h += gridDim.x) {
+  for (uint32_t h = blockIdx.x; h < n_hrows; h += gridDim.x) {
     for (int i = threadIdx.x; i < kHistW; i += blockDim.x) cnt[i] = 0u;
     __syncthreads();
-    const int32_t v = hubv[h];
+    const int32_t v = hrowv[h];
     const uint2 r = G.row[v];
     for (uint32_t j = threadIdx.x; j < r.y; j += blockDim.x) {
       const int32_t c = G.ccol[(uint64_t)r.x + j];
@@ -509,9 +538,14 @@ static int layout_rows(Graph* g, int reset_used, int32_t** new_col, int32_t** ne
     (void)hipFree(start);
     return rc;
   }
-  uint64_t total = 0;
-  DC_HIP(hipMemcpyAsync(&total, start + g->n, sizeof(uint64_t), hipMemcpyDeviceToHost, g->stream));
+  uint64_t rows = 0;
+  DC_HIP(hipMemcpyAsync(&rows, start + g->n, sizeof(uint64_t), hipMemcpyDeviceToHost, g->stream));
   DC_HIP(hipStreamSynchronize(g->stream));
+  // a spare tail of half the row slots (at least 1 Mi): rows that outgrow
+  // their capacity move there one by one instead of relaying every row
+  const uint64_t spare = g->opt.spare_slots > 0 ? (uint64_t)g->opt.spare_slots
+                                                : std::max<uint64_t>(rows / 2, 1ull << 20);
+  const uint64_t total = rows + spare;
   if (total >= 0xffffffffull) {
     (void)hipFree(start);
     set_error("graph exceeds 2^32 adjacency slots");
@@ -532,6 +566,10 @@ static int layout_rows(Graph* g, int reset_used, int32_t** new_col, int32_t** ne
   DC_HIP(hipGetLastError());
   DC_HIP(hipFree(start));
   g->total_cap = total;
+  g->rows_end = rows;
+  const unsigned long long t0 = rows;
+  DC_HIP(hipMemcpyAsync(g->tail, &t0, sizeof(t0), hipMemcpyHostToDevice, g->stream));
+  DC_HIP(hipStreamSynchronize(g->stream));
   *new_col = col;
   *new_ccol = ccol;
   *new_twin = twin;
@@ -552,9 +590,9 @@ int check_consistency(Graph* g, int64_t* bad) {
   unsigned long long* cnt = reinterpret_cast<unsigned long long*>(g->aux);
   hipLaunchKernelGGL(k_check, dim3(grid_for((int64_t)g->n_owned * 64)), dim3(kBlock), 0, g->stream,
                      g->view(), g->color[g->cur], cnt);
-  if (g->n_hubs)
-    hipLaunchKernelGGL(k_hist_check, dim3(std::min<uint32_t>(g->n_hubs, 4096u)), dim3(kBlock), 0,
-                       g->stream, g->view(), g->hubv, g->n_hubs, cnt);
+  if (g->n_hrows)
+    hipLaunchKernelGGL(k_hist_check, dim3(std::min<uint32_t>(g->n_hrows, 4096u)), dim3(kBlock), 0,
+                       g->stream, g->view(), g->hrowv, g->n_hrows, cnt);
   DC_HIP(hipGetLastError());
   unsigned long long h = 0;
   DC_HIP(hipMemcpyAsync(&h, cnt, sizeof(h), hipMemcpyDeviceToHost, g->stream));
@@ -710,20 +748,30 @@ __global__ void k_tombstone(const int2* del, uint32_t ndel, const int2* xdel, ui
     // (the same edge can come twice, in either orientation), which also
     // takes its colour out of the row's histogram.
     const int32_t hx = G.hidx[x], hy = cross ? -1 : G.hidx[y];
-    for (uint32_t j = lane; j < r.y; j += 64) {
-      const uint64_t s = (uint64_t)r.x + j;
-      if (G.col[s] != y) continue;
-      if (atomicExch(&G.col[s], -1) == y) {
-        const int32_t c = atomicExch(&G.ccol[s], -1);
-        if (hx >= 0 && c >= 0 && c < kHistW) atomicSub(&G.hist[(uint64_t)hx * kHistW + c], 1u);
-        ++tomb;
+    for (uint32_t j0 = lane; j0 < r.y; j0 += 8 * 64) {
+      // 8 loads in flight per lane; matches are rare
+      int32_t cv[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const uint32_t jj = j0 + k * 64;
+        cv[k] = jj < r.y ? G.col[(uint64_t)r.x + jj] : -2;
       }
-      if (!cross) {
-        const uint64_t s2 = (uint64_t)ry.x + (uint32_t)G.twin[s];
-        if (atomicExch(&G.col[s2], -1) == x) {
-          const int32_t c = atomicExch(&G.ccol[s2], -1);
-          if (hy >= 0 && c >= 0 && c < kHistW) atomicSub(&G.hist[(uint64_t)hy * kHistW + c], 1u);
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        if (cv[k] != y) continue;
+        const uint64_t s = (uint64_t)r.x + j0 + k * 64;
+        if (atomicExch(&G.col[s], -1) == y) {
+          const int32_t c = atomicExch(&G.ccol[s], -1);
+          if (hx >= 0 && c >= 0 && c < kHistW) atomicSub(&G.hist[(uint64_t)hx * kHistW + c], 1u);
           ++tomb;
+        }
+        if (!cross) {
+          const uint64_t s2 = (uint64_t)ry.x + (uint32_t)G.twin[s];
+          if (atomicExch(&G.col[s2], -1) == x) {
+            const int32_t c = atomicExch(&G.ccol[s2], -1);
+            if (hy >= 0 && c >= 0 && c < kHistW) atomicSub(&G.hist[(uint64_t)hy * kHistW + c], 1u);
+            ++tomb;
+          }
         }
       }
     }
@@ -768,6 +816,84 @@ __global__ void k_fix_overflow(const int4* ovf, uint32_t novf, GraphView G, cons
     G.ccol[s] = color[o.z];
     G.twin[s] = o.w;
   }
+}
+
+// Move each row that over-ran its capacity into the spare tail with room
+// used + slack (one wave per row: the first overflow entry of a row claims
+// it).  Twin links hold positions inside rows, so nothing else changes.
+// flags[0]: the tail ran out (the caller relays every row instead);
+// flags[1]: a row needs a new colour histogram (cap crossed kSmallLen).
+__global__ void k_relocate(const int4* ovf, uint32_t novf, GraphView G, uint32_t* mark,
+                           uint32_t epoch, unsigned long long* tail, unsigned long long tail_end,
+                           uint32_t* flags, float frac, int32_t smin) {
+  const int lane = lane_id();
+  const int64_t wave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  for (int64_t i = wave; i < novf; i += nwaves) {
+    const int32_t v = ovf[i].x;
+    bool lead = false;
+    if (lane == 0) lead = claim(mark, v, epoch);
+    if (!__shfl((int)lead, 0)) continue;
+    const uint2 r = G.row[v];
+    const uint32_t oldcap = G.cap[v];
+    // a row that had to move gets twice the usual slack
+    const uint32_t newcap = r.y + 2 * slack_for(r.y, frac, smin);
+    unsigned long long start = 0;
+    if (lane == 0) start = atomicAdd(tail, (unsigned long long)newcap);
+    start = __shfl(start, 0);
+    if (start + newcap > tail_end) {
+      if (lane == 0) flags[0] = 1u;
+      continue;
+    }
+    const uint32_t keep = min(r.y, oldcap);
+    for (uint32_t j = lane; j < keep; j += 64) {
+      G.col[start + j] = G.col[(uint64_t)r.x + j];
+      G.ccol[start + j] = G.ccol[(uint64_t)r.x + j];
+      G.twin[start + j] = G.twin[(uint64_t)r.x + j];
+    }
+    if (lane == 0) {
+      G.row[v].x = (uint32_t)start;
+      G.cap[v] = newcap;
+      if (oldcap < kSmallLen && newcap >= kSmallLen) flags[1] = 1u;
+    }
+  }
+}
+
+static int hub_chunk_table(Graph* g);
+static int regrow(Graph* g, uint32_t novf);
+
+// Make room for a batch's overflowing insertions: relocate the rows into the
+// spare tail, or relay every row when the tail is used up.
+static int make_room(Graph* g, uint32_t novf, bool* relaid) {
+  *relaid = false;
+  uint32_t* flags = g->aux + 8;
+  DC_HIP(hipMemsetAsync(flags, 0, sizeof(uint32_t) * 2, g->stream));
+  int rc = ensure_epochs(g, 1);
+  if (rc) return rc;
+  const uint32_t e = next_epoch(g);
+  hipLaunchKernelGGL(k_relocate, dim3(grid_for((int64_t)novf * 64)), dim3(kBlock), 0, g->stream,
+                     g->ovf, novf, g->view(), g->tmark, e, g->tail,
+                     (unsigned long long)g->total_cap, flags, g->opt.slack_frac, g->opt.slack_min);
+  DC_HIP(hipGetLastError());
+  uint32_t hf[2];
+  DC_HIP(hipMemcpyAsync(hf, flags, sizeof(hf), hipMemcpyDeviceToHost, g->stream));
+  DC_HIP(hipStreamSynchronize(g->stream));
+  if (hf[0]) {
+    *relaid = true;
+    return regrow(g, novf);
+  }
+  hipLaunchKernelGGL(k_fix_overflow, dim3(grid_for(novf)), dim3(kBlock), 0, g->stream, g->ovf, novf,
+                     g->view(), g->color[g->cur]);
+  DC_HIP(hipGetLastError());
+  // capacities grew: frontier hub buffers and the hub chunk table follow; a
+  // row past kSmallLen needs a histogram (the existing ones already count
+  // the batch)
+  if (hf[1]) {
+    if ((rc = hist_reindex(g))) return rc;
+  } else if ((rc = hub_chunk_table(g))) {
+    return rc;
+  }
+  return ensure_hub_buffers(g);
 }
 
 // Give every row that over-ran its capacity new room (used + slack), relay
@@ -840,10 +966,13 @@ int csr_apply(Graph* g, const Batch& b, int rule) {
   DC_HIP(hipStreamSynchronize(g->stream));
   Counters c = *g->hctr;
   g->staged_regrows = 0;
+  g->staged_moves = 0;
   if (c.overflow > 0) {
-    rc = regrow(g, c.overflow);
+    bool relaid = false;
+    rc = make_room(g, c.overflow, &relaid);
     if (rc) return rc;
-    g->staged_regrows = 1;
+    g->staged_regrows = relaid ? 1 : 0;
+    g->staged_moves = 1;
   }
   uint32_t items = c.n_del + c.n_xdel;
   if (items > 0) {

@@ -26,7 +26,7 @@ constexpr uint32_t kMidLen = 512;
 constexpr uint32_t kChunk = 1024;
 constexpr int kWinBits = 8192;               // LDS mex window (bits)
 constexpr int kWinWords = kWinBits / 64;     // u64 words per window
-constexpr int kHistW = 512;                  // hub colour histogram width (colours 0..511)
+constexpr int kHistW = 512;                  // colour histogram width (colours 0..511)
 
 // One frontier = three class lists + counters.
 struct Frontier {
@@ -71,7 +71,7 @@ struct GraphView {
   int32_t* ccol;      // slot -> neighbour's colour (snapshot of the current round)
   int32_t* twin;      // slot -> reverse slot position in the neighbour's row
   int32_t* mexc;      // vertex -> mex of all its live neighbours' colours, -1 = stale
-  int32_t* hidx;      // vertex -> hub histogram row, -1 for rows with cap < kMidLen
+  int32_t* hidx;      // vertex -> histogram row, -1 for rows with cap < kSmallLen
   uint32_t* hist;     // hub row -> live neighbours per colour 0..kHistW-1
 };
 
@@ -86,7 +86,10 @@ struct Graph {
   int device = 0;
   dc_options opt{};
   int32_t n = 0, n_owned = 0;
-  uint64_t total_cap = 0;     // col[] slots
+  uint64_t total_cap = 0;     // col[] slots (rows, then a spare tail for relocated rows)
+  uint64_t rows_end = 0;      // end of the laid-out rows = start of the spare tail
+  unsigned long long* tail = nullptr;  // device: next free slot of the spare tail
+  int32_t staged_moves = 0;   // rows relocated into the tail by the staged batch
   hipStream_t stream = nullptr;
 
   // CSR
@@ -100,12 +103,13 @@ struct Graph {
   // and independent-rule loop rounds).  A row edit or a neighbour's colour
   // change sets it to -1; the next evaluation of the vertex rescans its row.
   int32_t* mexc = nullptr;
-  // Hub rows (cap >= kMidLen) also keep a histogram of their live
-  // neighbours' colours below kHistW, so a hub's mex is one short read.
+  // Rows with cap >= kSmallLen also keep a histogram of their live
+  // neighbours' colours below kHistW, so their mex is a short read instead of
+  // a row scan.
   int32_t* hidx = nullptr;
   uint32_t* hist = nullptr;
-  int32_t* hubv = nullptr;      // hub row -> vertex
-  uint32_t n_hubs = 0, hist_cap = 0;
+  int32_t* hrowv = nullptr;     // histogram row -> vertex
+  uint32_t n_hrows = 0, hist_cap = 0;
   // every kChunk-slot chunk of every hub row's capacity: {vertex, chunk}
   int2* hch = nullptr;
   uint32_t n_hch = 0, hch_cap = 0;
@@ -212,7 +216,7 @@ struct LoopResult {
   double kernel_ms = 0.0;
   int64_t launches = 0;
 };
-int loop_hi(Graph* g, Frontier L0, bool persistent, LoopResult* res);
+int loop_hi(Graph* g, Frontier L0, int32_t nl0, bool persistent, LoopResult* res);
 void loop_free(Graph* g);
 int loop_check(Graph* g, int64_t* bad);   // candidate counts / bitmaps clean at rest
 // fused.hip

@@ -690,12 +690,21 @@ __device__ inline uint32_t wave_distinct_colors(const GraphView& G, const int32_
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
     int32_t lmax = 0;
-    for (uint32_t j = lane; j < r.y; j += 64) {
-      const int32_t c = G.ccol[(uint64_t)r.x + j];
-      if (c < 0) continue;
-      lmax = max(lmax, c);
-      const int32_t cw = c - base;
-      if (cw >= 0 && cw < kWinBits) atomicOr(&ww[cw >> 6], 1ull << (cw & 63));
+    for (uint32_t j0 = lane; j0 < r.y; j0 += 8 * 64) {
+      int32_t cc[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const uint32_t jj = j0 + k * 64;
+        cc[k] = jj < r.y ? G.ccol[(uint64_t)r.x + jj] : -1;
+      }
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const int32_t c = cc[k];
+        if (c < 0) continue;
+        lmax = max(lmax, c);
+        const int32_t cw = c - base;
+        if (cw >= 0 && cw < kWinBits) atomicOr(&ww[cw >> 6], 1ull << (cw & 63));
+      }
     }
     for (int off = 32; off > 0; off >>= 1) lmax = max(lmax, __shfl_xor(lmax, off));
     if (base == 0) rowmax = lmax;

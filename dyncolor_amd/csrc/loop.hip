@@ -19,26 +19,35 @@
 // For x in D the count pass adds the change of x's terms to every neighbour y
 // outside D (one atomic per changed pair) and recounts K[x] from x's row.
 // Candidates of the next C are the old C, D and the neighbours whose count
-// rose (R).  A mex comes from the per-vertex cache (mexc) unless a row edit
-// or a neighbour's colour change made it stale.  Colours, traces and round
-// counts are those of the full re-derivation.
+// rose (R).  A mex comes from the per-vertex cache (mexc), a hub's colour
+// histogram, or a scan of a short row.  Colours, traces and round counts are
+// those of the full re-derivation.
+//
+// Sets are bitmaps (one bit per vertex, small enough to stay in L2): C_r in
+// B[r & 1] (so L_r = C_{r-1} is the other one), D_r in Db[r & 1], R_r in
+// Rb[r & 1].  A phase finds its vertices by scanning bitmap words; each wave
+// compacts the set bits of its 64 words (spread over the whole bitmap, so the
+// dense low-ID region of a power-law graph lands on many waves) into a list
+// in LDS.  There are no global list counters: on this GPU a few thousand
+// atomics on one address serialise into tens of microseconds, so every update
+// here is a bit set, a count change at a scattered vertex, or a
+// per-workgroup partial sum.
 //
 // One round = two phases:
-//   A(r): filter the candidates of round r-1 by K > 0 into C_r and evaluate
-//         each member at once (cached mex, or a scan of its row by an 8-lane
-//         group / a wave / a whole workgroup for hubs), commit; members of
-//         L_r = C_{r-1} that dropped out join D_r.
-//   C(r): counts over D_r's rows, the new colours of X_r (changed vertices)
-//         into the slot cache, and L_r's bitmap cleared.  The recount reads
-//         the colour of a C_r member from the colour array, so it does not
-//         wait for the slot cache.
+//   A(r): candidates = L_r | D_{r-1} | R_{r-1}.  K > 0 makes a member of C_r,
+//         evaluated at once (cached mex; a short row scanned by its lane; a
+//         mid row by the wave; a hub from its histogram) and committed;
+//         members of L_r with K == 0 leave and join D_r.
+//   C(r): rows of D_r: count changes, recounts, and for vertices whose colour
+//         changed the slot cache / mex cache / histograms of their neighbours.
+//         Hub rows go by the static hub chunk table.  The previous round's
+//         bitmaps are cleared.
 // The plain driver launches one kernel per phase (the host enqueues rounds in
 // blocks and reads one control word per block; rounds past the last one
-// return at once); the persistent driver runs both phases of every round in
-// one cooperative launch with grid barriers between phases.
+// return at once); the persistent driver runs the rounds in one cooperative
+// launch with grid barriers between phases.
 //
-// Outside the loop K is zero everywhere, and so are the bitmaps (C/L members
-// in B[0..1], D members in Db[0..1], R claims in Rb).
+// Outside the loop K is zero everywhere and so are the bitmaps.
 #include <hip/hip_runtime.h>
 #include <hip/hip_cooperative_groups.h>
 
@@ -50,15 +59,15 @@
 
 namespace dc {
 
+constexpr int kMaxBlocks = 2048;   // grid of the loop kernels (per-block partials)
+constexpr int kWaveList = 64 * 32; // set bits of 64 bitmap words
+
 struct LoopCtl {
-  int32_t done_round;     // the round whose C was empty (INT_MAX while running)
-  uint32_t pad[3];
-  unsigned long long rslots, rlive;   // rows scanned for a stale mex
-  unsigned long long uslots, ulive;   // rows scanned by the count pass
+  int32_t done_round;   // the round whose C was empty (INT_MAX while running)
+  int32_t nl0;          // |L_0|
+  uint32_t flag[2];     // C_r non-empty, by round parity
 };
 
-// Everything a loop kernel needs, by value; per-round list roles are derived
-// from the round number (lists rotate over three buffers).
 struct LoopState {
   GraphView G;
   int32_t* col0;       // colour buffers: both are written, so they agree at the end
@@ -66,26 +75,35 @@ struct LoopState {
   int32_t* prev;
   int32_t* K;
   int2* dold;          // D members: {colour, (prev + 1) | in_L << 31} before the round
-  uint32_t* B[2];      // C_r members in B[r & 1] (so L_r = C_{r-1} in the other)
-  uint32_t* Db[2];     // D_r members in Db[r & 1]
-  uint32_t* Rb;        // R claims
-  Frontier C[3], D[3], X[3], R[3];
-  Frontier L0;         // the marked set (plays C_{-1})
-  uint32_t hub_cap;
+  uint32_t* B[2];      // C_r members in B[r & 1]
+  uint32_t* Db[2];     // D_r in Db[r & 1]
+  uint32_t* Rb[2];     // R_r in Rb[r & 1]
+  uint32_t words;      // words per bitmap
+  Frontier L0;         // the marked set
+  const int2* hch;     // hub chunk table {vertex, chunk}
+  uint32_t n_hch;
+  unsigned long long heavy_slots;  // D_r rows beyond this many slots: pull recount
+  uint32_t* part;      // [2][kMaxBlocks][2]: members, changes of phase A, by round parity
+  unsigned long long* dpart;  // [2][kMaxBlocks]: slots of the rows phase A put into D
+  unsigned long long* bstat;  // [kMaxBlocks][4]: stale-scan slots/live, count-pass slots/live
   int4* trace;         // per round {|L|, |C|, changed, 0}
   LoopCtl* ctl;
 };
 
+// LDS of a loop workgroup, shared by both phases
+struct LoopLds {
+  int32_t lists[kBlock / 64][kWaveList];
+  unsigned long long wins[kBlock / 64][kWinWords];
+  uint32_t red[kBlock / 64];
+  unsigned long long red64[kBlock / 64];
+};
+
 struct IncState {
   LoopState S{};
-  int32_t* lbuf = nullptr;
-  int4* hb = nullptr;
-  uint32_t hcap = 0;
-  uint32_t* cnt = nullptr;
-  uint32_t* bits = nullptr;   // Db[0], Db[1], Rb
-  size_t words = 0;
+  uint32_t* bits = nullptr;   // Db[0..1], Rb[0..1]
   int32_t trace_cap = 0;
   LoopCtl* hctl = nullptr;
+  unsigned long long* hstat = nullptr;
 };
 
 __device__ __forceinline__ bool has_bit(const uint32_t* b, int32_t v) {
@@ -95,494 +113,631 @@ __device__ __forceinline__ void set_bit(uint32_t* b, int32_t v) {
   atomicOr(&b[v >> 5], 1u << (v & 31));
 }
 
-__device__ __forceinline__ uint32_t fsize_dev(const Frontier& F) { return F.cnt[0] + F.cnt[1] + F.cnt[3]; }
-
 // The marking predicate of findEligibleNeighbors for a slot (v -> w).
 __device__ __forceinline__ bool marks(int32_t v, int32_t cv, int32_t pv, int32_t w, int32_t cw) {
   return (cv == cw && v < w) || pv < cw;
 }
 
-__device__ __forceinline__ Frontier c_list(const LoopState& S, int r) {
-  return r < 0 ? S.L0 : S.C[r % 3];
-}
-__device__ __forceinline__ int rot(int r) { return (r + 3) % 3; }
-
-// Rows of a frontier: fn(v, row, lo, hi, stride, width) for each lane's
-// share; `width` lanes share (v, this call) for reductions (8 for S groups,
-// 64 for M rows and for each wave of an H chunk).  Idle S lanes get v = -1.
-template <class Fn>
-__device__ __forceinline__ void visit_rows(const GraphView& G, const Frontier& F, Fn fn) {
+// Walk the set bits of a bitmap given word by word (word(i) -> bits).  Each
+// wave takes 64 words strided over the bitmap, compacts their set bits into
+// its LDS list and calls fn(list, count) with the whole wave.
+template <class WordFn, class Fn>
+__device__ __forceinline__ void walk_bits(uint32_t words, int32_t* list, WordFn word, Fn fn) {
   const int lane = lane_id();
-  const int64_t gwave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
-  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
-  const uint32_t nS = F.cnt[0], nM = F.cnt[1], nH = F.cnt[2];
-  for (uint32_t ci = blockIdx.x; ci < nH; ci += gridDim.x) {
-    const int4 it = F.h[ci];
-    const uint2 r = G.row[it.x];
-    const uint32_t lo = (uint32_t)it.y * kChunk;
-    fn(it.x, r, lo + threadIdx.x, min(r.y, lo + kChunk), (uint32_t)kBlock, 64);
-  }
-  for (int64_t i = gwave; i < nM; i += nwaves) {
-    const int32_t v = F.m[i];
-    const uint2 r = G.row[v];
-    fn(v, r, (uint32_t)lane, r.y, 64u, 64);
-  }
-  const int grp = lane >> 3, sub = lane & 7;
-  for (int64_t b = gwave * 8; b < nS; b += nwaves * 8) {
-    const int64_t idx = b + grp;
-    const bool act = idx < nS;
-    const int32_t v = act ? F.s[idx] : -1;
-    const uint2 r = act ? G.row[v] : make_uint2(0u, 0u);
-    fn(v, r, (uint32_t)sub, r.y, (uint32_t)kGroup, 8);
-  }
-}
-
-// Each member of a frontier once (hubs through their first chunk entry), one
-// thread per list entry; fn(v, valid) is called by whole waves.
-template <class Fn>
-__device__ __forceinline__ void visit_members(const Frontier& F, Fn fn) {
-  const uint32_t nS = F.cnt[0], nM = F.cnt[1], nH = F.cnt[2];
-  const uint32_t tot = nS + nM + nH;
-  const uint32_t stride = gridDim.x * blockDim.x;
-  for (uint32_t base = blockIdx.x * blockDim.x; base < tot; base += stride) {
-    const uint32_t i = base + threadIdx.x;
-    int32_t v = -1;
-    if (i < nS) {
-      v = F.s[i];
-    } else if (i < nS + nM) {
-      v = F.m[i - nS];
-    } else if (i < tot) {
-      const int4 it = F.h[i - nS - nM];
-      if (it.y == 0) v = it.x;
+  const uint32_t gwave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
+  for (uint64_t base = 0; base < words; base += 64ull * nwaves) {
+    const uint64_t wi = base + (uint64_t)lane * nwaves + gwave;
+    uint32_t bits = wi < words ? word((uint32_t)wi) : 0u;
+    const uint32_t cnt = (uint32_t)__popc(bits);
+    uint32_t x = cnt;
+    for (int off = 1; off < 64; off <<= 1) {
+      const uint32_t y = __shfl_up(x, off);
+      if (lane >= off) x += y;
     }
-    fn(v, v >= 0);
+    const uint32_t total = __shfl(x, 63);
+    if (total == 0) continue;
+    uint32_t pos = x - cnt;
+    while (bits) {
+      const int b = __ffs(bits) - 1;
+      bits &= bits - 1u;
+      list[pos++] = (int32_t)(wi * 32 + b);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    fn(list, total);
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
   }
 }
 
-__device__ __forceinline__ uint32_t group_sum(uint32_t x, int width) {
-  for (int off = width >> 1; off > 0; off >>= 1) x += __shfl_xor(x, off);
-  return x;
+// Sum over the workgroup (every thread gets the total).
+__device__ __forceinline__ uint32_t block_sum(uint32_t v, uint32_t* scratch) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  const int wib = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane_id() == 0) scratch[wib] = v;
+  __syncthreads();
+  uint32_t t = 0;
+  for (int i = 0; i < (int)(blockDim.x >> 6); ++i) t += scratch[i];
+  __syncthreads();
+  return t;
+}
+
+__device__ __forceinline__ unsigned long long block_sum64(unsigned long long v,
+                                                          unsigned long long* scratch) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  const int wib = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane_id() == 0) scratch[wib] = v;
+  __syncthreads();
+  unsigned long long t = 0;
+  for (int i = 0; i < (int)(blockDim.x >> 6); ++i) t += scratch[i];
+  __syncthreads();
+  return t;
 }
 
 // ---------------------------------------------------------------------------
-// set-up: L0's bits, then every marking slot of L0's rows counted into K; the
-// marked vertices outside L0 become the candidates R_{-1}.
-__device__ __forceinline__ void init_bits(const LoopState& S) {
-  visit_members(S.L0, [&](int32_t v, bool valid) {
-    if (valid) set_bit(S.B[1], v);
-  });
-}
-
-__device__ __forceinline__ void init_push(const LoopState& S) {
+// set-up: L0's bits, every marking slot of L0's rows counted into K, marked
+// vertices become candidates (R_{-1}).
+__global__ __launch_bounds__(kBlock) void k_loop_init(LoopState S) {
   const GraphView& G = S.G;
-  const Frontier Rn = S.R[2];
-  visit_rows(G, S.L0, [&](int32_t v, uint2 r, uint32_t lo, uint32_t hi, uint32_t stride, int width) {
-    (void)width;
-    const int32_t cv = v >= 0 ? S.col0[v] : 0;
-    const int32_t pv = v >= 0 ? S.prev[v] : 0;
+  const Frontier F = S.L0;
+  const int lane = lane_id();
+  const int64_t gwave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  auto row = [&](int32_t v, uint2 r, uint32_t lo, uint32_t hi, uint32_t stride, bool head) {
+    if (head) set_bit(S.B[1], v);
+    const int32_t cv = S.col0[v];
+    const int32_t pv = S.prev[v];
     for (uint32_t j = lo; j < hi; j += stride) {
       const int32_t w = G.col[(uint64_t)r.x + j];
       const int32_t cw = G.ccol[(uint64_t)r.x + j];
-      const bool t = w >= 0 && w < G.n_owned && marks(v, cv, pv, w, cw);
-      if (t) atomicAdd(&S.K[w], 1);
-      const bool push = t && !has_bit(S.B[1], w) && claim_bit(S.Rb, w);
-      frontier_push(push, w, G, Rn, S.hub_cap);
+      if (w >= 0 && w < G.n_owned && marks(v, cv, pv, w, cw)) {
+        atomicAdd(&S.K[w], 1);
+        set_bit(S.Rb[1], w);
+      }
     }
-  });
+  };
+  for (uint32_t ci = blockIdx.x; ci < F.cnt[2]; ci += gridDim.x) {
+    const int4 it = F.h[ci];
+    const uint2 r = G.row[it.x];
+    const uint32_t lo = (uint32_t)it.y * kChunk;
+    row(it.x, r, lo + threadIdx.x, min(r.y, lo + kChunk), kBlock, it.y == 0 && threadIdx.x == 0);
+  }
+  for (int64_t i = gwave; i < F.cnt[1]; i += nwaves) {
+    const int32_t v = F.m[i];
+    const uint2 r = G.row[v];
+    row(v, r, lane, r.y, 64, lane == 0);
+  }
+  for (int64_t i = gwave * 64 + lane; i < F.cnt[0]; i += nwaves * 64) {
+    const int32_t v = F.s[i];
+    const uint2 r = G.row[v];
+    row(v, r, 0, r.y, 1, true);
+  }
 }
 
 // ---------------------------------------------------------------------------
 // phase A
-//
-// Pushes go to per-workgroup queues in LDS and are appended to the global
-// lists once per workgroup at the end of the phase (one global atomic per
-// list and class per wave of the flush, instead of one per vertex).
-constexpr int kQCap = 1024;
-struct BlockQueues {
-  int32_t v[3][kQCap];   // 0: C_r, 1: X_r, 2: D_r
-  uint32_t n[3];
-};
 
-// Append a wave's pushes to a frontier; every lane of the wave must be
-// active.  Hub chunk entries are reserved with one atomic per wave.
-__device__ __forceinline__ void push_converged(bool p, int32_t v, const GraphView& G, Frontier F,
-                                               uint32_t hub_cap) {
-  const int lane = lane_id();
-  const uint32_t len = p ? G.row[v].y : 0u;
-  wave_append(p && len < kSmallLen, v, F.s, &F.cnt[0]);
-  wave_append(p && len >= kSmallLen && len < kMidLen, v, F.m, &F.cnt[1]);
-  const bool is_h = p && len >= kMidLen;
-  const unsigned long long hm = __ballot(is_h);
-  if (hm == 0ull) return;
-  const uint32_t nch = is_h ? (len + kChunk - 1) / kChunk : 0u;
-  uint32_t x = nch;
-  for (int off = 1; off < 64; off <<= 1) {
-    const uint32_t y = __shfl_up(x, off);
-    if (lane >= off) x += y;
-  }
-  const uint32_t total = __shfl(x, 63);
-  const int leader = __ffsll((long long)hm) - 1;
-  uint32_t sbase = 0, cbase = 0;
-  if (lane == leader) {
-    sbase = atomicAdd(&F.cnt[3], (uint32_t)__popcll(hm));
-    cbase = atomicAdd(&F.cnt[2], total);
-  }
-  sbase = __shfl(sbase, leader);
-  cbase = __shfl(cbase, leader);
-  if (is_h) {
-    const uint32_t slot = sbase + lanes_below(hm);
-    const uint32_t base = cbase + x - nch;
-    if (slot < hub_cap)
-      for (uint32_t c = 0; c < nch; ++c) F.h[base + c] = make_int4(v, (int)c, (int)slot, (int)nch);
-  }
-}
-
-// Queue a push (wave-level call, divergence allowed); a full queue falls back
-// to a direct append.
-__device__ __forceinline__ void q_push(BlockQueues& Q, int qi, bool p, int32_t v, const GraphView& G,
-                                       const Frontier& F, uint32_t hub_cap) {
-  bool direct = false;
-  if (p) {
-    const uint32_t i = atomicAdd(&Q.n[qi], 1u);
-    if (i < (uint32_t)kQCap) Q.v[qi][i] = v;
-    else direct = true;
-  }
-  frontier_push(direct, v, G, F, hub_cap);
-}
-
-struct RoundA {
-  Frontier Cn, Dn, Xn;   // this round's C, D, X (written)
-  const uint32_t* cbp;   // C_{r-1} = L_r members
-  uint32_t* cb;          // C_r members
-  uint32_t* db;          // D_r members
-};
-
-// A candidate's state, loaded at once.
-struct Cand {
-  int32_t k, cm, so, po;
-  bool inl;
-};
-__device__ __forceinline__ Cand load_cand(const LoopState& S, const RoundA& P, int32_t w) {
-  Cand c;
-  c.k = S.K[w];
-  c.cm = S.G.mexc[w];
-  c.so = S.col0[w];
-  c.po = S.prev[w];
-  c.inl = has_bit(P.cbp, w);
-  return c;
-}
-// 0: not a candidate here, 1: member of C_r, 2: member of L_r leaving it.
-// src 0 = the C_{r-1} list, 1 = D_{r-1} (members of C_{r-1} are covered by
-// their C entry), 2 = R_{r-1}.
-__device__ __forceinline__ int role_of(int src, const Cand& c) {
-  if (src == 1 && c.inl) return 0;
-  if (c.k > 0) return 1;
-  return src == 0 ? 2 : 0;
-}
-
-// Commit w's new colour m (one lane).  chv: the colour changed; dv: w's
-// (in_L, colour, prev) state changed, i.e. w joins D.
-__device__ __forceinline__ void commit_member(const LoopState& S, const RoundA& P, int32_t w,
-                                              const Cand& c, int32_t m, bool& chv, bool& dv) {
-  set_bit(P.cb, w);
-  if (c.cm < 0) S.G.mexc[w] = m;
+// Commit w's new colour m (one lane); w joins D when its (in_L, colour,
+// prev) state changed.
+__device__ __forceinline__ bool commit_member(const LoopState& S, uint32_t* cb, uint32_t* dn,
+                                              int32_t w, int32_t cm, int32_t so, int32_t po,
+                                              bool inl, int32_t m, uint32_t& changed) {
+  set_bit(cb, w);
+  if (cm < 0) S.G.mexc[w] = m;
   S.col0[w] = m;
   S.col1[w] = m;
-  S.prev[w] = c.so;
-  chv = m != c.so;
-  dv = !c.inl || c.po != c.so || chv;
-  if (dv) {
-    S.dold[w] = make_int2(c.so, (c.po + 1) | (c.inl ? (int32_t)0x80000000 : 0));
+  S.prev[w] = so;
+  const bool chv = m != so;
+  changed += chv ? 1u : 0u;
+  if (!inl || po != so || chv) {
+    S.dold[w] = make_int2(so, (po + 1) | (inl ? (int32_t)0x80000000 : 0));
     S.K[w] = 0;
-    set_bit(P.db, w);
+    set_bit(dn, w);
+    return true;
   }
+  return false;
 }
 
-__device__ __forceinline__ void leave_member(const LoopState& S, const RoundA& P, int32_t x,
-                                             const Cand& c) {
-  S.dold[x] = make_int2(c.so, (c.po + 1) | (int32_t)0x80000000);
-  set_bit(P.db, x);
+// mex of a row with fewer than 64 slots, by one lane (8 loads in flight)
+__device__ __forceinline__ int32_t lane_small_mex(const GraphView& G, uint2 r,
+                                                  unsigned long long& slots) {
+  unsigned long long mask = 0ull;
+  for (uint32_t j = 0; j < r.y; j += 8) {
+    int32_t c[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) c[k] = j + k < r.y ? G.ccol[(uint64_t)r.x + j + k] : -1;
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+      if (c[k] >= 0 && c[k] < 64) mask |= 1ull << c[k];
+  }
+  slots += r.y;
+  return __ffsll((long long)~mask) - 1;
 }
 
-// M and H candidates of one source list: one wave per vertex.  A stale mex
-// comes from the hub's colour histogram, or a scan of the (< 512-slot) row.
-template <bool HUB>
-__device__ __forceinline__ void phase_a_wave(const LoopState& S, const RoundA& P, int src,
-                                             const Frontier& F, BlockQueues& Q, uint32_t& changed,
-                                             unsigned long long& slots, unsigned long long& live) {
-  __shared__ unsigned long long wwin[kBlock / 64][HUB ? kWinWords : 8];
+// mex from a colour histogram by one lane: 32 counts per step (8 loads in
+// flight); -1 when colours 0..kHistW-1 are all present
+__device__ __forceinline__ int32_t lane_hist_mex(const uint32_t* hrow) {
+  const uint4* p = reinterpret_cast<const uint4*>(hrow);
+  for (int q = 0; q < kHistW / 4; q += 8) {
+    uint4 v[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] = p[q + k];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int c = 4 * (q + k);
+      if (v[k].x == 0u) return c;
+      if (v[k].y == 0u) return c + 1;
+      if (v[k].z == 0u) return c + 2;
+      if (v[k].w == 0u) return c + 3;
+    }
+  }
+  return -1;
+}
+
+__device__ __forceinline__ void round_a(const LoopState& S, int r, LoopLds& lds) {
+  uint32_t* red = lds.red;
   const GraphView& G = S.G;
+  const uint32_t* Lb = S.B[(r + 1) & 1];
+  uint32_t* Cb = S.B[r & 1];
+  const uint32_t* Dp = S.Db[(r + 1) & 1];
+  uint32_t* Dn = S.Db[r & 1];
+  const uint32_t* Rp = S.Rb[(r + 1) & 1];
+  const int lane = lane_id();
+  const int wib = threadIdx.x >> 6;
+  int32_t* list = lds.lists[wib];
+  unsigned long long* win = lds.wins[wib];
   RecolorArgs RA{};
   RA.G = G;
-  const int lane = lane_id();
-  unsigned long long* hw = wwin[threadIdx.x >> 6];
-  const int64_t gwave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
-  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
-  const uint32_t n = HUB ? F.cnt[2] : F.cnt[1];
-  for (int64_t i = gwave; i < n; i += nwaves) {
-    int32_t w;
-    if constexpr (HUB) {
-      const int4 it = F.h[i];
-      if (it.y != 0) continue;
-      w = it.x;
-    } else {
-      w = F.m[i];
-    }
-    const Cand c = load_cand(S, P, w);
-    const int role = role_of(src, c);
-    if (role == 0) continue;
-    if (role == 2) {
-      if (lane == 0) leave_member(S, P, w, c);
-      q_push(Q, 2, lane == 0, w, G, P.Dn, S.hub_cap);
-      continue;
-    }
-    int32_t m = c.cm;
-    if (m < 0) {
-      if constexpr (HUB) {
-        m = hist_mex(G.hist + (uint64_t)G.hidx[w] * kHistW);
-        if (m < 0) m = wave_row_mex<M_LOOP_HI>(RA, w, hw);
-      } else {
-        const uint2 rw = G.row[w];
-        if (lane < 8) hw[lane] = 0ull;
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        RegMask mk;
-        mk.clear();
-        mex_slots<M_LOOP_HI>(RA, w, 0, rw.x, lane, rw.y, 64, mk, hw, 512, slots, live);
-        mk.reduce(64);
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        m = mk.first_zero();
-        if (m < 0) {
-          const unsigned long long x = lane >= 4 && lane < 8 ? hw[lane] : 0ull;
-          const unsigned long long b = __ballot(lane >= 4 && lane < 8 && x != ~0ull);
-          const int wi = __ffsll((long long)b) - 1;
-          const unsigned long long word = __shfl(x, wi < 0 ? 0 : wi);
-          m = wi * 64 + __ffsll((long long)~word) - 1;
+  uint32_t members = 0, changed = 0;
+  unsigned long long slots = 0, live = 0, dsum = 0;
+
+  walk_bits(S.words, list, [&](uint32_t i) { return Lb[i] | Dp[i] | Rp[i]; },
+            [&](int32_t* lst, uint32_t total) {
+    uint32_t nw = 0;  // members whose mex is past the histogram, at the front of the list
+    for (uint32_t base = 0; base < total; base += 64) {
+      const uint32_t i = base + lane;
+      const bool valid = i < total;
+      const int32_t w = valid ? lst[i] : 0;
+      int32_t k = 0, cm = 0, so = 0, po = 0;
+      bool inl = false;
+      if (valid) {
+        k = S.K[w];
+        cm = G.mexc[w];
+        so = S.col0[w];
+        po = S.prev[w];
+        inl = has_bit(Lb, w);
+      }
+      const bool member = valid && k > 0;
+      const bool leaving = valid && !member && inl;
+      // a stale mex: the row's colour histogram, or a scan of a short row
+      int32_t m = cm;
+      if (member && cm < 0) {
+        const int32_t h = G.hidx[w];
+        if (h >= 0) {
+          m = lane_hist_mex(G.hist + (uint64_t)h * kHistW);
+        } else {
+          m = lane_small_mex(G, G.row[w], slots);
         }
       }
+      const bool wstale = member && m < 0;  // colours 0..kHistW-1 all present
+      bool dv = leaving;
+      if (leaving) {
+        S.dold[w] = make_int2(so, (po + 1) | (int32_t)0x80000000);
+        set_bit(Dn, w);
+      }
+      if (member && !wstale) dv = commit_member(S, Cb, Dn, w, cm, so, po, inl, m, changed);
+      if (dv) dsum += G.row[w].y;
+      members += member ? 1u : 0u;
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      const unsigned long long wm = __ballot(wstale);
+      if (wstale) lst[nw + lanes_below(wm)] = w;
+      nw += (uint32_t)__popcll(wm);
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
     }
-    bool chv = false, dv = false;
-    if (lane == 0) {
-      commit_member(S, P, w, c, m, chv, dv);
-      changed += chv ? 1u : 0u;
+    // mex past the histogram: one at a time by the whole wave (windowed scan)
+    for (uint32_t q = 0; q < nw; ++q) {
+      const int32_t w = lst[q];
+      const int32_t m = wave_row_mex<M_LOOP_HI>(RA, w, win);
+      if (lane == 0 &&
+          commit_member(S, Cb, Dn, w, -1, S.col0[w], S.prev[w], has_bit(Lb, w), m, changed))
+        dsum += G.row[w].y;
     }
-    q_push(Q, 0, lane == 0, w, G, P.Cn, S.hub_cap);
-    q_push(Q, 1, chv, w, G, P.Xn, S.hub_cap);
-    q_push(Q, 2, dv, w, G, P.Dn, S.hub_cap);
-  }
-}
-
-// Small candidates of one source list: 8-lane groups.
-__device__ __forceinline__ void phase_a_small(const LoopState& S, const RoundA& P, int src,
-                                              const Frontier& F, BlockQueues& Q, uint32_t& changed,
-                                              unsigned long long& slots, unsigned long long& live) {
-  const GraphView& G = S.G;
-  RecolorArgs RA{};
-  RA.G = G;
-  const int lane = lane_id();
-  const int grp = lane >> 3, sub = lane & 7;
-  const int64_t gwave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
-  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
-  const uint32_t nS = F.cnt[0];
-  for (int64_t b = gwave * 8; b < nS; b += nwaves * 8) {
-    const int64_t idx = b + grp;
-    const bool act = idx < nS;
-    const int32_t w = act ? F.s[idx] : 0;
-    Cand c{};
-    if (act) c = load_cand(S, P, w);
-    const int role = act ? role_of(src, c) : 0;
-    const uint2 rw = (role == 1 && c.cm < 0) ? G.row[w] : make_uint2(0u, 0u);
-    RegMask mk;
-    mk.clear();
-    mex_slots<M_LOOP_HI>(RA, w, 0, rw.x, sub, rw.y, kGroup, mk, nullptr, 0, slots, live);
-    unsigned long long mask = mk.m0;
-    mask |= __shfl_xor(mask, 1);
-    mask |= __shfl_xor(mask, 2);
-    mask |= __shfl_xor(mask, 4);
-    const bool head = act && sub == 0;
-    bool chv = false, dv = false;
-    if (head && role == 1) {
-      const int32_t m = c.cm >= 0 ? c.cm : __ffsll((long long)~mask) - 1;
-      commit_member(S, P, w, c, m, chv, dv);
-      changed += chv ? 1u : 0u;
-    } else if (head && role == 2) {
-      leave_member(S, P, w, c);
-      dv = true;
-    }
-    q_push(Q, 0, head && role == 1, w, G, P.Cn, S.hub_cap);
-    q_push(Q, 1, chv, w, G, P.Xn, S.hub_cap);
-    q_push(Q, 2, dv, w, G, P.Dn, S.hub_cap);
-  }
-}
-
-__device__ __forceinline__ void round_a(const LoopState& S, int r) {
-  __shared__ BlockQueues Q;
-  const Frontier Cp = c_list(S, r - 1), Dp = S.D[rot(r - 1)], Rp = S.R[rot(r - 1)];
-  RoundA P;
-  P.Cn = S.C[r % 3];
-  P.Dn = S.D[r % 3];
-  P.Xn = S.X[r % 3];
-  P.cbp = S.B[(r + 1) & 1];
-  P.cb = S.B[r & 1];
-  P.db = S.Db[r & 1];
-  uint32_t* dbp = S.Db[(r + 1) & 1];
-  if (threadIdx.x < 3) Q.n[threadIdx.x] = 0u;
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    S.trace[r].x = (int32_t)fsize_dev(Cp);
-    const Frontier z[4] = {S.C[(r + 1) % 3], S.D[(r + 1) % 3], S.X[(r + 1) % 3], S.R[r % 3]};
-    for (int k = 0; k < 4; ++k)
-      for (int i = 0; i < 4; ++i) z[k].cnt[i] = 0u;
-  }
-  __syncthreads();
-  // D_{r-1}'s bits and R_{r-1}'s claims are no longer read
-  visit_members(Dp, [&](int32_t x, bool valid) {
-    if (valid) dbp[x >> 5] = 0u;
   });
-  visit_members(Rp, [&](int32_t y, bool valid) {
-    if (valid) S.Rb[y >> 5] = 0u;
-  });
-  uint32_t changed = 0;
-  unsigned long long slots = 0, live = 0;
-  const Frontier* src[3] = {&Cp, &Dp, &Rp};
-  for (int k = 0; k < 3; ++k) phase_a_wave<true>(S, P, k, *src[k], Q, changed, slots, live);
-  for (int k = 0; k < 3; ++k) phase_a_wave<false>(S, P, k, *src[k], Q, changed, slots, live);
-  for (int k = 0; k < 3; ++k) phase_a_small(S, P, k, *src[k], Q, changed, slots, live);
-  wave_add_u32(changed, reinterpret_cast<uint32_t*>(&S.trace[r].z));
-  wave_add_u64(slots, &S.ctl->rslots);
-  wave_add_u64(live, &S.ctl->rlive);
-  // flush the queues
-  __syncthreads();
-  const Frontier* dst[3] = {&P.Cn, &P.Xn, &P.Dn};
-  for (int k = 0; k < 3; ++k) {
-    const uint32_t cnt = min(Q.n[k], (uint32_t)kQCap);
-    for (uint32_t base = 0; base < cnt; base += blockDim.x) {
-      const uint32_t i = base + threadIdx.x;
-      const bool valid = i < cnt;
-      push_converged(valid, valid ? Q.v[k][i] : 0, S.G, *dst[k], S.hub_cap);
-    }
+
+  const uint32_t bm = block_sum(members, red);
+  const uint32_t bc = block_sum(changed, red);
+  const unsigned long long bd = block_sum64(dsum, lds.red64);
+  for (int off = 32; off > 0; off >>= 1) {
+    slots += __shfl_xor(slots, off);
+    live += __shfl_xor(live, off);
+  }
+  if (threadIdx.x == 0) {
+    uint32_t* p = S.part + ((size_t)(r & 1) * kMaxBlocks + blockIdx.x) * 2;
+    p[0] = bm;
+    p[1] = bc;
+    S.dpart[(size_t)(r & 1) * kMaxBlocks + blockIdx.x] = bd;
+    if (bm) S.ctl->flag[r & 1] = 1u;
+  }
+  if (lane == 0 && (slots | live)) {
+    atomicAdd(&S.bstat[blockIdx.x * 4 + 0], slots);
+    atomicAdd(&S.bstat[blockIdx.x * 4 + 1], live);
   }
 }
 
 // ---------------------------------------------------------------------------
 // phase C.  For x in D_r and each slot (x, y):
-//   y outside D_r: K[y] += (x's new term) - (x's old term); a rise at a
-//                  non-member of C_r makes y a candidate (R_r);
-//   y in C_r (= L_{r+1}): y's term towards x is part of x's recount.
-__device__ __forceinline__ void round_c(const LoopState& S, int r) {
+//   y outside D_r: K[y] += (x's new term) - (x's old term); a rise makes y a
+//                  candidate (R_r);
+//   y in C_r (= L_{r+1}): y's term towards x is part of x's recount;
+// and if x changed colour, y's slot cache, cached mex and histogram follow.
+// A slot of a neighbour that changed colour is never read here: such
+// neighbours are in D_r.
+struct DState {
+  int32_t x, so, po, sn, pn;
+  bool inl, inc, chg;
+};
+
+__device__ __forceinline__ DState load_d(const LoopState& S, const uint32_t* cb, int32_t x) {
+  DState d;
+  d.x = x;
+  const int2 o = S.dold[x];
+  d.so = o.x;
+  d.po = (o.y & 0x7fffffff) - 1;
+  d.inl = o.y < 0;
+  d.sn = S.col0[x];
+  d.pn = S.prev[x];
+  d.inc = has_bit(cb, x);
+  d.chg = d.sn != d.so;
+  return d;
+}
+
+// One lane's share of a count-pass visit of x's row: slots lo, lo+stride, ...
+// Returns this lane's part of x's recount.
+__device__ __forceinline__ uint32_t count_slots(const LoopState& S, const DState& d, uint64_t rbase,
+                                                uint32_t lo, uint32_t hi, uint32_t stride,
+                                                const uint32_t* cb, const uint32_t* db,
+                                                uint32_t* rn, unsigned long long& slots,
+                                                unsigned long long& live) {
   const GraphView& G = S.G;
-  const Frontier Cr = S.C[r % 3], Dr = S.D[r % 3], Xr = S.X[r % 3], Rr = S.R[r % 3];
-  const Frontier Cp = c_list(S, r - 1);
-  const uint32_t* cb = S.B[r & 1];
-  uint32_t* cbp = S.B[(r + 1) & 1];
-  uint32_t* db = S.Db[r & 1];
-  const uint32_t nC = fsize_dev(Cr);
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    S.trace[r].y = (int32_t)nC;
-    if (nC == 0) S.ctl->done_round = r;
+  uint32_t cnt = 0;
+  for (uint32_t j = lo; j < hi; j += kUnroll * stride) {
+    int32_t y[kUnroll], c[kUnroll];
+#pragma unroll
+    for (int k = 0; k < kUnroll; ++k) {
+      const uint32_t jj = j + k * stride;
+      const bool in = jj < hi;
+      y[k] = in ? G.col[rbase + jj] : -2;
+      c[k] = in ? G.ccol[rbase + jj] : -1;
+    }
+#pragma unroll
+    for (int k = 0; k < kUnroll; ++k) {
+      if (y[k] == -2) continue;
+      ++slots;
+      if (y[k] < 0) continue;
+      ++live;
+      const int32_t yy = y[k];
+      if (yy >= G.n_owned) continue;   // frozen ghost: no count, no cache
+      if (!has_bit(db, yy)) {
+        const bool to = d.inl && marks(d.x, d.so, d.po, yy, c[k]);
+        const bool tn = d.inc && marks(d.x, d.sn, d.pn, yy, c[k]);
+        if (to != tn) {
+          atomicAdd(&S.K[yy], tn ? 1 : -1);
+          if (tn) set_bit(rn, yy);
+        }
+      }
+      if (has_bit(cb, yy) && marks(yy, S.col0[yy], S.prev[yy], d.x, d.sn)) ++cnt;
+      if (d.chg) {
+        const uint64_t tslot = rbase + (j + k * stride);
+        G.ccol[(uint64_t)G.row[yy].x + (uint32_t)G.twin[tslot]] = d.sn;
+        G.mexc[yy] = -1;
+        const int32_t h = G.hidx[yy];
+        if (h >= 0) {
+          uint32_t* hr = G.hist + (uint64_t)h * kHistW;
+          if (d.so >= 0 && d.so < kHistW) atomicSub(&hr[d.so], 1u);
+          if (d.sn < kHistW) atomicAdd(&hr[d.sn], 1u);
+        }
+      }
+    }
   }
-  visit_members(Cp, [&](int32_t x, bool valid) {
-    if (valid) cbp[x >> 5] = 0u;
+  return cnt;
+}
+
+__device__ __forceinline__ void clear_words(uint32_t* b, uint32_t words) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < words; i += gridDim.x * blockDim.x)
+    b[i] = 0u;
+}
+
+// ---- heavy rounds: every K recounted by pull, no count atomics ------------
+// y's count over slots lo, lo+stride, ... of its row: members x of C_r whose
+// marking predicate holds for (x, y).
+template <int U>
+__device__ __forceinline__ uint32_t pull_slots(const LoopState& S, int32_t y, int32_t cy,
+                                               uint64_t rbase, uint32_t lo, uint32_t hi,
+                                               uint32_t stride, const uint32_t* cb,
+                                               unsigned long long& slots) {
+  const GraphView& G = S.G;
+  uint32_t cnt = 0;
+  for (uint32_t j = lo; j < hi; j += U * stride) {
+    int32_t x[U];
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+      const uint32_t jj = j + k * stride;
+      x[k] = jj < hi ? G.col[rbase + jj] : -2;
+    }
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+      if (x[k] == -2) continue;
+      ++slots;
+      const int32_t xx = x[k];
+      if (xx >= 0 && xx < G.n_owned && has_bit(cb, xx) && marks(xx, S.col0[xx], S.prev[xx], y, cy))
+        ++cnt;
+    }
+  }
+  return cnt;
+}
+
+__device__ __forceinline__ void set_count(const LoopState& S, int32_t y, uint32_t cnt,
+                                          const uint32_t* cb, uint32_t* rn) {
+  S.K[y] = (int32_t)cnt;
+  if (cnt && !has_bit(cb, y)) set_bit(rn, y);
+}
+
+__device__ __forceinline__ void pull_all(const LoopState& S, const uint32_t* cb, uint32_t* rn,
+                                         LoopLds& lds, unsigned long long& slots) {
+  const GraphView& G = S.G;
+  const int lane = lane_id();
+  int32_t* lst = lds.lists[threadIdx.x >> 6];
+  const uint32_t gwave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
+  // short rows by their lane, mid rows by the wave (vertices strided over
+  // waves so the dense low IDs spread out)
+  for (uint64_t base = 0; base < (uint64_t)G.n_owned; base += 64ull * nwaves) {
+    const uint64_t vi = base + (uint64_t)lane * nwaves + gwave;
+    const bool valid = vi < (uint64_t)G.n_owned;
+    const int32_t y = (int32_t)vi;
+    const uint2 rw = valid ? G.row[y] : make_uint2(0u, 0u);
+    const bool small = valid && rw.y < kSmallLen;
+    const bool mid = valid && rw.y >= kSmallLen && rw.y < kMidLen;
+    if (small) set_count(S, y, pull_slots<8>(S, y, S.col0[y], rw.x, 0, rw.y, 1, cb, slots), cb, rn);
+    const unsigned long long mm = __ballot(mid);
+    if (mid) lst[lanes_below(mm)] = y;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    const uint32_t nm = (uint32_t)__popcll(mm);
+    for (uint32_t q = 0; q < nm; ++q) {
+      const int32_t x = lst[q];
+      const uint2 r2 = G.row[x];
+      uint32_t cnt = pull_slots<4>(S, x, S.col0[x], r2.x, lane, r2.y, 64, cb, slots);
+      for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off);
+      if (lane == 0) set_count(S, x, cnt, cb, rn);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+  }
+  // hub rows: one workgroup per hub (the chunk table's first entries)
+  for (uint32_t i = blockIdx.x; i < S.n_hch; i += gridDim.x) {
+    const int2 e = S.hch[i];
+    if (e.y != 0) continue;
+    const int32_t y = e.x;
+    const uint2 rw = G.row[y];
+    if (rw.y < kMidLen) continue;  // went by the short/mid pass (uniform)
+    const uint32_t cnt = pull_slots<4>(S, y, S.col0[y], rw.x, threadIdx.x, rw.y, kBlock, cb, slots);
+    const uint32_t tot = block_sum(cnt, lds.red);
+    if (threadIdx.x == 0) set_count(S, y, tot, cb, rn);
+  }
+}
+
+// The new colours of D_r's members that changed into the slot cache, mex
+// cache and histograms of their neighbours.
+__device__ __forceinline__ void propagate_changed(const LoopState& S, uint32_t* db, LoopLds& lds) {
+  const GraphView& G = S.G;
+  const int lane = lane_id();
+  int32_t* list = lds.lists[threadIdx.x >> 6];
+  walk_bits(S.words, list, [&](uint32_t i) { return db[i]; }, [&](int32_t* lst, uint32_t total) {
+    uint32_t nw = 0;
+    for (uint32_t base = 0; base < total; base += 64) {
+      const uint32_t i = base + lane;
+      const bool valid = i < total;
+      const int32_t x = valid ? lst[i] : 0;
+      bool chg = false;
+      int32_t so = 0, sn = 0;
+      uint2 rw = make_uint2(0u, 0u);
+      if (valid) {
+        so = S.dold[x].x;
+        sn = S.col0[x];
+        chg = so != sn;
+        if (chg) rw = G.row[x];
+      }
+      if (chg && rw.y < kSmallLen) propagate_slots(G, sn, so, rw.x, 0, rw.y, 1);
+      const bool mid = chg && rw.y >= kSmallLen && rw.y < kMidLen;
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      const unsigned long long mm = __ballot(mid);
+      if (mid) lst[nw + lanes_below(mm)] = x;
+      nw += (uint32_t)__popcll(mm);
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+    }
+    for (uint32_t q = 0; q < nw; ++q) {
+      const int32_t x = lst[q];
+      const uint2 rw = G.row[x];
+      propagate_slots(G, S.col0[x], S.dold[x].x, rw.x, lane, rw.y, 64);
+    }
   });
-  if (nC == 0) {
-    visit_members(Dr, [&](int32_t x, bool valid) {
-      if (valid) db[x >> 5] = 0u;
-    });
+  const uint32_t gwave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
+  for (uint64_t base = 0; base < S.n_hch; base += 64ull * nwaves) {
+    const uint64_t i = base + (uint64_t)lane * nwaves + gwave;
+    const int2 e = i < S.n_hch ? S.hch[i] : make_int2(0, -1);
+    const bool act = e.y >= 0 && has_bit(db, e.x) && S.col0[e.x] != S.dold[e.x].x;
+    unsigned long long am = __ballot(act);
+    while (am) {
+      const int l = __ffsll((long long)am) - 1;
+      am &= am - 1ull;
+      const int32_t x = __shfl(e.x, l);
+      const uint32_t c = (uint32_t)__shfl(e.y, l);
+      const uint2 rw = G.row[x];
+      const uint32_t lo = c * kChunk;
+      if (rw.y < kMidLen || lo >= rw.y) continue;
+      propagate_slots(G, S.col0[x], S.dold[x].x, rw.x, lo + lane, min(rw.y, lo + kChunk), 64);
+    }
+  }
+}
+
+__device__ __forceinline__ void round_c(const LoopState& S, int r, LoopLds& lds) {
+  uint32_t* red = lds.red;
+  const GraphView& G = S.G;
+  const uint32_t* cb = S.B[r & 1];
+  uint32_t* db = S.Db[r & 1];
+  uint32_t* rn = S.Rb[r & 1];
+  const bool nonempty = S.ctl->flag[r & 1] != 0u;
+  const int lane = lane_id();
+  int32_t* list = lds.lists[threadIdx.x >> 6];
+  // every workgroup sums the same partials, so the choice is uniform
+  bool heavy;
+  {
+    unsigned long long d = 0;
+    for (uint32_t b = threadIdx.x; b < gridDim.x; b += blockDim.x)
+      d += S.dpart[(size_t)(r & 1) * kMaxBlocks + b];
+    heavy = block_sum64(d, lds.red64) > S.heavy_slots;
+  }
+
+  // round r's counts into the trace; the flag of round r+1 starts clear
+  if (blockIdx.x == 0) {
+    uint32_t m = 0, c = 0;
+    for (uint32_t b = threadIdx.x; b < gridDim.x; b += blockDim.x) {
+      const uint32_t* p = S.part + ((size_t)(r & 1) * kMaxBlocks + b) * 2;
+      m += p[0];
+      c += p[1];
+    }
+    m = block_sum(m, red);
+    c = block_sum(c, red);
+    if (threadIdx.x == 0) {
+      S.trace[r].x = r == 0 ? S.ctl->nl0 : S.trace[r - 1].y;
+      S.trace[r].y = (int32_t)m;
+      S.trace[r].z = (int32_t)c;
+      S.ctl->flag[(r + 1) & 1] = 0u;
+      if (!nonempty) S.ctl->done_round = r;
+    }
+  }
+  // L_r, D_{r-1} and R_{r-1} are no longer read
+  clear_words(S.B[(r + 1) & 1], S.words);
+  clear_words(S.Db[(r + 1) & 1], S.words);
+  clear_words(S.Rb[(r + 1) & 1], S.words);
+  if (!nonempty) {
+    clear_words(db, S.words);
     return;
   }
   unsigned long long slots = 0, live = 0;
-  visit_rows(G, Dr, [&](int32_t x, uint2 rw, uint32_t lo, uint32_t hi, uint32_t stride, int width) {
-    int32_t so = 0, po = 0, sn = 0, pn = 0;
-    bool inl = false, inc = false;
-    if (x >= 0) {
-      const int2 o = S.dold[x];
-      so = o.x;
-      po = (o.y & 0x7fffffff) - 1;
-      inl = o.y < 0;
-      sn = S.col0[x];
-      pn = S.prev[x];
-      inc = has_bit(cb, x);
-    }
-    uint32_t cnt = 0;
-    for (uint32_t j = lo; j < hi; j += kUnroll * stride) {
-      int32_t y[kUnroll], c[kUnroll];
-#pragma unroll
-      for (int k = 0; k < kUnroll; ++k) {
-        const uint32_t jj = j + k * stride;
-        const bool in = jj < hi;
-        y[k] = in ? G.col[(uint64_t)rw.x + jj] : -2;
-        c[k] = in ? G.ccol[(uint64_t)rw.x + jj] : -1;
+  if (heavy) {
+    pull_all(S, cb, rn, lds, slots);
+    propagate_changed(S, db, lds);
+  } else {
+  // short and mid rows of D_r (hubs go by the chunk table below)
+  walk_bits(S.words, list, [&](uint32_t i) { return db[i]; }, [&](int32_t* lst, uint32_t total) {
+    uint32_t nw = 0;
+    for (uint32_t base = 0; base < total; base += 64) {
+      const uint32_t i = base + lane;
+      const bool valid = i < total;
+      const int32_t x = valid ? lst[i] : 0;
+      const uint2 rw = valid ? G.row[x] : make_uint2(0u, 0u);
+      const bool small = valid && rw.y < kSmallLen;
+      const bool mid = valid && rw.y >= kSmallLen && rw.y < kMidLen;
+      if (small) {
+        const DState d = load_d(S, cb, x);
+        S.K[x] = (int32_t)count_slots(S, d, rw.x, 0, rw.y, 1, cb, db, rn, slots, live);
       }
-#pragma unroll
-      for (int k = 0; k < kUnroll; ++k) {
-        slots += y[k] != -2 ? 1ull : 0ull;
-        bool pushr = false;
-        if (y[k] >= 0) {
-          ++live;
-          const int32_t yy = y[k];
-          if (yy < G.n_owned && !has_bit(db, yy)) {
-            // y's colour is unchanged this round, so its slot value is current
-            const bool to = inl && marks(x, so, po, yy, c[k]);
-            const bool tn = inc && marks(x, sn, pn, yy, c[k]);
-            if (to != tn) {
-              atomicAdd(&S.K[yy], tn ? 1 : -1);
-              pushr = tn && !has_bit(cb, yy) && claim_bit(S.Rb, yy);
-            }
-          }
-          if (has_bit(cb, yy) && marks(yy, S.col0[yy], S.prev[yy], x, sn)) ++cnt;
-        }
-        frontier_push(pushr, y[k], G, Rr, S.hub_cap);
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      const unsigned long long mm = __ballot(mid);
+      if (mid) lst[nw + lanes_below(mm)] = x;
+      nw += (uint32_t)__popcll(mm);
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+    }
+    for (uint32_t q = 0; q < nw; ++q) {
+      const int32_t x = lst[q];
+      const uint2 rw = G.row[x];
+      const DState d = load_d(S, cb, x);
+      uint32_t cnt = count_slots(S, d, rw.x, lane, rw.y, 64, cb, db, rn, slots, live);
+      for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off);
+      if (lane == 0) S.K[x] = (int32_t)cnt;
+    }
+  });
+  // hub rows of D_r, one wave per kChunk-slot chunk (K[x] was zeroed in A).
+  // A hub's chunks are adjacent in the table; lane l of wave w looks at entry
+  // l * nwaves + w, so they go to different waves.
+  {
+    const uint32_t gwave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
+    for (uint64_t base = 0; base < S.n_hch; base += 64ull * nwaves) {
+      const uint64_t i = base + (uint64_t)lane * nwaves + gwave;
+      const int2 e = i < S.n_hch ? S.hch[i] : make_int2(0, -1);
+      const bool act = e.y >= 0 && has_bit(db, e.x);
+      unsigned long long am = __ballot(act);
+      while (am) {
+        const int l = __ffsll((long long)am) - 1;
+        am &= am - 1ull;
+        const int32_t x = __shfl(e.x, l);
+        const uint32_t c = (uint32_t)__shfl(e.y, l);
+        const uint2 rw = G.row[x];
+        const uint32_t lo = c * kChunk;
+        // rows now shorter than kMidLen went by the short/mid path
+        if (rw.y < kMidLen || lo >= rw.y) continue;
+        const DState d = load_d(S, cb, x);
+        uint32_t cnt = count_slots(S, d, rw.x, lo + lane, min(rw.y, lo + kChunk), 64, cb, db, rn,
+                                   slots, live);
+        for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off);
+        if (lane == 0 && cnt) atomicAdd(&S.K[x], (int32_t)cnt);
       }
     }
-    cnt = group_sum(cnt, width);
-    if ((lane_id() & (width - 1)) == 0 && x >= 0 && cnt) atomicAdd(&S.K[x], (int32_t)cnt);
-  });
-  // the new colours into the slot cache (the count pass above never reads a
-  // slot of a changed neighbour: changed vertices are in D_r)
-  visit_rows(G, Xr, [&](int32_t v, uint2 rw, uint32_t lo, uint32_t hi, uint32_t stride, int width) {
-    (void)width;
-    if (v >= 0) propagate_slots(G, S.col0[v], S.dold[v].x, rw.x, lo, hi, stride);
-  });
-  wave_add_u64(slots, &S.ctl->uslots);
-  wave_add_u64(live, &S.ctl->ulive);
+  }
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    slots += __shfl_xor(slots, off);
+    live += __shfl_xor(live, off);
+  }
+  if (lane == 0 && (slots | live)) {
+    atomicAdd(&S.bstat[blockIdx.x * 4 + 2], slots);
+    atomicAdd(&S.bstat[blockIdx.x * 4 + 3], live);
+  }
 }
 
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_loop_init_bits(LoopState S) { init_bits(S); }
-__global__ __launch_bounds__(kBlock) void k_loop_init_push(LoopState S) { init_push(S); }
-
 __global__ __launch_bounds__(kBlock) void k_round_a(LoopState S, int r) {
+  __shared__ LoopLds lds;
   if (S.ctl->done_round < r) return;
-  round_a(S, r);
+  round_a(S, r, lds);
 }
 
 __global__ __launch_bounds__(kBlock) void k_round_c(LoopState S, int r) {
+  __shared__ LoopLds lds;
   if (S.ctl->done_round < r) return;
-  round_c(S, r);
+  round_c(S, r, lds);
 }
 
-// Persistent form: every round in one cooperative launch.  The exit test
-// reads the control word after a grid barrier, so every workgroup takes the
-// same branch.
+// Persistent form: rounds r0..r1-1 in one cooperative launch.  The exit test
+// reads a flag written before the last grid barrier, so every workgroup
+// takes the same branch.
 __global__ __launch_bounds__(kBlock) void k_loop_persistent(LoopState S, int r0, int r1) {
   namespace cg = cooperative_groups;
   cg::grid_group grid = cg::this_grid();
-  if (r0 == 0) {
-    init_bits(S);
-    grid.sync();
-    init_push(S);
-    grid.sync();
-  }
+  __shared__ LoopLds lds;
   for (int r = r0; r < r1; ++r) {
-    round_a(S, r);
+    round_a(S, r, lds);
     grid.sync();
-    round_c(S, r);
+    round_c(S, r, lds);
     grid.sync();
-    if (__atomic_load_n(&S.ctl->done_round, __ATOMIC_RELAXED) != INT_MAX) break;
+    if (S.ctl->done_round != INT_MAX) break;
   }
 }
 
-// K, the D/R bitmaps and the C/L bitmaps are zero between batches.
+// K and the loop's bitmaps are zero between batches.
 __global__ void k_zero_check(const int32_t* K, size_t nk, const uint32_t* bits, size_t words,
                              unsigned long long* bad) {
   unsigned long long cnt = 0;
@@ -601,40 +756,23 @@ static IncState* inc_state(Graph* g) { return static_cast<IncState*>(g->inc); }
 static int inc_alloc(Graph* g, IncState* s) {
   const size_t n = (size_t)g->n + 1;
   LoopState& S = s->S;
-  s->words = n / 32 + 2;
+  S.words = (uint32_t)g->cbits_words;
   DC_HIP(hipMalloc(&S.K, sizeof(int32_t) * n));
   DC_HIP(hipMalloc(&S.dold, sizeof(int2) * n));
-  DC_HIP(hipMalloc(&s->bits, sizeof(uint32_t) * 3 * s->words));
+  DC_HIP(hipMalloc(&s->bits, sizeof(uint32_t) * 4 * (size_t)S.words));
   S.Db[0] = s->bits;
-  S.Db[1] = s->bits + s->words;
-  S.Rb = s->bits + 2 * s->words;
-  DC_HIP(hipMalloc(&s->lbuf, sizeof(int32_t) * n * 24));
-  DC_HIP(hipMalloc(&s->cnt, sizeof(uint32_t) * 48));
+  S.Db[1] = s->bits + S.words;
+  S.Rb[0] = s->bits + 2 * (size_t)S.words;
+  S.Rb[1] = s->bits + 3 * (size_t)S.words;
+  DC_HIP(hipMalloc(&S.part, sizeof(uint32_t) * 4 * kMaxBlocks));
+  DC_HIP(hipMalloc(&S.dpart, sizeof(unsigned long long) * 2 * kMaxBlocks));
+  DC_HIP(hipMalloc(&S.bstat, sizeof(unsigned long long) * 4 * kMaxBlocks));
   DC_HIP(hipMalloc(&S.ctl, sizeof(LoopCtl)));
   DC_HIP(hipHostMalloc(&s->hctl, sizeof(LoopCtl), hipHostMallocDefault));
+  s->hstat = new unsigned long long[4 * kMaxBlocks];
   DC_HIP(hipMemsetAsync(S.K, 0, sizeof(int32_t) * n, g->stream));
-  DC_HIP(hipMemsetAsync(s->bits, 0, sizeof(uint32_t) * 3 * s->words, g->stream));
-  Frontier* fr[12] = {&S.C[0], &S.C[1], &S.C[2], &S.D[0], &S.D[1], &S.D[2],
-                      &S.X[0], &S.X[1], &S.X[2], &S.R[0], &S.R[1], &S.R[2]};
-  for (int i = 0; i < 12; ++i) {
-    fr[i]->s = s->lbuf + n * (2 * i);
-    fr[i]->m = s->lbuf + n * (2 * i + 1);
-    fr[i]->cnt = s->cnt + 4 * i;
-  }
-  return DC_OK;
-}
-
-static int inc_hubs(Graph* g, IncState* s) {
-  if (s->hcap >= g->hchunk_cap && s->hb) return DC_OK;
-  if (s->hb) DC_HIP(hipFree(s->hb));
-  s->hb = nullptr;
-  const uint32_t c = g->hchunk_cap;
-  DC_HIP(hipMalloc(&s->hb, sizeof(int4) * (size_t)c * 12));
-  s->hcap = c;
-  LoopState& S = s->S;
-  Frontier* fr[12] = {&S.C[0], &S.C[1], &S.C[2], &S.D[0], &S.D[1], &S.D[2],
-                      &S.X[0], &S.X[1], &S.X[2], &S.R[0], &S.R[1], &S.R[2]};
-  for (int i = 0; i < 12; ++i) fr[i]->h = s->hb + (size_t)c * i;
+  DC_HIP(hipMemsetAsync(s->bits, 0, sizeof(uint32_t) * 4 * (size_t)S.words, g->stream));
+  DC_HIP(hipMemsetAsync(S.part, 0, sizeof(uint32_t) * 4 * kMaxBlocks, g->stream));
   return DC_OK;
 }
 
@@ -661,9 +799,10 @@ void loop_free(Graph* g) {
   auto f = [](void* p) {
     if (p) (void)hipFree(p);
   };
-  f(s->S.K); f(s->S.dold); f(s->bits); f(s->lbuf); f(s->hb); f(s->cnt); f(s->S.trace);
+  f(s->S.K); f(s->S.dold); f(s->bits); f(s->S.part); f(s->S.dpart); f(s->S.bstat); f(s->S.trace);
   f(s->S.ctl);
   if (s->hctl) (void)hipHostFree(s->hctl);
+  delete[] s->hstat;
   delete s;
   g->inc = nullptr;
 }
@@ -676,7 +815,7 @@ int loop_check(Graph* g, int64_t* bad) {
   IncState* s = inc_state(g);
   if (s)
     hipLaunchKernelGGL(k_zero_check, dim3(1024), dim3(kBlock), 0, g->stream, s->S.K,
-                       (size_t)g->n, s->bits, 3 * s->words, cnt);
+                       (size_t)g->n, s->bits, 4 * (size_t)s->S.words, cnt);
   DC_HIP(hipGetLastError());
   unsigned long long h = 0;
   DC_HIP(hipMemcpyAsync(&h, cnt, sizeof(h), hipMemcpyDeviceToHost, g->stream));
@@ -687,7 +826,7 @@ int loop_check(Graph* g, int64_t* bad) {
 
 static int loop_reset(Graph* g, IncState* s) {
   DC_HIP(hipMemsetAsync(s->S.K, 0, sizeof(int32_t) * ((size_t)g->n + 1), g->stream));
-  DC_HIP(hipMemsetAsync(s->bits, 0, sizeof(uint32_t) * 3 * s->words, g->stream));
+  DC_HIP(hipMemsetAsync(s->bits, 0, sizeof(uint32_t) * 4 * (size_t)s->S.words, g->stream));
   DC_HIP(hipMemsetAsync(g->cbits[0], 0, sizeof(uint32_t) * 2 * g->cbits_words, g->stream));
   return DC_OK;
 }
@@ -701,11 +840,11 @@ static int persistent_grid(Graph* g, int* grid) {
     set_error("persistent loop kernel does not fit on a CU");
     return DC_ERR_HIP;
   }
-  *grid = std::max(1, std::min(bpm, g->fused_blocks_per_cu)) * cus;
+  *grid = std::min(kMaxBlocks, std::max(1, std::min(bpm, g->fused_blocks_per_cu)) * cus);
   return DC_OK;
 }
 
-int loop_hi(Graph* g, Frontier L0, bool persistent, LoopResult* res) {
+int loop_hi(Graph* g, Frontier L0, int32_t nl0, bool persistent, LoopResult* res) {
   IncState* s = inc_state(g);
   if (!s) {
     s = new IncState();
@@ -713,10 +852,9 @@ int loop_hi(Graph* g, Frontier L0, bool persistent, LoopResult* res) {
     int rc = inc_alloc(g, s);
     if (rc) return rc;
   }
-  int rc = inc_hubs(g, s);
+  int rc = inc_trace(g, s, 256);
   if (rc) return rc;
   const int32_t max_rounds = g->opt.max_rounds;
-  if ((rc = inc_trace(g, s, 256))) return rc;
 
   LoopState& S = s->S;
   S.G = g->view();
@@ -726,83 +864,66 @@ int loop_hi(Graph* g, Frontier L0, bool persistent, LoopResult* res) {
   S.B[0] = g->cbits[0];
   S.B[1] = g->cbits[1];
   S.L0 = L0;
-  S.hub_cap = g->hub_cap;
+  S.hch = g->hch;
+  S.n_hch = g->n_hch;
+  // rounds whose D rows hold more than 1/32 of all slots recount by pull
+  S.heavy_slots = std::max<unsigned long long>(g->rows_end / 32, 1ull << 16);
 
-  DC_HIP(hipMemsetAsync(s->cnt, 0, sizeof(uint32_t) * 48, g->stream));
   *s->hctl = LoopCtl{};
   s->hctl->done_round = INT_MAX;
+  s->hctl->nl0 = nl0;
   DC_HIP(hipMemcpyAsync(S.ctl, s->hctl, sizeof(LoopCtl), hipMemcpyHostToDevice, g->stream));
-  int64_t launches = 0;
+  DC_HIP(hipMemsetAsync(S.bstat, 0, sizeof(unsigned long long) * 4 * kMaxBlocks, g->stream));
+  const dim3 blk(kBlock);
+  hipLaunchKernelGGL(k_loop_init, dim3(kMaxBlocks), blk, 0, g->stream, S);
+  DC_HIP(hipGetLastError());
+  int64_t launches = 1;
+  int grid = kMaxBlocks;
+  if (persistent && (rc = persistent_grid(g, &grid))) return rc;
   double kms = 0.0;
   hipEvent_t e0 = g->ev[2], e1 = g->ev[3];
-  int32_t done = INT_MAX;
-
-  if (persistent) {
-    // rounds in launches of up to 4096 (the trace grows between launches)
-    int grid = 0;
-    if ((rc = persistent_grid(g, &grid))) return rc;
-    int32_t r0 = 0;
-    while (r0 < max_rounds && done == INT_MAX) {
-      int32_t r1 = std::min(max_rounds, r0 + 4096);
-      if ((rc = inc_trace(g, s, r1))) return rc;
-      DC_HIP(hipMemsetAsync(S.trace + r0, 0, sizeof(int4) * (size_t)(r1 - r0), g->stream));
+  int32_t rounds = -1, r = 0, block = persistent ? 4096 : 8;
+  while (r < max_rounds && rounds < 0) {
+    const int32_t nb = std::min(block, max_rounds - r);
+    if ((rc = inc_trace(g, s, r + nb))) return rc;
+    if (g->profiling) DC_HIP(hipEventRecord(e0, g->stream));
+    if (persistent) {
+      int32_t r0 = r, r1 = r + nb;
       void* args[] = {&S, &r0, &r1};
-      if (g->profiling) DC_HIP(hipEventRecord(e0, g->stream));
-      DC_HIP(hipLaunchCooperativeKernel((const void*)k_loop_persistent, dim3(grid), dim3(kBlock), args,
-                                        0, g->stream));
-      if (g->profiling) DC_HIP(hipEventRecord(e1, g->stream));
-      DC_HIP(hipMemcpyAsync(s->hctl, S.ctl, sizeof(LoopCtl), hipMemcpyDeviceToHost, g->stream));
-      DC_HIP(hipStreamSynchronize(g->stream));
-      if (g->profiling) {
-        float x = 0.f;
-        DC_HIP(hipEventElapsedTime(&x, e0, e1));
-        kms += x;
-      }
+      DC_HIP(hipLaunchCooperativeKernel((const void*)k_loop_persistent, dim3(grid), blk, args, 0,
+                                        g->stream));
       ++launches;
-      done = s->hctl->done_round;
-      r0 = r1;
-    }
-  } else {
-    const dim3 grid(2048), blk(kBlock);
-    hipLaunchKernelGGL(k_loop_init_bits, grid, blk, 0, g->stream, S);
-    hipLaunchKernelGGL(k_loop_init_push, grid, blk, 0, g->stream, S);
-    DC_HIP(hipGetLastError());
-    launches = 2;
-    int32_t r = 0, block = 8;
-    while (r < max_rounds) {
-      const int32_t nb = std::min(block, max_rounds - r);
-      if ((rc = inc_trace(g, s, r + nb))) return rc;
-      DC_HIP(hipMemsetAsync(S.trace + r, 0, sizeof(int4) * (size_t)nb, g->stream));
-      if (g->profiling) DC_HIP(hipEventRecord(e0, g->stream));
-      for (int32_t i = 0; i < nb; ++i, ++r) {
-        hipLaunchKernelGGL(k_round_a, grid, blk, 0, g->stream, S, r);
-        hipLaunchKernelGGL(k_round_c, grid, blk, 0, g->stream, S, r);
-        launches += 2;
+    } else {
+      for (int32_t i = 0; i < nb; ++i) {
+        hipLaunchKernelGGL(k_round_a, dim3(grid), blk, 0, g->stream, S, r + i);
+        hipLaunchKernelGGL(k_round_c, dim3(grid), blk, 0, g->stream, S, r + i);
       }
       DC_HIP(hipGetLastError());
-      if (g->profiling) DC_HIP(hipEventRecord(e1, g->stream));
-      DC_HIP(hipMemcpyAsync(s->hctl, S.ctl, sizeof(LoopCtl), hipMemcpyDeviceToHost, g->stream));
-      DC_HIP(hipStreamSynchronize(g->stream));
-      if (g->profiling) {
-        float x = 0.f;
-        DC_HIP(hipEventElapsedTime(&x, e0, e1));
-        kms += x;
-      }
-      done = s->hctl->done_round;
-      if (done != INT_MAX) break;
-      block = std::min(block * 2, 32);
+      launches += 2 * nb;
     }
+    if (g->profiling) DC_HIP(hipEventRecord(e1, g->stream));
+    DC_HIP(hipMemcpyAsync(s->hctl, S.ctl, sizeof(LoopCtl), hipMemcpyDeviceToHost, g->stream));
+    DC_HIP(hipStreamSynchronize(g->stream));
+    if (g->profiling) {
+      float x = 0.f;
+      DC_HIP(hipEventElapsedTime(&x, e0, e1));
+      kms += x;
+    }
+    r += nb;
+    if (s->hctl->done_round != INT_MAX) rounds = s->hctl->done_round + 1;
+    block = std::min(block * 2, persistent ? 4096 : 32);
   }
-  if (done == INT_MAX) {
+  if (rounds < 0) {
     int rc2 = loop_reset(g, s);
     if (rc2) return rc2;
     set_error("recolor loop hit max_rounds");
     return DC_ERR_ROUNDS;
   }
-  const int32_t rounds = done + 1;
   std::vector<int4> tr((size_t)rounds);
   DC_HIP(hipMemcpyAsync(tr.data(), S.trace, sizeof(int4) * (size_t)rounds, hipMemcpyDeviceToHost,
                         g->stream));
+  DC_HIP(hipMemcpyAsync(s->hstat, S.bstat, sizeof(unsigned long long) * 4 * kMaxBlocks,
+                        hipMemcpyDeviceToHost, g->stream));
   DC_HIP(hipStreamSynchronize(g->stream));
   res->rounds = rounds;
   for (int32_t i = 0; i < rounds; ++i) {
@@ -812,10 +933,12 @@ int loop_hi(Graph* g, Frontier L0, bool persistent, LoopResult* res) {
     res->evaluated += (uint64_t)tr[i].y;
     res->changed += (uint64_t)tr[i].z;
   }
-  res->rslots = s->hctl->rslots;
-  res->rlive = s->hctl->rlive;
-  res->slots = s->hctl->uslots;
-  res->live = s->hctl->ulive;
+  for (int b = 0; b < kMaxBlocks; ++b) {
+    res->rslots += s->hstat[4 * b + 0];
+    res->rlive += s->hstat[4 * b + 1];
+    res->slots += s->hstat[4 * b + 2];
+    res->live += s->hstat[4 * b + 3];
+  }
   res->kernel_ms = kms;
   res->launches = launches;
   return DC_OK;

@@ -388,11 +388,14 @@ int recolor_plain(Graph* g, int rule, dc_stats* st, bool persistent_loop) {
   // recolor loop: L = marked set, then L <- C every round
   Frontier L = Mk;
   DC_HIP(hipMemsetAsync(g->cbits[0], 0, sizeof(uint32_t) * 2 * g->cbits_words, g->stream));
+  DC_HIP(hipMemcpyAsync(rb.cnt, Mk.cnt, sizeof(uint32_t) * 4, hipMemcpyDeviceToHost, g->stream));
+  DC_HIP(hipStreamSynchronize(g->stream));
+  uint64_t nL = fsize(rb.cnt);
   if (rule == DC_RULE_INDEPENDENT && change) {
     // candidate counts kept incrementally (loop.hip); both colour buffers
     // are written there, so no catch-up follows
     LoopResult lr;
-    if ((rc = loop_hi(g, Mk, persistent_loop, &lr))) return rc;
+    if ((rc = loop_hi(g, Mk, (int32_t)nL, persistent_loop, &lr))) return rc;
     evaluated += lr.evaluated;
     changed_total += lr.changed;
     frontier_sum += lr.frontier_sum;
@@ -422,9 +425,6 @@ int recolor_plain(Graph* g, int rule, dc_stats* st, bool persistent_loop) {
     }
     return DC_OK;
   }
-  DC_HIP(hipMemcpyAsync(rb.cnt, Mk.cnt, sizeof(uint32_t) * 4, hipMemcpyDeviceToHost, g->stream));
-  DC_HIP(hipStreamSynchronize(g->stream));
-  uint64_t nL = fsize(rb.cnt);
   int cidx = 0;
   int32_t rounds = 0;
   while (change) {

@@ -56,7 +56,9 @@ typedef struct {
   int32_t max_rounds;  /* recolor loop cap (default 1000000)                          */
   int32_t driver;      /* DC_DRIVER_PLAIN (default) or DC_DRIVER_FUSED                */
   int32_t device;      /* HIP device ordinal (default 0)                              */
-  int32_t reserved[3];
+  int32_t spare_slots; /* spare tail for rows that outgrow their capacity; 0 = half
+                          the row slots, at least 2^20 (default 0)                    */
+  int32_t reserved[2];
 } dc_options;
 
 /* Per-batch counters.  Byte counts follow the algorithmic model: each visited

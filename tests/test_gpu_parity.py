@@ -149,12 +149,15 @@ def test_rmat_batch_stream(oracle_mod, rule, scale, nb):
     assert g.validate()[0] == 0
 
 
-def test_regrow_path(oracle_mod):
-    # no spare capacity at all: every insertion overflows its row
-    say("regrow")
+@pytest.mark.parametrize("spare", [0, 16])
+def test_regrow_path(oracle_mod, spare):
+    # no spare row capacity: insertions overflow their rows, which move into
+    # the spare tail (spare=0: default size) or, once a 16-slot tail is used
+    # up, make every row relaid out
+    say("regrow", spare)
     u, v = d.gen_er(3000, 9000, 8)
     batches = d.gen_batches(3000, u, v, 3, 1500, 0.8, 3)
-    opts = d.options(slack_frac=0.0, slack_min=1)
+    opts = d.options(slack_frac=0.0, slack_min=1, spare_slots=spare)
     g, og, st = run_both(oracle_mod, 3000, u, v, batches, 0, opts=opts)
     assert sum(s.regrows for s in st) >= 1
     for x in range(0, 3000, 97):
