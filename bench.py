@@ -67,6 +67,21 @@ def roofline(kernel, algo_bytes, kernel_ms, launches):
             "algorithmic_bytes": int(algo_bytes)}
 
 
+def traffic_for(kernel):
+    """HBM bytes per launch of the kernel from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel factor), or
+    None when no summary covers it."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        "pmc_traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    ent = t.get(kernel.split(" ")[0])
+    return None if ent is None else ent.get("bytes_per_launch")
+
+
 def set_driver(g, d, args):
     g.set_driver(d.DRIVER_FUSED if args.driver == "fused" else d.DRIVER_PLAIN, args.blocks_per_cu)
 
@@ -74,7 +89,8 @@ def set_driver(g, d, args):
 # ----------------------------------------------------------------- c1 / c2 --
 def run_stream(args, d, rule, n, u, v, workload):
     """One graph, a stream of device-resident batches (c1, c2)."""
-    nb = args.warmup + args.steps
+    nprof = 2  # batches after the timed ones, run with per-launch events
+    nb = args.warmup + args.steps + nprof
     t = time.perf_counter()
     batches = d.gen_batches(n, u, v, nb, args.batch, 0.5, args.seed + 1000)
     log(f"{nb} batches of {args.batch} ({time.perf_counter() - t:.1f}s)")
@@ -86,7 +102,7 @@ def run_stream(args, d, rule, n, u, v, workload):
     log(f"device graph + static coloring: {time.perf_counter() - t:.1f}s, "
         f"colors {maxc0 + 1}, monochromatic {mono0}")
     dev_batches = [g.upload_batch(*b) for b in batches]
-    g.set_profiling(True)
+    g.set_profiling(False)
     set_driver(g, d, args)
 
     colors_after_first = None
@@ -101,7 +117,7 @@ def run_stream(args, d, rule, n, u, v, workload):
     tot = {"bytes": 0, "kms": 0.0, "launches": 0, "rounds": 0, "changed": 0, "evaluated": 0,
            "slots": 0, "regrows": 0}
     t0 = time.perf_counter()
-    for i in range(args.warmup, nb):
+    for i in range(args.warmup, args.warmup + args.steps):
         tb = time.perf_counter()
         g.apply_device_batch(dev_batches[i])
         st = g.recolor(rule)
@@ -116,6 +132,21 @@ def run_stream(args, d, rule, n, u, v, workload):
         tot["regrows"] += st.regrows
     g.synchronize()
     elapsed = time.perf_counter() - t0
+    # untimed: the same stream's next batches with a hipEvent pair around every
+    # launch of the loop's count pass (on the library's stream)
+    g.set_profiling(True)
+    prof = {"ms": 0.0, "bytes": 0, "launches": 0, "kms": 0.0, "rbytes": 0, "rl": 0}
+    for i in range(args.warmup + args.steps, nb):
+        g.apply_device_batch(dev_batches[i])
+        st = g.recolor(rule)
+        prof["ms"] += st.ms_count_pass
+        prof["bytes"] += st.count_pass_bytes
+        prof["launches"] += st.count_pass_launches
+        prof["kms"] += st.ms_recolor_kernel
+        prof["rbytes"] += st.recolor_bytes
+        prof["rl"] += st.recolor_launches
+    g.set_profiling(False)
+    g.synchronize()
     mono, maxc = g.validate()
     if mono != 0:
         log(f"INVALID: {mono} monochromatic edges after the timed batches")
@@ -127,8 +158,14 @@ def run_stream(args, d, rule, n, u, v, workload):
         "rule": args.rule, "driver": args.driver, "colors": int(maxc + 1),
         "monochromatic": int(mono)})
     out["latency_ms"] = {"p50": pct(lat_ms, 50), "p99": pct(lat_ms, 99), "max": float(lat_ms.max())}
-    out["roofline"] = roofline("k_recolor<M_LOOP>" if args.driver == "plain" else "k_fused",
-                               tot["bytes"], tot["kms"], tot["launches"])
+    if args.driver == "plain" and rule == d.RULE_INDEPENDENT and prof["launches"]:
+        out["roofline"] = roofline("k_round_c (count pass)", prof["bytes"], prof["ms"],
+                                   prof["launches"])
+    else:
+        out["roofline"] = roofline("k_loop_persistent" if args.driver == "fused" else
+                                   "k_recolor", prof["rbytes"], prof["kms"], prof["rl"])
+    out["roofline"]["measured_on"] = f"{nprof} batches after the timed ones"
+    out["roofline"]["traffic"] = traffic_for(out["roofline"]["kernel"])
     out["work"] = {"loop_rounds": tot["rounds"], "evaluated": tot["evaluated"],
                    "changed": tot["changed"], "row_slots_scanned": tot["slots"],
                    "batches_with_row_growth": tot["regrows"]}
@@ -447,7 +484,21 @@ def main():
     ap.add_argument("--budget-lines", type=int, default=4_000_000)
     ap.add_argument("--c3-scale", type=int, default=24)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="copy 1 GiB five times (device to device) and exit: the reference "
+                         "kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE readings")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        n = 1 << 28
+        a = torch.ones(n, dtype=torch.int32, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_each_way": n * 4, "copies": 5}), flush=True)
+        return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the recolor path has no multi-GPU "

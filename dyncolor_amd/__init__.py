@@ -54,7 +54,8 @@ class Stats(C.Structure):
                 ("recolor_bytes", C.c_int64), ("ms_apply", C.c_double),
                 ("ms_recolor", C.c_double), ("ms_recolor_kernel", C.c_double),
                 ("recolor_launches", C.c_int32), ("regrows", C.c_int32),
-                ("reserved", C.c_int32 * 6)]
+                ("ms_count_pass", C.c_double), ("count_pass_bytes", C.c_int64),
+                ("count_pass_launches", C.c_int32), ("reserved", C.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}

@@ -215,6 +215,9 @@ struct LoopResult {
   uint64_t slots = 0, live = 0, rslots = 0, rlive = 0;
   double kernel_ms = 0.0;
   int64_t launches = 0;
+  double count_ms = 0.0;        // summed count-pass launch durations (profiling on)
+  int64_t count_launches = 0;
+  uint64_t count_bytes = 0;     // algorithmic bytes of those launches
 };
 int loop_hi(Graph* g, Frontier L0, int32_t nl0, bool persistent, LoopResult* res);
 void loop_free(Graph* g);

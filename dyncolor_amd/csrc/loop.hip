@@ -60,12 +60,12 @@
 namespace dc {
 
 constexpr int kMaxBlocks = 2048;   // grid of the loop kernels (per-block partials)
+constexpr int kAcc = 32;           // phase A sums: workgroup b adds into slot b % kAcc
 constexpr int kWaveList = 64 * 32; // set bits of 64 bitmap words
 
 struct LoopCtl {
   int32_t done_round;   // the round whose C was empty (INT_MAX while running)
   int32_t nl0;          // |L_0|
-  uint32_t flag[2];     // C_r non-empty, by round parity
 };
 
 struct LoopState {
@@ -83,9 +83,13 @@ struct LoopState {
   const int2* hch;     // hub chunk table {vertex, chunk}
   uint32_t n_hch;
   unsigned long long heavy_slots;  // D_r rows beyond this many slots: pull recount
-  uint32_t* part;      // [2][kMaxBlocks][2]: members, changes of phase A, by round parity
-  unsigned long long* dpart;  // [2][kMaxBlocks]: slots of the rows phase A put into D
-  unsigned long long* bstat;  // [kMaxBlocks][4]: stale-scan slots/live, count-pass slots/live
+  // phase A's totals by round parity, spread over kAcc slots so that no
+  // address takes more than gridDim / kAcc atomics: {members, changes, slots
+  // of the rows put into D, hub rows put into D}
+  unsigned long long* acc;    // [2][kAcc][4]
+  // per workgroup: {stale-scan slots (A), pull slots (C), count slots (C),
+  // row and chunk visits (C)}
+  unsigned long long* bstat;  // [kMaxBlocks][4]
   int4* trace;         // per round {|L|, |C|, changed, 0}
   LoopCtl* ctl;
 };
@@ -96,10 +100,12 @@ struct LoopLds {
   unsigned long long wins[kBlock / 64][kWinWords];
   uint32_t red[kBlock / 64];
   unsigned long long red64[kBlock / 64];
+  unsigned long long tot[4];
 };
 
 struct IncState {
   LoopState S{};
+  std::vector<hipEvent_t> cev;   // profiling: start/stop around each count-pass launch
   uint32_t* bits = nullptr;   // Db[0..1], Rb[0..1]
   int32_t trace_cap = 0;
   LoopCtl* hctl = nullptr;
@@ -291,7 +297,7 @@ __device__ __forceinline__ void round_a(const LoopState& S, int r, LoopLds& lds)
   unsigned long long* win = lds.wins[wib];
   RecolorArgs RA{};
   RA.G = G;
-  uint32_t members = 0, changed = 0;
+  uint32_t members = 0, changed = 0, dhubs = 0;
   unsigned long long slots = 0, live = 0, dsum = 0;
 
   walk_bits(S.words, list, [&](uint32_t i) { return Lb[i] | Dp[i] | Rp[i]; },
@@ -329,7 +335,11 @@ __device__ __forceinline__ void round_a(const LoopState& S, int r, LoopLds& lds)
         set_bit(Dn, w);
       }
       if (member && !wstale) dv = commit_member(S, Cb, Dn, w, cm, so, po, inl, m, changed);
-      if (dv) dsum += G.row[w].y;
+      if (dv) {
+        const uint32_t len = G.row[w].y;
+        dsum += len;
+        dhubs += len >= kMidLen ? 1u : 0u;
+      }
       members += member ? 1u : 0u;
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
       __builtin_amdgcn_wave_barrier();
@@ -344,29 +354,30 @@ __device__ __forceinline__ void round_a(const LoopState& S, int r, LoopLds& lds)
       const int32_t w = lst[q];
       const int32_t m = wave_row_mex<M_LOOP_HI>(RA, w, win);
       if (lane == 0 &&
-          commit_member(S, Cb, Dn, w, -1, S.col0[w], S.prev[w], has_bit(Lb, w), m, changed))
-        dsum += G.row[w].y;
+          commit_member(S, Cb, Dn, w, -1, S.col0[w], S.prev[w], has_bit(Lb, w), m, changed)) {
+        const uint32_t len = G.row[w].y;
+        dsum += len;
+        dhubs += len >= kMidLen ? 1u : 0u;
+      }
     }
   });
 
   const uint32_t bm = block_sum(members, red);
   const uint32_t bc = block_sum(changed, red);
+  const uint32_t bh = block_sum(dhubs, red);
   const unsigned long long bd = block_sum64(dsum, lds.red64);
   for (int off = 32; off > 0; off >>= 1) {
     slots += __shfl_xor(slots, off);
     live += __shfl_xor(live, off);
   }
   if (threadIdx.x == 0) {
-    uint32_t* p = S.part + ((size_t)(r & 1) * kMaxBlocks + blockIdx.x) * 2;
-    p[0] = bm;
-    p[1] = bc;
-    S.dpart[(size_t)(r & 1) * kMaxBlocks + blockIdx.x] = bd;
-    if (bm) S.ctl->flag[r & 1] = 1u;
+    unsigned long long* a = S.acc + ((size_t)(r & 1) * kAcc + blockIdx.x % kAcc) * 4;
+    if (bm) atomicAdd(&a[0], (unsigned long long)bm);
+    if (bc) atomicAdd(&a[1], (unsigned long long)bc);
+    if (bd) atomicAdd(&a[2], bd);
+    if (bh) atomicAdd(&a[3], (unsigned long long)bh);
   }
-  if (lane == 0 && (slots | live)) {
-    atomicAdd(&S.bstat[blockIdx.x * 4 + 0], slots);
-    atomicAdd(&S.bstat[blockIdx.x * 4 + 1], live);
-  }
+  if (lane == 0 && slots) atomicAdd(&S.bstat[blockIdx.x * 4 + 0], slots);
 }
 
 // ---------------------------------------------------------------------------
@@ -488,33 +499,46 @@ __device__ __forceinline__ void set_count(const LoopState& S, int32_t y, uint32_
 }
 
 __device__ __forceinline__ void pull_all(const LoopState& S, const uint32_t* cb, uint32_t* rn,
-                                         LoopLds& lds, unsigned long long& slots) {
+                                         LoopLds& lds, unsigned long long& slots,
+                                         unsigned long long& rows) {
   const GraphView& G = S.G;
   const int lane = lane_id();
   int32_t* lst = lds.lists[threadIdx.x >> 6];
   const uint32_t gwave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
-  // short rows by their lane, mid rows by the wave (vertices strided over
-  // waves so the dense low IDs spread out)
-  for (uint64_t base = 0; base < (uint64_t)G.n_owned; base += 64ull * nwaves) {
-    const uint64_t vi = base + (uint64_t)lane * nwaves + gwave;
-    const bool valid = vi < (uint64_t)G.n_owned;
+  // short rows by 8-lane groups, mid rows by the wave; consecutive groups of
+  // 8 vertices go to different waves, so the dense low IDs spread out
+  const int grp = lane >> 3, sub = lane & 7;
+  const uint64_t nv = (uint64_t)G.n_owned;
+  for (uint64_t base = (uint64_t)gwave * 8; base < nv; base += 8ull * nwaves) {
+    const uint64_t vi = base + grp;
+    const bool valid = vi < nv;
     const int32_t y = (int32_t)vi;
     const uint2 rw = valid ? G.row[y] : make_uint2(0u, 0u);
     const bool small = valid && rw.y < kSmallLen;
     const bool mid = valid && rw.y >= kSmallLen && rw.y < kMidLen;
-    if (small) set_count(S, y, pull_slots<8>(S, y, S.col0[y], rw.x, 0, rw.y, 1, cb, slots), cb, rn);
-    const unsigned long long mm = __ballot(mid);
-    if (mid) lst[lanes_below(mm)] = y;
+    uint32_t cnt = small ? pull_slots<8>(S, y, S.col0[y], rw.x, sub, rw.y, kGroup, cb, slots) : 0u;
+    cnt += __shfl_xor(cnt, 1);
+    cnt += __shfl_xor(cnt, 2);
+    cnt += __shfl_xor(cnt, 4);
+    if (small && sub == 0) {
+      set_count(S, y, cnt, cb, rn);
+      ++rows;
+    }
+    const unsigned long long mm = __ballot(mid && sub == 0);
+    if (mid && sub == 0) lst[lanes_below(mm)] = y;
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
     const uint32_t nm = (uint32_t)__popcll(mm);
     for (uint32_t q = 0; q < nm; ++q) {
       const int32_t x = lst[q];
       const uint2 r2 = G.row[x];
-      uint32_t cnt = pull_slots<4>(S, x, S.col0[x], r2.x, lane, r2.y, 64, cb, slots);
-      for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off);
-      if (lane == 0) set_count(S, x, cnt, cb, rn);
+      uint32_t c2 = pull_slots<4>(S, x, S.col0[x], r2.x, lane, r2.y, 64, cb, slots);
+      for (int off = 32; off > 0; off >>= 1) c2 += __shfl_xor(c2, off);
+      if (lane == 0) {
+        set_count(S, x, c2, cb, rn);
+        ++rows;
+      }
     }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -528,7 +552,10 @@ __device__ __forceinline__ void pull_all(const LoopState& S, const uint32_t* cb,
     if (rw.y < kMidLen) continue;  // went by the short/mid pass (uniform)
     const uint32_t cnt = pull_slots<4>(S, y, S.col0[y], rw.x, threadIdx.x, rw.y, kBlock, cb, slots);
     const uint32_t tot = block_sum(cnt, lds.red);
-    if (threadIdx.x == 0) set_count(S, y, tot, cb, rn);
+    if (threadIdx.x == 0) {
+      set_count(S, y, tot, cb, rn);
+      ++rows;
+    }
   }
 }
 
@@ -595,35 +622,36 @@ __device__ __forceinline__ void round_c(const LoopState& S, int r, LoopLds& lds)
   const uint32_t* cb = S.B[r & 1];
   uint32_t* db = S.Db[r & 1];
   uint32_t* rn = S.Rb[r & 1];
-  const bool nonempty = S.ctl->flag[r & 1] != 0u;
   const int lane = lane_id();
   int32_t* list = lds.lists[threadIdx.x >> 6];
-  // every workgroup sums the same partials, so the choice is uniform
-  bool heavy;
-  {
-    unsigned long long d = 0;
-    for (uint32_t b = threadIdx.x; b < gridDim.x; b += blockDim.x)
-      d += S.dpart[(size_t)(r & 1) * kMaxBlocks + b];
-    heavy = block_sum64(d, lds.red64) > S.heavy_slots;
-  }
-
-  // round r's counts into the trace; the flag of round r+1 starts clear
-  if (blockIdx.x == 0) {
-    uint32_t m = 0, c = 0;
-    for (uint32_t b = threadIdx.x; b < gridDim.x; b += blockDim.x) {
-      const uint32_t* p = S.part + ((size_t)(r & 1) * kMaxBlocks + b) * 2;
-      m += p[0];
-      c += p[1];
+  // phase A's totals (every workgroup reads the same slots: uniform choices)
+  if (threadIdx.x < 64) {
+    unsigned long long v[4] = {0ull, 0ull, 0ull, 0ull};
+    if (lane < kAcc) {
+      const unsigned long long* a = S.acc + ((size_t)(r & 1) * kAcc + lane) * 4;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) v[k] = a[k];
     }
-    m = block_sum(m, red);
-    c = block_sum(c, red);
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_xor(v[k], off);
+    if (lane == 0)
+      for (int k = 0; k < 4; ++k) lds.tot[k] = v[k];
+  }
+  __syncthreads();
+  const unsigned long long nmem = lds.tot[0];
+  const bool nonempty = nmem != 0ull;
+  const bool heavy = lds.tot[2] > S.heavy_slots;
+  const bool dhubs = lds.tot[3] != 0ull;
+  if (blockIdx.x == 0) {
+    // round r's counts into the trace; round r+1's sums start at zero
     if (threadIdx.x == 0) {
       S.trace[r].x = r == 0 ? S.ctl->nl0 : S.trace[r - 1].y;
-      S.trace[r].y = (int32_t)m;
-      S.trace[r].z = (int32_t)c;
-      S.ctl->flag[(r + 1) & 1] = 0u;
+      S.trace[r].y = (int32_t)nmem;
+      S.trace[r].z = (int32_t)lds.tot[1];
       if (!nonempty) S.ctl->done_round = r;
     }
+    if (threadIdx.x < kAcc * 4) S.acc[(size_t)((r + 1) & 1) * kAcc * 4 + threadIdx.x] = 0ull;
   }
   // L_r, D_{r-1} and R_{r-1} are no longer read
   clear_words(S.B[(r + 1) & 1], S.words);
@@ -633,9 +661,9 @@ __device__ __forceinline__ void round_c(const LoopState& S, int r, LoopLds& lds)
     clear_words(db, S.words);
     return;
   }
-  unsigned long long slots = 0, live = 0;
+  unsigned long long slots = 0, live = 0, pslots = 0, rows = 0;
   if (heavy) {
-    pull_all(S, cb, rn, lds, slots);
+    pull_all(S, cb, rn, lds, pslots, rows);
     propagate_changed(S, db, lds);
   } else {
   // short and mid rows of D_r (hubs go by the chunk table below)
@@ -651,6 +679,7 @@ __device__ __forceinline__ void round_c(const LoopState& S, int r, LoopLds& lds)
       if (small) {
         const DState d = load_d(S, cb, x);
         S.K[x] = (int32_t)count_slots(S, d, rw.x, 0, rw.y, 1, cb, db, rn, slots, live);
+        ++rows;
       }
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
       __builtin_amdgcn_wave_barrier();
@@ -666,13 +695,16 @@ __device__ __forceinline__ void round_c(const LoopState& S, int r, LoopLds& lds)
       const DState d = load_d(S, cb, x);
       uint32_t cnt = count_slots(S, d, rw.x, lane, rw.y, 64, cb, db, rn, slots, live);
       for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off);
-      if (lane == 0) S.K[x] = (int32_t)cnt;
+      if (lane == 0) {
+        S.K[x] = (int32_t)cnt;
+        ++rows;
+      }
     }
   });
   // hub rows of D_r, one wave per kChunk-slot chunk (K[x] was zeroed in A).
   // A hub's chunks are adjacent in the table; lane l of wave w looks at entry
   // l * nwaves + w, so they go to different waves.
-  {
+  if (dhubs) {
     const uint32_t gwave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
     for (uint64_t base = 0; base < S.n_hch; base += 64ull * nwaves) {
@@ -693,18 +725,24 @@ __device__ __forceinline__ void round_c(const LoopState& S, int r, LoopLds& lds)
         uint32_t cnt = count_slots(S, d, rw.x, lo + lane, min(rw.y, lo + kChunk), 64, cb, db, rn,
                                    slots, live);
         for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off);
-        if (lane == 0 && cnt) atomicAdd(&S.K[x], (int32_t)cnt);
+        if (lane == 0) {
+          if (cnt) atomicAdd(&S.K[x], (int32_t)cnt);
+          ++rows;
+        }
       }
     }
   }
   }
+  (void)live;
   for (int off = 32; off > 0; off >>= 1) {
     slots += __shfl_xor(slots, off);
-    live += __shfl_xor(live, off);
+    pslots += __shfl_xor(pslots, off);
+    rows += __shfl_xor(rows, off);
   }
-  if (lane == 0 && (slots | live)) {
-    atomicAdd(&S.bstat[blockIdx.x * 4 + 2], slots);
-    atomicAdd(&S.bstat[blockIdx.x * 4 + 3], live);
+  if (lane == 0) {
+    if (pslots) atomicAdd(&S.bstat[blockIdx.x * 4 + 1], pslots);
+    if (slots) atomicAdd(&S.bstat[blockIdx.x * 4 + 2], slots);
+    if (rows) atomicAdd(&S.bstat[blockIdx.x * 4 + 3], rows);
   }
 }
 
@@ -722,8 +760,8 @@ __global__ __launch_bounds__(kBlock) void k_round_c(LoopState S, int r) {
 }
 
 // Persistent form: rounds r0..r1-1 in one cooperative launch.  The exit test
-// reads a flag written before the last grid barrier, so every workgroup
-// takes the same branch.
+// reads the control word written before the last grid barrier, so every
+// workgroup takes the same branch.
 __global__ __launch_bounds__(kBlock) void k_loop_persistent(LoopState S, int r0, int r1) {
   namespace cg = cooperative_groups;
   cg::grid_group grid = cg::this_grid();
@@ -764,15 +802,14 @@ static int inc_alloc(Graph* g, IncState* s) {
   S.Db[1] = s->bits + S.words;
   S.Rb[0] = s->bits + 2 * (size_t)S.words;
   S.Rb[1] = s->bits + 3 * (size_t)S.words;
-  DC_HIP(hipMalloc(&S.part, sizeof(uint32_t) * 4 * kMaxBlocks));
-  DC_HIP(hipMalloc(&S.dpart, sizeof(unsigned long long) * 2 * kMaxBlocks));
+  DC_HIP(hipMalloc(&S.acc, sizeof(unsigned long long) * 2 * kAcc * 4));
   DC_HIP(hipMalloc(&S.bstat, sizeof(unsigned long long) * 4 * kMaxBlocks));
   DC_HIP(hipMalloc(&S.ctl, sizeof(LoopCtl)));
   DC_HIP(hipHostMalloc(&s->hctl, sizeof(LoopCtl), hipHostMallocDefault));
   s->hstat = new unsigned long long[4 * kMaxBlocks];
   DC_HIP(hipMemsetAsync(S.K, 0, sizeof(int32_t) * n, g->stream));
   DC_HIP(hipMemsetAsync(s->bits, 0, sizeof(uint32_t) * 4 * (size_t)S.words, g->stream));
-  DC_HIP(hipMemsetAsync(S.part, 0, sizeof(uint32_t) * 4 * kMaxBlocks, g->stream));
+
   return DC_OK;
 }
 
@@ -799,9 +836,10 @@ void loop_free(Graph* g) {
   auto f = [](void* p) {
     if (p) (void)hipFree(p);
   };
-  f(s->S.K); f(s->S.dold); f(s->bits); f(s->S.part); f(s->S.dpart); f(s->S.bstat); f(s->S.trace);
+  f(s->S.K); f(s->S.dold); f(s->bits); f(s->S.acc); f(s->S.bstat); f(s->S.trace);
   f(s->S.ctl);
   if (s->hctl) (void)hipHostFree(s->hctl);
+  for (hipEvent_t e : s->cev) (void)hipEventDestroy(e);
   delete[] s->hstat;
   delete s;
   g->inc = nullptr;
@@ -874,19 +912,28 @@ int loop_hi(Graph* g, Frontier L0, int32_t nl0, bool persistent, LoopResult* res
   s->hctl->nl0 = nl0;
   DC_HIP(hipMemcpyAsync(S.ctl, s->hctl, sizeof(LoopCtl), hipMemcpyHostToDevice, g->stream));
   DC_HIP(hipMemsetAsync(S.bstat, 0, sizeof(unsigned long long) * 4 * kMaxBlocks, g->stream));
+  DC_HIP(hipMemsetAsync(S.acc, 0, sizeof(unsigned long long) * 2 * kAcc * 4, g->stream));
   const dim3 blk(kBlock);
   hipLaunchKernelGGL(k_loop_init, dim3(kMaxBlocks), blk, 0, g->stream, S);
   DC_HIP(hipGetLastError());
   int64_t launches = 1;
   int grid = kMaxBlocks;
   if (persistent && (rc = persistent_grid(g, &grid))) return rc;
-  double kms = 0.0;
+  double kms = 0.0, cms = 0.0;
+  int64_t claunches = 0;
   hipEvent_t e0 = g->ev[2], e1 = g->ev[3];
   int32_t rounds = -1, r = 0, block = persistent ? 4096 : 8;
   while (r < max_rounds && rounds < 0) {
     const int32_t nb = std::min(block, max_rounds - r);
     if ((rc = inc_trace(g, s, r + nb))) return rc;
-    if (g->profiling) DC_HIP(hipEventRecord(e0, g->stream));
+    if (g->profiling) {
+      while (s->cev.size() < 2 * (size_t)nb) {
+        hipEvent_t e;
+        DC_HIP(hipEventCreate(&e));
+        s->cev.push_back(e);
+      }
+      DC_HIP(hipEventRecord(e0, g->stream));
+    }
     if (persistent) {
       int32_t r0 = r, r1 = r + nb;
       void* args[] = {&S, &r0, &r1};
@@ -896,7 +943,9 @@ int loop_hi(Graph* g, Frontier L0, int32_t nl0, bool persistent, LoopResult* res
     } else {
       for (int32_t i = 0; i < nb; ++i) {
         hipLaunchKernelGGL(k_round_a, dim3(grid), blk, 0, g->stream, S, r + i);
+        if (g->profiling) DC_HIP(hipEventRecord(s->cev[2 * i], g->stream));
         hipLaunchKernelGGL(k_round_c, dim3(grid), blk, 0, g->stream, S, r + i);
+        if (g->profiling) DC_HIP(hipEventRecord(s->cev[2 * i + 1], g->stream));
       }
       DC_HIP(hipGetLastError());
       launches += 2 * nb;
@@ -904,13 +953,22 @@ int loop_hi(Graph* g, Frontier L0, int32_t nl0, bool persistent, LoopResult* res
     if (g->profiling) DC_HIP(hipEventRecord(e1, g->stream));
     DC_HIP(hipMemcpyAsync(s->hctl, S.ctl, sizeof(LoopCtl), hipMemcpyDeviceToHost, g->stream));
     DC_HIP(hipStreamSynchronize(g->stream));
+    const int32_t done = s->hctl->done_round;
     if (g->profiling) {
       float x = 0.f;
       DC_HIP(hipEventElapsedTime(&x, e0, e1));
       kms += x;
+      if (!persistent) {
+        // count-pass launches of executed rounds (later ones return at once)
+        for (int32_t i = 0; i < nb && r + i <= done; ++i) {
+          DC_HIP(hipEventElapsedTime(&x, s->cev[2 * i], s->cev[2 * i + 1]));
+          cms += x;
+          ++claunches;
+        }
+      }
     }
     r += nb;
-    if (s->hctl->done_round != INT_MAX) rounds = s->hctl->done_round + 1;
+    if (done != INT_MAX) rounds = done + 1;
     block = std::min(block * 2, persistent ? 4096 : 32);
   }
   if (rounds < 0) {
@@ -933,14 +991,23 @@ int loop_hi(Graph* g, Frontier L0, int32_t nl0, bool persistent, LoopResult* res
     res->evaluated += (uint64_t)tr[i].y;
     res->changed += (uint64_t)tr[i].z;
   }
+  uint64_t pull = 0, cnt = 0, rowv = 0;
   for (int b = 0; b < kMaxBlocks; ++b) {
     res->rslots += s->hstat[4 * b + 0];
-    res->rlive += s->hstat[4 * b + 1];
-    res->slots += s->hstat[4 * b + 2];
-    res->live += s->hstat[4 * b + 3];
+    pull += s->hstat[4 * b + 1];
+    cnt += s->hstat[4 * b + 2];
+    rowv += s->hstat[4 * b + 3];
   }
+  res->slots = pull + cnt;
   res->kernel_ms = kms;
   res->launches = launches;
+  // Algorithmic bytes of the count pass: per round the four bitmaps it clears
+  // or walks (4 B per word each); per count-pass slot col + ccol (8 B); per
+  // pulled slot col (4 B); per row or chunk visit its bounds and count or
+  // saved state (16 B).
+  res->count_ms = cms;
+  res->count_launches = claunches;
+  res->count_bytes = (uint64_t)rounds * 16ull * S.words + 8ull * cnt + 4ull * pull + 16ull * rowv;
   return DC_OK;
 }
 

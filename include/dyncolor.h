@@ -79,8 +79,12 @@ typedef struct {
   double ms_recolor;        /* device time of rounds + loop (hipEvent)        */
   double ms_recolor_kernel; /* summed duration of the recolor kernels (profiling on) */
   int32_t recolor_launches; /* recolor kernel launches in the batch           */
-  int32_t regrows;          /* row-capacity regrows triggered by the batch    */
-  int32_t reserved[6];
+  int32_t regrows;          /* batches whose rows outgrew their capacity      */
+  double ms_count_pass;     /* summed duration of the independent-rule loop's
+                               count-pass launches (profiling on, plain driver) */
+  int64_t count_pass_bytes; /* algorithmic bytes of those launches            */
+  int32_t count_pass_launches;
+  int32_t reserved;
 } dc_stats;
 
 /* ---- graph lifetime ------------------------------------------------------ */
