@@ -453,8 +453,32 @@ __device__ __forceinline__ void hub_finalize_body(const RecolorArgs& A) {
 // colour newc[v] into the ccol slot that points back at v in each owned
 // neighbour's row (found through v's twins).  Runs between rounds, when no
 // row visit reads ccol.
+// A neighbour w of a vertex that changed colour `old` -> `c`: its colour
+// histogram (if it keeps one) follows, and its cached mex m becomes stale only
+// if the change can move it: `old` < m lost its last copy (known from the
+// histogram's count; without one, any `old` < m), or `c` == m.  When several
+// neighbours of w change in the same round this stays safe: a colour below m
+// that ends at count 0 had a decrement that left it at 0, and colour m can
+// only be gained by a change to m.
+__device__ __forceinline__ void neighbour_follows(const GraphView& G, int32_t w, int32_t c,
+                                                  int32_t old) {
+  const int32_t h = G.hidx[w];
+  const int32_t m = G.mexc[w];
+  bool stale;
+  if (h >= 0) {
+    uint32_t* hr = G.hist + (uint64_t)h * kHistW;
+    uint32_t before = 2u;
+    if (old >= 0 && old < kHistW) before = atomicSub(&hr[old], 1u);
+    if (c < kHistW) atomicAdd(&hr[c], 1u);
+    stale = (old < m && (old >= kHistW || before == 1u)) || c == m;
+  } else {
+    stale = old < m || c == m;
+  }
+  if (m >= 0 && stale) G.mexc[w] = -1;
+}
+
 // (v changed from colour `old` to `c`: each owned neighbour's slot cache,
-// cached mex and, for hubs, colour histogram follow)
+// histogram and cached mex follow)
 __device__ __forceinline__ void propagate_slots(const GraphView& G, int32_t c, int32_t old,
                                                 uint64_t rbase, uint32_t lo, uint32_t hi,
                                                 uint32_t stride) {
@@ -462,13 +486,7 @@ __device__ __forceinline__ void propagate_slots(const GraphView& G, int32_t c, i
     const int32_t w = G.col[rbase + j];
     if (w < 0 || w >= G.n_owned) continue;
     G.ccol[(uint64_t)G.row[w].x + (uint32_t)G.twin[rbase + j]] = c;
-    G.mexc[w] = -1;
-    const int32_t h = G.hidx[w];
-    if (h >= 0) {
-      uint32_t* hr = G.hist + (uint64_t)h * kHistW;
-      if (old >= 0 && old < kHistW) atomicSub(&hr[old], 1u);
-      if (c < kHistW) atomicAdd(&hr[c], 1u);
-    }
+    neighbour_follows(G, w, c, old);
   }
 }
 

@@ -61,11 +61,15 @@ namespace dc {
 
 constexpr int kMaxBlocks = 2048;   // grid of the loop kernels (per-block partials)
 constexpr int kAcc = 32;           // phase A sums: workgroup b adds into slot b % kAcc
+constexpr int kBitmaps = 6;        // Db, Rb, Xb (two each) besides the C/L pair
+constexpr int32_t kLightC = 4096;  // rounds after one with fewer members use the light grid
 constexpr int kWaveList = 64 * 32; // set bits of 64 bitmap words
 
 struct LoopCtl {
   int32_t done_round;   // the round whose C was empty (INT_MAX while running)
   int32_t nl0;          // |L_0|
+  int32_t last_nc;      // |C| of the latest round
+  int32_t pad;
 };
 
 struct LoopState {
@@ -78,6 +82,7 @@ struct LoopState {
   uint32_t* B[2];      // C_r members in B[r & 1]
   uint32_t* Db[2];     // D_r in Db[r & 1]
   uint32_t* Rb[2];     // R_r in Rb[r & 1]
+  uint32_t* Xb[2];     // members of C_r whose colour changed in round r, in Xb[r & 1]
   uint32_t words;      // words per bitmap
   Frontier L0;         // the marked set
   const int2* hch;     // hub chunk table {vertex, chunk}
@@ -106,7 +111,7 @@ struct LoopLds {
 struct IncState {
   LoopState S{};
   std::vector<hipEvent_t> cev;   // profiling: start/stop around each count-pass launch
-  uint32_t* bits = nullptr;   // Db[0..1], Rb[0..1]
+  uint32_t* bits = nullptr;   // Db[0..1], Rb[0..1], Xb[0..1]
   int32_t trace_cap = 0;
   LoopCtl* hctl = nullptr;
   unsigned long long* hstat = nullptr;
@@ -229,8 +234,8 @@ __global__ __launch_bounds__(kBlock) void k_loop_init(LoopState S) {
 // Commit w's new colour m (one lane); w joins D when its (in_L, colour,
 // prev) state changed.
 __device__ __forceinline__ bool commit_member(const LoopState& S, uint32_t* cb, uint32_t* dn,
-                                              int32_t w, int32_t cm, int32_t so, int32_t po,
-                                              bool inl, int32_t m, uint32_t& changed) {
+                                              uint32_t* xb, int32_t w, int32_t cm, int32_t so,
+                                              int32_t po, bool inl, int32_t m, uint32_t& changed) {
   set_bit(cb, w);
   if (cm < 0) S.G.mexc[w] = m;
   S.col0[w] = m;
@@ -238,6 +243,7 @@ __device__ __forceinline__ bool commit_member(const LoopState& S, uint32_t* cb, 
   S.prev[w] = so;
   const bool chv = m != so;
   changed += chv ? 1u : 0u;
+  if (chv) set_bit(xb, w);
   if (!inl || po != so || chv) {
     S.dold[w] = make_int2(so, (po + 1) | (inl ? (int32_t)0x80000000 : 0));
     S.K[w] = 0;
@@ -291,6 +297,7 @@ __device__ __forceinline__ void round_a(const LoopState& S, int r, LoopLds& lds)
   const uint32_t* Dp = S.Db[(r + 1) & 1];
   uint32_t* Dn = S.Db[r & 1];
   const uint32_t* Rp = S.Rb[(r + 1) & 1];
+  uint32_t* Xn = S.Xb[r & 1];
   const int lane = lane_id();
   const int wib = threadIdx.x >> 6;
   int32_t* list = lds.lists[wib];
@@ -334,7 +341,7 @@ __device__ __forceinline__ void round_a(const LoopState& S, int r, LoopLds& lds)
         S.dold[w] = make_int2(so, (po + 1) | (int32_t)0x80000000);
         set_bit(Dn, w);
       }
-      if (member && !wstale) dv = commit_member(S, Cb, Dn, w, cm, so, po, inl, m, changed);
+      if (member && !wstale) dv = commit_member(S, Cb, Dn, Xn, w, cm, so, po, inl, m, changed);
       if (dv) {
         const uint32_t len = G.row[w].y;
         dsum += len;
@@ -354,7 +361,7 @@ __device__ __forceinline__ void round_a(const LoopState& S, int r, LoopLds& lds)
       const int32_t w = lst[q];
       const int32_t m = wave_row_mex<M_LOOP_HI>(RA, w, win);
       if (lane == 0 &&
-          commit_member(S, Cb, Dn, w, -1, S.col0[w], S.prev[w], has_bit(Lb, w), m, changed)) {
+          commit_member(S, Cb, Dn, Xn, w, -1, S.col0[w], S.prev[w], has_bit(Lb, w), m, changed)) {
         const uint32_t len = G.row[w].y;
         dsum += len;
         dhubs += len >= kMidLen ? 1u : 0u;
@@ -407,12 +414,28 @@ __device__ __forceinline__ DState load_d(const LoopState& S, const uint32_t* cb,
   return d;
 }
 
+// Colour and prev of a member v of C_r seen from a slot holding v's colour
+// cv: a member that kept its colour this round has prev == colour == the slot
+// value (its slot cache is current); only members that changed colour (xb)
+// are gathered, since their slots may not be updated yet.
+__device__ __forceinline__ void member_state(const LoopState& S, const uint32_t* xb, int32_t v,
+                                             int32_t cv, int32_t& c, int32_t& p) {
+  if (has_bit(xb, v)) {
+    c = S.col0[v];
+    p = S.prev[v];
+  } else {
+    c = cv;
+    p = cv;
+  }
+}
+
 // One lane's share of a count-pass visit of x's row: slots lo, lo+stride, ...
 // Returns this lane's part of x's recount.
 __device__ __forceinline__ uint32_t count_slots(const LoopState& S, const DState& d, uint64_t rbase,
                                                 uint32_t lo, uint32_t hi, uint32_t stride,
                                                 const uint32_t* cb, const uint32_t* db,
-                                                uint32_t* rn, unsigned long long& slots,
+                                                const uint32_t* xb, uint32_t* rn,
+                                                unsigned long long& slots,
                                                 unsigned long long& live) {
   const GraphView& G = S.G;
   uint32_t cnt = 0;
@@ -441,17 +464,15 @@ __device__ __forceinline__ uint32_t count_slots(const LoopState& S, const DState
           if (tn) set_bit(rn, yy);
         }
       }
-      if (has_bit(cb, yy) && marks(yy, S.col0[yy], S.prev[yy], d.x, d.sn)) ++cnt;
+      if (has_bit(cb, yy)) {
+        int32_t cy, py;
+        member_state(S, xb, yy, c[k], cy, py);
+        if (marks(yy, cy, py, d.x, d.sn)) ++cnt;
+      }
       if (d.chg) {
         const uint64_t tslot = rbase + (j + k * stride);
         G.ccol[(uint64_t)G.row[yy].x + (uint32_t)G.twin[tslot]] = d.sn;
-        G.mexc[yy] = -1;
-        const int32_t h = G.hidx[yy];
-        if (h >= 0) {
-          uint32_t* hr = G.hist + (uint64_t)h * kHistW;
-          if (d.so >= 0 && d.so < kHistW) atomicSub(&hr[d.so], 1u);
-          if (d.sn < kHistW) atomicAdd(&hr[d.sn], 1u);
-        }
+        neighbour_follows(G, yy, d.sn, d.so);
       }
     }
   }
@@ -470,23 +491,27 @@ template <int U>
 __device__ __forceinline__ uint32_t pull_slots(const LoopState& S, int32_t y, int32_t cy,
                                                uint64_t rbase, uint32_t lo, uint32_t hi,
                                                uint32_t stride, const uint32_t* cb,
-                                               unsigned long long& slots) {
+                                               const uint32_t* xb, unsigned long long& slots) {
   const GraphView& G = S.G;
   uint32_t cnt = 0;
   for (uint32_t j = lo; j < hi; j += U * stride) {
-    int32_t x[U];
+    int32_t x[U], c[U];
 #pragma unroll
     for (int k = 0; k < U; ++k) {
       const uint32_t jj = j + k * stride;
       x[k] = jj < hi ? G.col[rbase + jj] : -2;
+      c[k] = jj < hi ? G.ccol[rbase + jj] : -1;
     }
 #pragma unroll
     for (int k = 0; k < U; ++k) {
       if (x[k] == -2) continue;
       ++slots;
       const int32_t xx = x[k];
-      if (xx >= 0 && xx < G.n_owned && has_bit(cb, xx) && marks(xx, S.col0[xx], S.prev[xx], y, cy))
-        ++cnt;
+      if (xx >= 0 && xx < G.n_owned && has_bit(cb, xx)) {
+        int32_t cx, px;
+        member_state(S, xb, xx, c[k], cx, px);
+        if (marks(xx, cx, px, y, cy)) ++cnt;
+      }
     }
   }
   return cnt;
@@ -498,9 +523,9 @@ __device__ __forceinline__ void set_count(const LoopState& S, int32_t y, uint32_
   if (cnt && !has_bit(cb, y)) set_bit(rn, y);
 }
 
-__device__ __forceinline__ void pull_all(const LoopState& S, const uint32_t* cb, uint32_t* rn,
-                                         LoopLds& lds, unsigned long long& slots,
-                                         unsigned long long& rows) {
+__device__ __forceinline__ void pull_all(const LoopState& S, const uint32_t* cb,
+                                         const uint32_t* xb, uint32_t* rn, LoopLds& lds,
+                                         unsigned long long& slots, unsigned long long& rows) {
   const GraphView& G = S.G;
   const int lane = lane_id();
   int32_t* lst = lds.lists[threadIdx.x >> 6];
@@ -517,7 +542,7 @@ __device__ __forceinline__ void pull_all(const LoopState& S, const uint32_t* cb,
     const uint2 rw = valid ? G.row[y] : make_uint2(0u, 0u);
     const bool small = valid && rw.y < kSmallLen;
     const bool mid = valid && rw.y >= kSmallLen && rw.y < kMidLen;
-    uint32_t cnt = small ? pull_slots<8>(S, y, S.col0[y], rw.x, sub, rw.y, kGroup, cb, slots) : 0u;
+    uint32_t cnt = small ? pull_slots<8>(S, y, S.col0[y], rw.x, sub, rw.y, kGroup, cb, xb, slots) : 0u;
     cnt += __shfl_xor(cnt, 1);
     cnt += __shfl_xor(cnt, 2);
     cnt += __shfl_xor(cnt, 4);
@@ -533,7 +558,7 @@ __device__ __forceinline__ void pull_all(const LoopState& S, const uint32_t* cb,
     for (uint32_t q = 0; q < nm; ++q) {
       const int32_t x = lst[q];
       const uint2 r2 = G.row[x];
-      uint32_t c2 = pull_slots<4>(S, x, S.col0[x], r2.x, lane, r2.y, 64, cb, slots);
+      uint32_t c2 = pull_slots<4>(S, x, S.col0[x], r2.x, lane, r2.y, 64, cb, xb, slots);
       for (int off = 32; off > 0; off >>= 1) c2 += __shfl_xor(c2, off);
       if (lane == 0) {
         set_count(S, x, c2, cb, rn);
@@ -550,7 +575,7 @@ __device__ __forceinline__ void pull_all(const LoopState& S, const uint32_t* cb,
     const int32_t y = e.x;
     const uint2 rw = G.row[y];
     if (rw.y < kMidLen) continue;  // went by the short/mid pass (uniform)
-    const uint32_t cnt = pull_slots<4>(S, y, S.col0[y], rw.x, threadIdx.x, rw.y, kBlock, cb, slots);
+    const uint32_t cnt = pull_slots<4>(S, y, S.col0[y], rw.x, threadIdx.x, rw.y, kBlock, cb, xb, slots);
     const uint32_t tot = block_sum(cnt, lds.red);
     if (threadIdx.x == 0) {
       set_count(S, y, tot, cb, rn);
@@ -622,6 +647,7 @@ __device__ __forceinline__ void round_c(const LoopState& S, int r, LoopLds& lds)
   const uint32_t* cb = S.B[r & 1];
   uint32_t* db = S.Db[r & 1];
   uint32_t* rn = S.Rb[r & 1];
+  const uint32_t* xb = S.Xb[r & 1];
   const int lane = lane_id();
   int32_t* list = lds.lists[threadIdx.x >> 6];
   // phase A's totals (every workgroup reads the same slots: uniform choices)
@@ -649,6 +675,7 @@ __device__ __forceinline__ void round_c(const LoopState& S, int r, LoopLds& lds)
       S.trace[r].x = r == 0 ? S.ctl->nl0 : S.trace[r - 1].y;
       S.trace[r].y = (int32_t)nmem;
       S.trace[r].z = (int32_t)lds.tot[1];
+      S.ctl->last_nc = (int32_t)nmem;
       if (!nonempty) S.ctl->done_round = r;
     }
     if (threadIdx.x < kAcc * 4) S.acc[(size_t)((r + 1) & 1) * kAcc * 4 + threadIdx.x] = 0ull;
@@ -657,13 +684,14 @@ __device__ __forceinline__ void round_c(const LoopState& S, int r, LoopLds& lds)
   clear_words(S.B[(r + 1) & 1], S.words);
   clear_words(S.Db[(r + 1) & 1], S.words);
   clear_words(S.Rb[(r + 1) & 1], S.words);
+  clear_words(S.Xb[(r + 1) & 1], S.words);
   if (!nonempty) {
     clear_words(db, S.words);
     return;
   }
   unsigned long long slots = 0, live = 0, pslots = 0, rows = 0;
   if (heavy) {
-    pull_all(S, cb, rn, lds, pslots, rows);
+    pull_all(S, cb, xb, rn, lds, pslots, rows);
     propagate_changed(S, db, lds);
   } else {
   // short and mid rows of D_r (hubs go by the chunk table below)
@@ -678,7 +706,7 @@ __device__ __forceinline__ void round_c(const LoopState& S, int r, LoopLds& lds)
       const bool mid = valid && rw.y >= kSmallLen && rw.y < kMidLen;
       if (small) {
         const DState d = load_d(S, cb, x);
-        S.K[x] = (int32_t)count_slots(S, d, rw.x, 0, rw.y, 1, cb, db, rn, slots, live);
+        S.K[x] = (int32_t)count_slots(S, d, rw.x, 0, rw.y, 1, cb, db, xb, rn, slots, live);
         ++rows;
       }
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -693,7 +721,7 @@ __device__ __forceinline__ void round_c(const LoopState& S, int r, LoopLds& lds)
       const int32_t x = lst[q];
       const uint2 rw = G.row[x];
       const DState d = load_d(S, cb, x);
-      uint32_t cnt = count_slots(S, d, rw.x, lane, rw.y, 64, cb, db, rn, slots, live);
+      uint32_t cnt = count_slots(S, d, rw.x, lane, rw.y, 64, cb, db, xb, rn, slots, live);
       for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off);
       if (lane == 0) {
         S.K[x] = (int32_t)cnt;
@@ -722,8 +750,8 @@ __device__ __forceinline__ void round_c(const LoopState& S, int r, LoopLds& lds)
         // rows now shorter than kMidLen went by the short/mid path
         if (rw.y < kMidLen || lo >= rw.y) continue;
         const DState d = load_d(S, cb, x);
-        uint32_t cnt = count_slots(S, d, rw.x, lo + lane, min(rw.y, lo + kChunk), 64, cb, db, rn,
-                                   slots, live);
+        uint32_t cnt = count_slots(S, d, rw.x, lo + lane, min(rw.y, lo + kChunk), 64, cb, db, xb,
+                                   rn, slots, live);
         for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off);
         if (lane == 0) {
           if (cnt) atomicAdd(&S.K[x], (int32_t)cnt);
@@ -797,18 +825,20 @@ static int inc_alloc(Graph* g, IncState* s) {
   S.words = (uint32_t)g->cbits_words;
   DC_HIP(hipMalloc(&S.K, sizeof(int32_t) * n));
   DC_HIP(hipMalloc(&S.dold, sizeof(int2) * n));
-  DC_HIP(hipMalloc(&s->bits, sizeof(uint32_t) * 4 * (size_t)S.words));
+  DC_HIP(hipMalloc(&s->bits, sizeof(uint32_t) * kBitmaps * (size_t)S.words));
   S.Db[0] = s->bits;
   S.Db[1] = s->bits + S.words;
   S.Rb[0] = s->bits + 2 * (size_t)S.words;
   S.Rb[1] = s->bits + 3 * (size_t)S.words;
+  S.Xb[0] = s->bits + 4 * (size_t)S.words;
+  S.Xb[1] = s->bits + 5 * (size_t)S.words;
   DC_HIP(hipMalloc(&S.acc, sizeof(unsigned long long) * 2 * kAcc * 4));
   DC_HIP(hipMalloc(&S.bstat, sizeof(unsigned long long) * 4 * kMaxBlocks));
   DC_HIP(hipMalloc(&S.ctl, sizeof(LoopCtl)));
   DC_HIP(hipHostMalloc(&s->hctl, sizeof(LoopCtl), hipHostMallocDefault));
   s->hstat = new unsigned long long[4 * kMaxBlocks];
   DC_HIP(hipMemsetAsync(S.K, 0, sizeof(int32_t) * n, g->stream));
-  DC_HIP(hipMemsetAsync(s->bits, 0, sizeof(uint32_t) * 4 * (size_t)S.words, g->stream));
+  DC_HIP(hipMemsetAsync(s->bits, 0, sizeof(uint32_t) * kBitmaps * (size_t)S.words, g->stream));
 
   return DC_OK;
 }
@@ -853,7 +883,7 @@ int loop_check(Graph* g, int64_t* bad) {
   IncState* s = inc_state(g);
   if (s)
     hipLaunchKernelGGL(k_zero_check, dim3(1024), dim3(kBlock), 0, g->stream, s->S.K,
-                       (size_t)g->n, s->bits, 4 * (size_t)s->S.words, cnt);
+                       (size_t)g->n, s->bits, kBitmaps * (size_t)s->S.words, cnt);
   DC_HIP(hipGetLastError());
   unsigned long long h = 0;
   DC_HIP(hipMemcpyAsync(&h, cnt, sizeof(h), hipMemcpyDeviceToHost, g->stream));
@@ -864,7 +894,7 @@ int loop_check(Graph* g, int64_t* bad) {
 
 static int loop_reset(Graph* g, IncState* s) {
   DC_HIP(hipMemsetAsync(s->S.K, 0, sizeof(int32_t) * ((size_t)g->n + 1), g->stream));
-  DC_HIP(hipMemsetAsync(s->bits, 0, sizeof(uint32_t) * 4 * (size_t)s->S.words, g->stream));
+  DC_HIP(hipMemsetAsync(s->bits, 0, sizeof(uint32_t) * kBitmaps * (size_t)s->S.words, g->stream));
   DC_HIP(hipMemsetAsync(g->cbits[0], 0, sizeof(uint32_t) * 2 * g->cbits_words, g->stream));
   return DC_OK;
 }
@@ -904,8 +934,9 @@ int loop_hi(Graph* g, Frontier L0, int32_t nl0, bool persistent, LoopResult* res
   S.L0 = L0;
   S.hch = g->hch;
   S.n_hch = g->n_hch;
-  // rounds whose D rows hold more than 1/32 of all slots recount by pull
-  S.heavy_slots = std::max<unsigned long long>(g->rows_end / 32, 1ull << 16);
+  // rounds whose D rows hold more than half of all slots recount by pull
+  // (measured on R-MAT 22: the delta pass is cheaper below that)
+  S.heavy_slots = std::max<unsigned long long>(g->rows_end / 2, 1ull << 16);
 
   *s->hctl = LoopCtl{};
   s->hctl->done_round = INT_MAX;
@@ -919,6 +950,17 @@ int loop_hi(Graph* g, Frontier L0, int32_t nl0, bool persistent, LoopResult* res
   int64_t launches = 1;
   int grid = kMaxBlocks;
   if (persistent && (rc = persistent_grid(g, &grid))) return rc;
+  // light rounds (small C) launch fewer workgroups: their work is a few
+  // thousand vertices, and dispatching 2048 workgroups twice a round costs
+  // more than it brings
+  int light_grid = kMaxBlocks;
+  {
+    int dev = 0, cus = 0;
+    DC_HIP(hipGetDevice(&dev));
+    DC_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    light_grid = std::min(kMaxBlocks, std::max(32, 2 * cus));
+  }
+  int round_grid = grid;
   double kms = 0.0, cms = 0.0;
   int64_t claunches = 0;
   hipEvent_t e0 = g->ev[2], e1 = g->ev[3];
@@ -942,9 +984,9 @@ int loop_hi(Graph* g, Frontier L0, int32_t nl0, bool persistent, LoopResult* res
       ++launches;
     } else {
       for (int32_t i = 0; i < nb; ++i) {
-        hipLaunchKernelGGL(k_round_a, dim3(grid), blk, 0, g->stream, S, r + i);
+        hipLaunchKernelGGL(k_round_a, dim3(round_grid), blk, 0, g->stream, S, r + i);
         if (g->profiling) DC_HIP(hipEventRecord(s->cev[2 * i], g->stream));
-        hipLaunchKernelGGL(k_round_c, dim3(grid), blk, 0, g->stream, S, r + i);
+        hipLaunchKernelGGL(k_round_c, dim3(round_grid), blk, 0, g->stream, S, r + i);
         if (g->profiling) DC_HIP(hipEventRecord(s->cev[2 * i + 1], g->stream));
       }
       DC_HIP(hipGetLastError());
@@ -969,6 +1011,7 @@ int loop_hi(Graph* g, Frontier L0, int32_t nl0, bool persistent, LoopResult* res
     }
     r += nb;
     if (done != INT_MAX) rounds = done + 1;
+    round_grid = s->hctl->last_nc < kLightC ? light_grid : grid;
     block = std::min(block * 2, persistent ? 4096 : 32);
   }
   if (rounds < 0) {
