@@ -414,67 +414,67 @@ __device__ __forceinline__ DState load_d(const LoopState& S, const uint32_t* cb,
   return d;
 }
 
-// Colour and prev of a member v of C_r seen from a slot holding v's colour
-// cv: a member that kept its colour this round has prev == colour == the slot
-// value (its slot cache is current); only members that changed colour (xb)
-// are gathered, since their slots may not be updated yet.
-__device__ __forceinline__ void member_state(const LoopState& S, const uint32_t* xb, int32_t v,
-                                             int32_t cv, int32_t& c, int32_t& p) {
-  if (has_bit(xb, v)) {
-    c = S.col0[v];
-    p = S.prev[v];
-  } else {
-    c = cv;
-    p = cv;
-  }
-}
-
 // One lane's share of a count-pass visit of x's row: slots lo, lo+stride, ...
-// Returns this lane's part of x's recount.
+// Returns this lane's part of x's recount.  U slots at a time in four
+// stages (slots; their neighbours' bitmap words; count changes and the state
+// of neighbours that changed colour; the recount), each stage's gathers all
+// in flight before the next stage reads them.  x's own colour change goes
+// to the neighbours separately (propagate_slots).
+template <int U>
 __device__ __forceinline__ uint32_t count_slots(const LoopState& S, const DState& d, uint64_t rbase,
                                                 uint32_t lo, uint32_t hi, uint32_t stride,
                                                 const uint32_t* cb, const uint32_t* db,
                                                 const uint32_t* xb, uint32_t* rn,
-                                                unsigned long long& slots,
-                                                unsigned long long& live) {
+                                                unsigned long long& slots) {
   const GraphView& G = S.G;
   uint32_t cnt = 0;
-  for (uint32_t j = lo; j < hi; j += kUnroll * stride) {
-    int32_t y[kUnroll], c[kUnroll];
+  for (uint32_t j = lo; j < hi; j += U * stride) {
+    int32_t y[U], c[U];
 #pragma unroll
-    for (int k = 0; k < kUnroll; ++k) {
+    for (int k = 0; k < U; ++k) {
       const uint32_t jj = j + k * stride;
       const bool in = jj < hi;
-      y[k] = in ? G.col[rbase + jj] : -2;
+      y[k] = in ? G.col[rbase + jj] : -1;
       c[k] = in ? G.ccol[rbase + jj] : -1;
+      slots += in ? 1u : 0u;
     }
+    uint32_t wd[U], wc[U], wx[U];
 #pragma unroll
-    for (int k = 0; k < kUnroll; ++k) {
-      if (y[k] == -2) continue;
-      ++slots;
-      if (y[k] < 0) continue;
-      ++live;
-      const int32_t yy = y[k];
-      if (yy >= G.n_owned) continue;   // frozen ghost: no count, no cache
-      if (!has_bit(db, yy)) {
-        const bool to = d.inl && marks(d.x, d.so, d.po, yy, c[k]);
-        const bool tn = d.inc && marks(d.x, d.sn, d.pn, yy, c[k]);
+    for (int k = 0; k < U; ++k) {
+      // tombstones and frozen ghosts read word 0 and are masked out below
+      const bool own = y[k] >= 0 && y[k] < G.n_owned;
+      const uint32_t wi = own ? ((uint32_t)y[k] >> 5) : 0u;
+      wd[k] = db[wi];
+      wc[k] = cb[wi];
+      wx[k] = xb[wi];
+    }
+    int32_t gc[U], gp[U];
+    bool mem[U];
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+      const bool own = y[k] >= 0 && y[k] < G.n_owned;
+      const uint32_t bit = 1u << (y[k] & 31);
+      if (own && !(wd[k] & bit)) {
+        const bool to = d.inl && marks(d.x, d.so, d.po, y[k], c[k]);
+        const bool tn = d.inc && marks(d.x, d.sn, d.pn, y[k], c[k]);
         if (to != tn) {
-          atomicAdd(&S.K[yy], tn ? 1 : -1);
-          if (tn) set_bit(rn, yy);
+          atomicAdd(&S.K[y[k]], tn ? 1 : -1);
+          if (tn) set_bit(rn, y[k]);
         }
       }
-      if (has_bit(cb, yy)) {
-        int32_t cy, py;
-        member_state(S, xb, yy, c[k], cy, py);
-        if (marks(yy, cy, py, d.x, d.sn)) ++cnt;
-      }
-      if (d.chg) {
-        const uint64_t tslot = rbase + (j + k * stride);
-        G.ccol[(uint64_t)G.row[yy].x + (uint32_t)G.twin[tslot]] = d.sn;
-        neighbour_follows(G, yy, d.sn, d.so);
+      mem[k] = own && (wc[k] & bit);
+      // a member that kept its colour has prev == colour == the slot value;
+      // one that changed is gathered (its slots may not be updated yet)
+      gc[k] = c[k];
+      gp[k] = c[k];
+      if (mem[k] && (wx[k] & bit)) {
+        gc[k] = S.col0[y[k]];
+        gp[k] = S.prev[y[k]];
       }
     }
+#pragma unroll
+    for (int k = 0; k < U; ++k)
+      if (mem[k] && marks(y[k], gc[k], gp[k], d.x, d.sn)) ++cnt;
   }
   return cnt;
 }
@@ -499,20 +499,35 @@ __device__ __forceinline__ uint32_t pull_slots(const LoopState& S, int32_t y, in
 #pragma unroll
     for (int k = 0; k < U; ++k) {
       const uint32_t jj = j + k * stride;
-      x[k] = jj < hi ? G.col[rbase + jj] : -2;
-      c[k] = jj < hi ? G.ccol[rbase + jj] : -1;
+      const bool in = jj < hi;
+      x[k] = in ? G.col[rbase + jj] : -1;
+      c[k] = in ? G.ccol[rbase + jj] : -1;
+      slots += in ? 1u : 0u;
     }
+    uint32_t wc[U], wx[U];
 #pragma unroll
     for (int k = 0; k < U; ++k) {
-      if (x[k] == -2) continue;
-      ++slots;
-      const int32_t xx = x[k];
-      if (xx >= 0 && xx < G.n_owned && has_bit(cb, xx)) {
-        int32_t cx, px;
-        member_state(S, xb, xx, c[k], cx, px);
-        if (marks(xx, cx, px, y, cy)) ++cnt;
+      const bool own = x[k] >= 0 && x[k] < G.n_owned;
+      const uint32_t wi = own ? ((uint32_t)x[k] >> 5) : 0u;
+      wc[k] = cb[wi];
+      wx[k] = xb[wi];
+    }
+    int32_t gc[U], gp[U];
+    bool mem[U];
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+      const uint32_t bit = 1u << (x[k] & 31);
+      mem[k] = x[k] >= 0 && x[k] < G.n_owned && (wc[k] & bit);
+      gc[k] = c[k];
+      gp[k] = c[k];
+      if (mem[k] && (wx[k] & bit)) {
+        gc[k] = S.col0[x[k]];
+        gp[k] = S.prev[x[k]];
       }
     }
+#pragma unroll
+    for (int k = 0; k < U; ++k)
+      if (mem[k] && marks(x[k], gc[k], gp[k], y, cy)) ++cnt;
   }
   return cnt;
 }
@@ -558,7 +573,7 @@ __device__ __forceinline__ void pull_all(const LoopState& S, const uint32_t* cb,
     for (uint32_t q = 0; q < nm; ++q) {
       const int32_t x = lst[q];
       const uint2 r2 = G.row[x];
-      uint32_t c2 = pull_slots<4>(S, x, S.col0[x], r2.x, lane, r2.y, 64, cb, xb, slots);
+      uint32_t c2 = pull_slots<8>(S, x, S.col0[x], r2.x, lane, r2.y, 64, cb, xb, slots);
       for (int off = 32; off > 0; off >>= 1) c2 += __shfl_xor(c2, off);
       if (lane == 0) {
         set_count(S, x, c2, cb, rn);
@@ -575,7 +590,7 @@ __device__ __forceinline__ void pull_all(const LoopState& S, const uint32_t* cb,
     const int32_t y = e.x;
     const uint2 rw = G.row[y];
     if (rw.y < kMidLen) continue;  // went by the short/mid pass (uniform)
-    const uint32_t cnt = pull_slots<4>(S, y, S.col0[y], rw.x, threadIdx.x, rw.y, kBlock, cb, xb, slots);
+    const uint32_t cnt = pull_slots<8>(S, y, S.col0[y], rw.x, threadIdx.x, rw.y, kBlock, cb, xb, slots);
     const uint32_t tot = block_sum(cnt, lds.red);
     if (threadIdx.x == 0) {
       set_count(S, y, tot, cb, rn);
@@ -689,16 +704,19 @@ __device__ __forceinline__ void round_c(const LoopState& S, int r, LoopLds& lds)
     clear_words(db, S.words);
     return;
   }
-  unsigned long long slots = 0, live = 0, pslots = 0, rows = 0;
+  unsigned long long slots = 0, pslots = 0, rows = 0;
   if (heavy) {
     pull_all(S, cb, xb, rn, lds, pslots, rows);
     propagate_changed(S, db, lds);
   } else {
-  // short and mid rows of D_r (hubs go by the chunk table below)
+  // short rows of D_r by 8-lane groups, mid rows by the wave (hubs go by the
+  // chunk table below); a row whose owner changed colour then updates its
+  // neighbours' caches
   walk_bits(S.words, list, [&](uint32_t i) { return db[i]; }, [&](int32_t* lst, uint32_t total) {
+    const int grp = lane >> 3, sub = lane & 7;
     uint32_t nw = 0;
-    for (uint32_t base = 0; base < total; base += 64) {
-      const uint32_t i = base + lane;
+    for (uint32_t base = 0; base < total; base += 8) {
+      const uint32_t i = base + grp;
       const bool valid = i < total;
       const int32_t x = valid ? lst[i] : 0;
       const uint2 rw = valid ? G.row[x] : make_uint2(0u, 0u);
@@ -706,13 +724,20 @@ __device__ __forceinline__ void round_c(const LoopState& S, int r, LoopLds& lds)
       const bool mid = valid && rw.y >= kSmallLen && rw.y < kMidLen;
       if (small) {
         const DState d = load_d(S, cb, x);
-        S.K[x] = (int32_t)count_slots(S, d, rw.x, 0, rw.y, 1, cb, db, xb, rn, slots, live);
-        ++rows;
+        uint32_t cnt = count_slots<4>(S, d, rw.x, sub, rw.y, kGroup, cb, db, xb, rn, slots);
+        cnt += __shfl_xor(cnt, 1);
+        cnt += __shfl_xor(cnt, 2);
+        cnt += __shfl_xor(cnt, 4);
+        if (sub == 0) {
+          S.K[x] = (int32_t)cnt;
+          ++rows;
+        }
+        if (d.chg) propagate_slots(G, d.sn, d.so, rw.x, sub, rw.y, kGroup);
       }
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
       __builtin_amdgcn_wave_barrier();
-      const unsigned long long mm = __ballot(mid);
-      if (mid) lst[nw + lanes_below(mm)] = x;
+      const unsigned long long mm = __ballot(mid && sub == 0);
+      if (mid && sub == 0) lst[nw + lanes_below(mm)] = x;
       nw += (uint32_t)__popcll(mm);
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
       __builtin_amdgcn_wave_barrier();
@@ -721,12 +746,13 @@ __device__ __forceinline__ void round_c(const LoopState& S, int r, LoopLds& lds)
       const int32_t x = lst[q];
       const uint2 rw = G.row[x];
       const DState d = load_d(S, cb, x);
-      uint32_t cnt = count_slots(S, d, rw.x, lane, rw.y, 64, cb, db, xb, rn, slots, live);
+      uint32_t cnt = count_slots<8>(S, d, rw.x, lane, rw.y, 64, cb, db, xb, rn, slots);
       for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off);
       if (lane == 0) {
         S.K[x] = (int32_t)cnt;
         ++rows;
       }
+      if (d.chg) propagate_slots(G, d.sn, d.so, rw.x, lane, rw.y, 64);
     }
   });
   // hub rows of D_r, one wave per kChunk-slot chunk (K[x] was zeroed in A).
@@ -750,18 +776,18 @@ __device__ __forceinline__ void round_c(const LoopState& S, int r, LoopLds& lds)
         // rows now shorter than kMidLen went by the short/mid path
         if (rw.y < kMidLen || lo >= rw.y) continue;
         const DState d = load_d(S, cb, x);
-        uint32_t cnt = count_slots(S, d, rw.x, lo + lane, min(rw.y, lo + kChunk), 64, cb, db, xb,
-                                   rn, slots, live);
+        const uint32_t hi = min(rw.y, lo + kChunk);
+        uint32_t cnt = count_slots<8>(S, d, rw.x, lo + lane, hi, 64, cb, db, xb, rn, slots);
         for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off);
         if (lane == 0) {
           if (cnt) atomicAdd(&S.K[x], (int32_t)cnt);
           ++rows;
         }
+        if (d.chg) propagate_slots(G, d.sn, d.so, rw.x, lo + lane, hi, 64);
       }
     }
   }
   }
-  (void)live;
   for (int off = 32; off > 0; off >>= 1) {
     slots += __shfl_xor(slots, off);
     pslots += __shfl_xor(pslots, off);
