@@ -460,33 +460,60 @@ __device__ __forceinline__ void hub_finalize_body(const RecolorArgs& A) {
 // neighbours of w change in the same round this stays safe: a colour below m
 // that ends at count 0 had a decrement that left it at 0, and colour m can
 // only be gained by a change to m.
-__device__ __forceinline__ void neighbour_follows(const GraphView& G, int32_t w, int32_t c,
-                                                  int32_t old) {
-  const int32_t h = G.hidx[w];
-  const int32_t m = G.mexc[w];
-  bool stale;
-  if (h >= 0) {
-    uint32_t* hr = G.hist + (uint64_t)h * kHistW;
-    uint32_t before = 2u;
-    if (old >= 0 && old < kHistW) before = atomicSub(&hr[old], 1u);
-    if (c < kHistW) atomicAdd(&hr[c], 1u);
-    stale = (old < m && (old >= kHistW || before == 1u)) || c == m;
-  } else {
-    stale = old < m || c == m;
-  }
-  if (m >= 0 && stale) G.mexc[w] = -1;
-}
-
 // (v changed from colour `old` to `c`: each owned neighbour's slot cache,
-// histogram and cached mex follow)
+// histogram and cached mex follow.)  8 slots per step in three stages, each
+// stage's gathers in flight together: the slots and their twins; the
+// neighbours' row start, histogram index and cached mex; the histogram
+// updates, after which a cached mex that the change may have moved is
+// dropped.
 __device__ __forceinline__ void propagate_slots(const GraphView& G, int32_t c, int32_t old,
                                                 uint64_t rbase, uint32_t lo, uint32_t hi,
                                                 uint32_t stride) {
-  for (uint32_t j = lo; j < hi; j += stride) {
-    const int32_t w = G.col[rbase + j];
-    if (w < 0 || w >= G.n_owned) continue;
-    G.ccol[(uint64_t)G.row[w].x + (uint32_t)G.twin[rbase + j]] = c;
-    neighbour_follows(G, w, c, old);
+  constexpr int U = 8;
+  for (uint32_t j = lo; j < hi; j += U * stride) {
+    int32_t w[U], t[U];
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+      const uint32_t jj = j + k * stride;
+      const bool in = jj < hi;
+      w[k] = in ? G.col[rbase + jj] : -1;
+      t[k] = in ? G.twin[rbase + jj] : 0;
+    }
+    uint32_t rx[U];
+    int32_t h[U], m[U];
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+      const bool own = w[k] >= 0 && w[k] < G.n_owned;
+      rx[k] = 0u;
+      h[k] = -1;
+      m[k] = -1;
+      if (own) {
+        rx[k] = G.row[w[k]].x;
+        h[k] = G.hidx[w[k]];
+        m[k] = G.mexc[w[k]];
+      }
+    }
+    uint32_t before[U];
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+      const bool own = w[k] >= 0 && w[k] < G.n_owned;
+      before[k] = 2u;
+      if (own) {
+        G.ccol[(uint64_t)rx[k] + (uint32_t)t[k]] = c;
+        if (h[k] >= 0) {
+          uint32_t* hr = G.hist + (uint64_t)h[k] * kHistW;
+          if (old >= 0 && old < kHistW) before[k] = atomicSub(&hr[old], 1u);
+          if (c < kHistW) atomicAdd(&hr[c], 1u);
+        }
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+      const bool own = w[k] >= 0 && w[k] < G.n_owned;
+      const bool stale = h[k] >= 0 ? (old < m[k] && (old >= kHistW || before[k] == 1u)) || c == m[k]
+                                   : old < m[k] || c == m[k];
+      if (own && m[k] >= 0 && stale) G.mexc[w[k]] = -1;
+    }
   }
 }
 

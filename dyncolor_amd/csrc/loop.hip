@@ -253,38 +253,46 @@ __device__ __forceinline__ bool commit_member(const LoopState& S, uint32_t* cb, 
   return false;
 }
 
-// mex of a row with fewer than 64 slots, by one lane (8 loads in flight)
-__device__ __forceinline__ int32_t lane_small_mex(const GraphView& G, uint2 r,
-                                                  unsigned long long& slots) {
-  unsigned long long mask = 0ull;
-  for (uint32_t j = 0; j < r.y; j += 8) {
-    int32_t c[8];
+// mex of a row with fewer than 64 slots by an 8-lane group (lane `sub` reads
+// slots sub, sub + 8, ...: the whole row in one round trip)
+__device__ __forceinline__ int32_t group_small_mex(const GraphView& G, uint2 r, int sub) {
+  int32_t c[8];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) c[k] = j + k < r.y ? G.ccol[(uint64_t)r.x + j + k] : -1;
-#pragma unroll
-    for (int k = 0; k < 8; ++k)
-      if (c[k] >= 0 && c[k] < 64) mask |= 1ull << c[k];
+  for (int k = 0; k < 8; ++k) {
+    const uint32_t j = (uint32_t)sub + 8u * k;
+    c[k] = j < r.y ? G.ccol[(uint64_t)r.x + j] : -1;
   }
-  slots += r.y;
+  unsigned long long mask = 0ull;
+#pragma unroll
+  for (int k = 0; k < 8; ++k)
+    if (c[k] >= 0 && c[k] < 64) mask |= 1ull << c[k];
+  mask |= __shfl_xor(mask, 1);
+  mask |= __shfl_xor(mask, 2);
+  mask |= __shfl_xor(mask, 4);
   return __ffsll((long long)~mask) - 1;
 }
 
-// mex from a colour histogram by one lane: 32 counts per step (8 loads in
-// flight); -1 when colours 0..kHistW-1 are all present
-__device__ __forceinline__ int32_t lane_hist_mex(const uint32_t* hrow) {
+// mex from a colour histogram by an 8-lane group: 256 counts per step (lane
+// `sub` reads 32 of them); -1 when colours 0..kHistW-1 are all present
+__device__ __forceinline__ int32_t group_hist_mex(const uint32_t* hrow, int sub) {
   const uint4* p = reinterpret_cast<const uint4*>(hrow);
-  for (int q = 0; q < kHistW / 4; q += 8) {
+  for (int step = 0; step < kHistW / 256; ++step) {
     uint4 v[8];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) v[k] = p[q + k];
+    for (int k = 0; k < 8; ++k) v[k] = p[step * 64 + sub * 8 + k];
+    int first = INT_MAX;
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      const int c = 4 * (q + k);
-      if (v[k].x == 0u) return c;
-      if (v[k].y == 0u) return c + 1;
-      if (v[k].z == 0u) return c + 2;
-      if (v[k].w == 0u) return c + 3;
+    for (int k = 7; k >= 0; --k) {
+      const int c = step * 256 + sub * 32 + 4 * k;
+      if (v[k].w == 0u) first = c + 3;
+      if (v[k].z == 0u) first = c + 2;
+      if (v[k].y == 0u) first = c + 1;
+      if (v[k].x == 0u) first = c;
     }
+    first = min(first, __shfl_xor(first, 1));
+    first = min(first, __shfl_xor(first, 2));
+    first = min(first, __shfl_xor(first, 4));
+    if (first != INT_MAX) return first;
   }
   return -1;
 }
@@ -309,7 +317,9 @@ __device__ __forceinline__ void round_a(const LoopState& S, int r, LoopLds& lds)
 
   walk_bits(S.words, list, [&](uint32_t i) { return Lb[i] | Dp[i] | Rp[i]; },
             [&](int32_t* lst, uint32_t total) {
-    uint32_t nw = 0;  // members whose mex is past the histogram, at the front of the list
+    // members with a cached mex commit at once; the others (stale mex) are
+    // compacted to the front of the list and evaluated by 8-lane groups
+    uint32_t nd = 0;
     for (uint32_t base = 0; base < total; base += 64) {
       const uint32_t i = base + lane;
       const bool valid = i < total;
@@ -325,33 +335,57 @@ __device__ __forceinline__ void round_a(const LoopState& S, int r, LoopLds& lds)
       }
       const bool member = valid && k > 0;
       const bool leaving = valid && !member && inl;
-      // a stale mex: the row's colour histogram, or a scan of a short row
-      int32_t m = cm;
-      if (member && cm < 0) {
-        const int32_t h = G.hidx[w];
-        if (h >= 0) {
-          m = lane_hist_mex(G.hist + (uint64_t)h * kHistW);
-        } else {
-          m = lane_small_mex(G, G.row[w], slots);
-        }
-      }
-      const bool wstale = member && m < 0;  // colours 0..kHistW-1 all present
       bool dv = leaving;
       if (leaving) {
         S.dold[w] = make_int2(so, (po + 1) | (int32_t)0x80000000);
         set_bit(Dn, w);
       }
-      if (member && !wstale) dv = commit_member(S, Cb, Dn, Xn, w, cm, so, po, inl, m, changed);
+      if (member && cm >= 0) dv = commit_member(S, Cb, Dn, Xn, w, cm, so, po, inl, cm, changed);
       if (dv) {
         const uint32_t len = G.row[w].y;
         dsum += len;
         dhubs += len >= kMidLen ? 1u : 0u;
       }
       members += member ? 1u : 0u;
+      const bool defer = member && cm < 0;
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
       __builtin_amdgcn_wave_barrier();
-      const unsigned long long wm = __ballot(wstale);
-      if (wstale) lst[nw + lanes_below(wm)] = w;
+      const unsigned long long dm = __ballot(defer);
+      if (defer) lst[nd + lanes_below(dm)] = w;
+      nd += (uint32_t)__popcll(dm);
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+    }
+    // stale mex: a short row's slots or the row's colour histogram; a mex
+    // past the histogram goes to the front of the list again (nw)
+    const int grp = lane >> 3, sub = lane & 7;
+    uint32_t nw = 0;
+    for (uint32_t b8 = 0; b8 < nd; b8 += 8) {
+      const uint32_t q = b8 + grp;
+      const bool valid = q < nd;
+      const int32_t w = valid ? lst[q] : 0;
+      int32_t m = -1;
+      if (valid) {
+        const int32_t h = G.hidx[w];
+        if (h >= 0) {
+          m = group_hist_mex(G.hist + (uint64_t)h * kHistW, sub);
+        } else {
+          const uint2 rw = G.row[w];
+          m = group_small_mex(G, rw, sub);
+          if (sub == 0) slots += rw.y;
+        }
+      }
+      const bool wst = valid && m < 0;
+      if (valid && !wst && sub == 0 &&
+          commit_member(S, Cb, Dn, Xn, w, -1, S.col0[w], S.prev[w], has_bit(Lb, w), m, changed)) {
+        const uint32_t len = G.row[w].y;
+        dsum += len;
+        dhubs += len >= kMidLen ? 1u : 0u;
+      }
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      const unsigned long long wm = __ballot(wst && sub == 0);
+      if (wst && sub == 0) lst[nw + lanes_below(wm)] = w;
       nw += (uint32_t)__popcll(wm);
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
       __builtin_amdgcn_wave_barrier();
@@ -438,7 +472,7 @@ __device__ __forceinline__ uint32_t count_slots(const LoopState& S, const DState
       c[k] = in ? G.ccol[rbase + jj] : -1;
       slots += in ? 1u : 0u;
     }
-    uint32_t wd[U], wc[U], wx[U];
+    uint32_t wd[U], wc[U], wx[U], wr[U];
 #pragma unroll
     for (int k = 0; k < U; ++k) {
       // tombstones and frozen ghosts read word 0 and are masked out below
@@ -447,6 +481,7 @@ __device__ __forceinline__ uint32_t count_slots(const LoopState& S, const DState
       wd[k] = db[wi];
       wc[k] = cb[wi];
       wx[k] = xb[wi];
+      wr[k] = rn[wi];   // R_r only gains bits in this phase: a set bit stays set
     }
     int32_t gc[U], gp[U];
     bool mem[U];
@@ -459,7 +494,7 @@ __device__ __forceinline__ uint32_t count_slots(const LoopState& S, const DState
         const bool tn = d.inc && marks(d.x, d.sn, d.pn, y[k], c[k]);
         if (to != tn) {
           atomicAdd(&S.K[y[k]], tn ? 1 : -1);
-          if (tn) set_bit(rn, y[k]);
+          if (tn && !(wr[k] & bit)) set_bit(rn, y[k]);
         }
       }
       mem[k] = own && (wc[k] & bit);
