@@ -102,6 +102,8 @@ struct LoopState {
 // LDS of a loop workgroup, shared by both phases
 struct LoopLds {
   int32_t lists[kBlock / 64][kWaveList];
+  int2 chunks[kBlock];            // active hub chunks of one table slice
+  uint32_t nchunks[kBlock / 64];
   unsigned long long wins[kBlock / 64][kWinWords];
   uint32_t red[kBlock / 64];
   unsigned long long red64[kBlock / 64];
@@ -186,6 +188,45 @@ __device__ __forceinline__ unsigned long long block_sum64(unsigned long long v,
   for (int i = 0; i < (int)(blockDim.x >> 6); ++i) t += scratch[i];
   __syncthreads();
   return t;
+}
+
+// Hub chunks whose entry passes act(e): the workgroup takes the table entries
+// blockIdx, blockIdx + gridDim, ... (consecutive chunks of one hub go to
+// different workgroups), compacts the active ones in LDS, and runs each chunk
+// with all its waves: wave q of the workgroup calls fn(x, row, lo, hi) on
+// the q-th quarter of the chunk's slots.  Every thread must call this (it
+// synchronises the workgroup).
+template <class Act, class Fn>
+__device__ __forceinline__ void each_hub_chunk(const LoopState& S, LoopLds& lds, Act act, Fn fn) {
+  const GraphView& G = S.G;
+  const int lane = lane_id();
+  const int wib = threadIdx.x >> 6;
+  constexpr uint32_t kQuarter = kChunk / (kBlock / 64);
+  for (uint64_t base = 0; base < S.n_hch; base += (uint64_t)kBlock * gridDim.x) {
+    const uint64_t i = base + (uint64_t)threadIdx.x * gridDim.x + blockIdx.x;
+    const int2 e = i < S.n_hch ? S.hch[i] : make_int2(0, -1);
+    const bool a = e.y >= 0 && act(e);
+    const unsigned long long am = __ballot(a);
+    if (lane == 0) lds.nchunks[wib] = (uint32_t)__popcll(am);
+    __syncthreads();
+    uint32_t off = 0, n = 0;
+    for (int q = 0; q < kBlock / 64; ++q) {
+      const uint32_t c = lds.nchunks[q];
+      off += q < wib ? c : 0u;
+      n += c;
+    }
+    if (a) lds.chunks[off + lanes_below(am)] = e;
+    __syncthreads();
+    for (uint32_t q = 0; q < n; ++q) {
+      const int2 ce = lds.chunks[q];
+      const uint2 rw = G.row[ce.x];
+      const uint32_t lo = (uint32_t)ce.y * kChunk + (uint32_t)wib * kQuarter;
+      // rows now shorter than kMidLen went by the short/mid path
+      if (rw.y < kMidLen || lo >= rw.y) continue;
+      fn(ce.x, rw, lo, min(rw.y, lo + kQuarter));
+    }
+    __syncthreads();
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -671,28 +712,13 @@ __device__ __forceinline__ void propagate_changed(const LoopState& S, uint32_t* 
       propagate_slots(G, S.col0[x], S.dold[x].x, rw.x, lane, rw.y, 64);
     }
   });
-  const uint32_t gwave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
-  for (uint64_t base = 0; base < S.n_hch; base += 64ull * nwaves) {
-    const uint64_t i = base + (uint64_t)lane * nwaves + gwave;
-    const int2 e = i < S.n_hch ? S.hch[i] : make_int2(0, -1);
-    const bool act = e.y >= 0 && has_bit(db, e.x) && S.col0[e.x] != S.dold[e.x].x;
-    unsigned long long am = __ballot(act);
-    while (am) {
-      const int l = __ffsll((long long)am) - 1;
-      am &= am - 1ull;
-      const int32_t x = __shfl(e.x, l);
-      const uint32_t c = (uint32_t)__shfl(e.y, l);
-      const uint2 rw = G.row[x];
-      const uint32_t lo = c * kChunk;
-      if (rw.y < kMidLen || lo >= rw.y) continue;
-      propagate_slots(G, S.col0[x], S.dold[x].x, rw.x, lo + lane, min(rw.y, lo + kChunk), 64);
-    }
-  }
+  each_hub_chunk(S, lds, [&](int2 e) { return has_bit(db, e.x) && S.col0[e.x] != S.dold[e.x].x; },
+                 [&](int32_t x, uint2 rw, uint32_t lo, uint32_t hi) {
+    propagate_slots(G, S.col0[x], S.dold[x].x, rw.x, lo + lane, hi, 64);
+  });
 }
 
 __device__ __forceinline__ void round_c(const LoopState& S, int r, LoopLds& lds) {
-  uint32_t* red = lds.red;
   const GraphView& G = S.G;
   const uint32_t* cb = S.B[r & 1];
   uint32_t* db = S.Db[r & 1];
@@ -794,33 +820,17 @@ __device__ __forceinline__ void round_c(const LoopState& S, int r, LoopLds& lds)
   // A hub's chunks are adjacent in the table; lane l of wave w looks at entry
   // l * nwaves + w, so they go to different waves.
   if (dhubs) {
-    const uint32_t gwave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
-    for (uint64_t base = 0; base < S.n_hch; base += 64ull * nwaves) {
-      const uint64_t i = base + (uint64_t)lane * nwaves + gwave;
-      const int2 e = i < S.n_hch ? S.hch[i] : make_int2(0, -1);
-      const bool act = e.y >= 0 && has_bit(db, e.x);
-      unsigned long long am = __ballot(act);
-      while (am) {
-        const int l = __ffsll((long long)am) - 1;
-        am &= am - 1ull;
-        const int32_t x = __shfl(e.x, l);
-        const uint32_t c = (uint32_t)__shfl(e.y, l);
-        const uint2 rw = G.row[x];
-        const uint32_t lo = c * kChunk;
-        // rows now shorter than kMidLen went by the short/mid path
-        if (rw.y < kMidLen || lo >= rw.y) continue;
-        const DState d = load_d(S, cb, x);
-        const uint32_t hi = min(rw.y, lo + kChunk);
-        uint32_t cnt = count_slots<8>(S, d, rw.x, lo + lane, hi, 64, cb, db, xb, rn, slots);
-        for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off);
-        if (lane == 0) {
-          if (cnt) atomicAdd(&S.K[x], (int32_t)cnt);
-          ++rows;
-        }
-        if (d.chg) propagate_slots(G, d.sn, d.so, rw.x, lo + lane, hi, 64);
+    each_hub_chunk(S, lds, [&](int2 e) { return has_bit(db, e.x); },
+                   [&](int32_t x, uint2 rw, uint32_t lo, uint32_t hi) {
+      const DState d = load_d(S, cb, x);
+      uint32_t cnt = count_slots<4>(S, d, rw.x, lo + lane, hi, 64, cb, db, xb, rn, slots);
+      for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off);
+      if (lane == 0) {
+        if (cnt) atomicAdd(&S.K[x], (int32_t)cnt);
+        if (threadIdx.x == 0) ++rows;
       }
-    }
+      if (d.chg) propagate_slots(G, d.sn, d.so, rw.x, lo + lane, hi, 64);
+    });
   }
   }
   for (int off = 32; off > 0; off >>= 1) {
