@@ -61,6 +61,7 @@ namespace dc {
 
 constexpr int kMaxBlocks = 2048;   // grid of the loop kernels (per-block partials)
 constexpr int kAcc = 32;           // phase A sums: workgroup b adds into slot b % kAcc
+constexpr int kAccW = 8;           // words per slot (five used)
 constexpr int kBitmaps = 6;        // Db, Rb, Xb (two each) besides the C/L pair
 constexpr int32_t kLightC = 4096;  // rounds after one with fewer members use the light grid
 constexpr int kWaveList = 64 * 32; // set bits of 64 bitmap words
@@ -87,11 +88,14 @@ struct LoopState {
   Frontier L0;         // the marked set
   const int2* hch;     // hub chunk table {vertex, chunk}
   uint32_t n_hch;
+  const int32_t* hrowv;  // rows with a colour histogram (capacity >= kSmallLen)
+  uint32_t n_hrows;
   unsigned long long heavy_slots;  // D_r rows beyond this many slots: pull recount
   // phase A's totals by round parity, spread over kAcc slots so that no
   // address takes more than gridDim / kAcc atomics: {members, changes, slots
-  // of the rows put into D, hub rows put into D}
-  unsigned long long* acc;    // [2][kAcc][4]
+  // of the rows put into D, hub rows put into D, histogram rows below hub
+  // length put into D}
+  unsigned long long* acc;    // [2][kAcc][kAccW]
   // per workgroup: {stale-scan slots (A), pull slots (C), count slots (C),
   // row and chunk visits (C)}
   unsigned long long* bstat;  // [kMaxBlocks][4]
@@ -107,7 +111,7 @@ struct LoopLds {
   unsigned long long wins[kBlock / 64][kWinWords];
   uint32_t red[kBlock / 64];
   unsigned long long red64[kBlock / 64];
-  unsigned long long tot[4];
+  unsigned long long tot[5];
 };
 
 struct IncState {
@@ -224,6 +228,43 @@ __device__ __forceinline__ void each_hub_chunk(const LoopState& S, LoopLds& lds,
       // rows now shorter than kMidLen went by the short/mid path
       if (rw.y < kMidLen || lo >= rw.y) continue;
       fn(ce.x, rw, lo, min(rw.y, lo + kQuarter));
+    }
+    __syncthreads();
+  }
+}
+
+// Rows of the histogram-row table (every row of kSmallLen or more slots of
+// capacity) that pass act(v) and are shorter than kMidLen (longer ones go by
+// the hub chunk table): the workgroup takes entries blockIdx, blockIdx +
+// gridDim, ..., compacts the active ones in LDS and its waves take them in
+// turn, one row per wave: fn(x, row).  The table spreads these rows evenly
+// over the workgroups whatever their vertex IDs.  Every thread must call
+// this (it synchronises the workgroup).
+template <class Act, class Fn>
+__device__ __forceinline__ void each_mid_row(const LoopState& S, LoopLds& lds, Act act, Fn fn) {
+  const GraphView& G = S.G;
+  const int lane = lane_id();
+  const int wib = threadIdx.x >> 6;
+  for (uint64_t base = 0; base < S.n_hrows; base += (uint64_t)kBlock * gridDim.x) {
+    const uint64_t i = base + (uint64_t)threadIdx.x * gridDim.x + blockIdx.x;
+    const int32_t v = i < S.n_hrows ? S.hrowv[i] : -1;
+    const bool a = v >= 0 && act(v);
+    const unsigned long long am = __ballot(a);
+    if (lane == 0) lds.nchunks[wib] = (uint32_t)__popcll(am);
+    __syncthreads();
+    uint32_t off = 0, n = 0;
+    for (int q = 0; q < kBlock / 64; ++q) {
+      const uint32_t c = lds.nchunks[q];
+      off += q < wib ? c : 0u;
+      n += c;
+    }
+    if (a) lds.chunks[off + lanes_below(am)] = make_int2(v, 0);
+    __syncthreads();
+    for (uint32_t q = (uint32_t)wib; q < n; q += kBlock / 64) {
+      const int32_t x = lds.chunks[q].x;
+      const uint2 rw = G.row[x];
+      if (rw.y >= kMidLen) continue;
+      fn(x, rw);
     }
     __syncthreads();
   }
@@ -353,8 +394,16 @@ __device__ __forceinline__ void round_a(const LoopState& S, int r, LoopLds& lds)
   unsigned long long* win = lds.wins[wib];
   RecolorArgs RA{};
   RA.G = G;
-  uint32_t members = 0, changed = 0, dhubs = 0;
-  unsigned long long slots = 0, live = 0, dsum = 0;
+  uint32_t members = 0, changed = 0, dhubs = 0, dmids = 0;
+  unsigned long long slots = 0, dsum = 0;
+  // a row put into D: its slots, and which count-pass path takes it
+  auto d_row = [&](int32_t w) {
+    const uint32_t len = G.row[w].y;
+    const int32_t h = G.hidx[w];
+    dsum += len;
+    dhubs += len >= kMidLen ? 1u : 0u;
+    dmids += len < kMidLen && h >= 0 ? 1u : 0u;
+  };
 
   walk_bits(S.words, list, [&](uint32_t i) { return Lb[i] | Dp[i] | Rp[i]; },
             [&](int32_t* lst, uint32_t total) {
@@ -382,11 +431,7 @@ __device__ __forceinline__ void round_a(const LoopState& S, int r, LoopLds& lds)
         set_bit(Dn, w);
       }
       if (member && cm >= 0) dv = commit_member(S, Cb, Dn, Xn, w, cm, so, po, inl, cm, changed);
-      if (dv) {
-        const uint32_t len = G.row[w].y;
-        dsum += len;
-        dhubs += len >= kMidLen ? 1u : 0u;
-      }
+      if (dv) d_row(w);
       members += member ? 1u : 0u;
       const bool defer = member && cm < 0;
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -418,11 +463,8 @@ __device__ __forceinline__ void round_a(const LoopState& S, int r, LoopLds& lds)
       }
       const bool wst = valid && m < 0;
       if (valid && !wst && sub == 0 &&
-          commit_member(S, Cb, Dn, Xn, w, -1, S.col0[w], S.prev[w], has_bit(Lb, w), m, changed)) {
-        const uint32_t len = G.row[w].y;
-        dsum += len;
-        dhubs += len >= kMidLen ? 1u : 0u;
-      }
+          commit_member(S, Cb, Dn, Xn, w, -1, S.col0[w], S.prev[w], has_bit(Lb, w), m, changed))
+        d_row(w);
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
       __builtin_amdgcn_wave_barrier();
       const unsigned long long wm = __ballot(wst && sub == 0);
@@ -436,28 +478,24 @@ __device__ __forceinline__ void round_a(const LoopState& S, int r, LoopLds& lds)
       const int32_t w = lst[q];
       const int32_t m = wave_row_mex<M_LOOP_HI>(RA, w, win);
       if (lane == 0 &&
-          commit_member(S, Cb, Dn, Xn, w, -1, S.col0[w], S.prev[w], has_bit(Lb, w), m, changed)) {
-        const uint32_t len = G.row[w].y;
-        dsum += len;
-        dhubs += len >= kMidLen ? 1u : 0u;
-      }
+          commit_member(S, Cb, Dn, Xn, w, -1, S.col0[w], S.prev[w], has_bit(Lb, w), m, changed))
+        d_row(w);
     }
   });
 
   const uint32_t bm = block_sum(members, red);
   const uint32_t bc = block_sum(changed, red);
   const uint32_t bh = block_sum(dhubs, red);
+  const uint32_t bmid = block_sum(dmids, red);
   const unsigned long long bd = block_sum64(dsum, lds.red64);
-  for (int off = 32; off > 0; off >>= 1) {
-    slots += __shfl_xor(slots, off);
-    live += __shfl_xor(live, off);
-  }
+  for (int off = 32; off > 0; off >>= 1) slots += __shfl_xor(slots, off);
   if (threadIdx.x == 0) {
-    unsigned long long* a = S.acc + ((size_t)(r & 1) * kAcc + blockIdx.x % kAcc) * 4;
+    unsigned long long* a = S.acc + ((size_t)(r & 1) * kAcc + blockIdx.x % kAcc) * kAccW;
     if (bm) atomicAdd(&a[0], (unsigned long long)bm);
     if (bc) atomicAdd(&a[1], (unsigned long long)bc);
     if (bd) atomicAdd(&a[2], bd);
     if (bh) atomicAdd(&a[3], (unsigned long long)bh);
+    if (bmid) atomicAdd(&a[4], (unsigned long long)bmid);
   }
   if (lane == 0 && slots) atomicAdd(&S.bstat[blockIdx.x * 4 + 0], slots);
 }
@@ -619,20 +657,17 @@ __device__ __forceinline__ void pull_all(const LoopState& S, const uint32_t* cb,
                                          unsigned long long& slots, unsigned long long& rows) {
   const GraphView& G = S.G;
   const int lane = lane_id();
-  int32_t* lst = lds.lists[threadIdx.x >> 6];
   const uint32_t gwave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
-  // short rows by 8-lane groups, mid rows by the wave; consecutive groups of
-  // 8 vertices go to different waves, so the dense low IDs spread out
+  // rows without a histogram (fewer than kSmallLen slots) by 8-lane groups;
+  // consecutive groups of 8 vertices go to different waves
   const int grp = lane >> 3, sub = lane & 7;
   const uint64_t nv = (uint64_t)G.n_owned;
   for (uint64_t base = (uint64_t)gwave * 8; base < nv; base += 8ull * nwaves) {
     const uint64_t vi = base + grp;
-    const bool valid = vi < nv;
     const int32_t y = (int32_t)vi;
-    const uint2 rw = valid ? G.row[y] : make_uint2(0u, 0u);
-    const bool small = valid && rw.y < kSmallLen;
-    const bool mid = valid && rw.y >= kSmallLen && rw.y < kMidLen;
+    const bool small = vi < nv && G.hidx[y] < 0;
+    const uint2 rw = small ? G.row[y] : make_uint2(0u, 0u);
     uint32_t cnt = small ? pull_slots<8>(S, y, S.col0[y], rw.x, sub, rw.y, kGroup, cb, xb, slots) : 0u;
     cnt += __shfl_xor(cnt, 1);
     cnt += __shfl_xor(cnt, 2);
@@ -641,24 +676,16 @@ __device__ __forceinline__ void pull_all(const LoopState& S, const uint32_t* cb,
       set_count(S, y, cnt, cb, rn);
       ++rows;
     }
-    const unsigned long long mm = __ballot(mid && sub == 0);
-    if (mid && sub == 0) lst[lanes_below(mm)] = y;
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    const uint32_t nm = (uint32_t)__popcll(mm);
-    for (uint32_t q = 0; q < nm; ++q) {
-      const int32_t x = lst[q];
-      const uint2 r2 = G.row[x];
-      uint32_t c2 = pull_slots<8>(S, x, S.col0[x], r2.x, lane, r2.y, 64, cb, xb, slots);
-      for (int off = 32; off > 0; off >>= 1) c2 += __shfl_xor(c2, off);
-      if (lane == 0) {
-        set_count(S, x, c2, cb, rn);
-        ++rows;
-      }
-    }
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    __builtin_amdgcn_wave_barrier();
   }
+  // rows with a histogram and fewer than kMidLen slots, one per wave
+  each_mid_row(S, lds, [](int32_t) { return true; }, [&](int32_t x, uint2 rw) {
+    uint32_t c2 = pull_slots<8>(S, x, S.col0[x], rw.x, lane, rw.y, 64, cb, xb, slots);
+    for (int off = 32; off > 0; off >>= 1) c2 += __shfl_xor(c2, off);
+    if (lane == 0) {
+      set_count(S, x, c2, cb, rn);
+      ++rows;
+    }
+  });
   // hub rows: one workgroup per hub (the chunk table's first entries)
   for (uint32_t i = blockIdx.x; i < S.n_hch; i += gridDim.x) {
     const int2 e = S.hch[i];
@@ -728,23 +755,24 @@ __device__ __forceinline__ void round_c(const LoopState& S, int r, LoopLds& lds)
   int32_t* list = lds.lists[threadIdx.x >> 6];
   // phase A's totals (every workgroup reads the same slots: uniform choices)
   if (threadIdx.x < 64) {
-    unsigned long long v[4] = {0ull, 0ull, 0ull, 0ull};
+    unsigned long long v[5] = {0ull, 0ull, 0ull, 0ull, 0ull};
     if (lane < kAcc) {
-      const unsigned long long* a = S.acc + ((size_t)(r & 1) * kAcc + lane) * 4;
+      const unsigned long long* a = S.acc + ((size_t)(r & 1) * kAcc + lane) * kAccW;
 #pragma unroll
-      for (int k = 0; k < 4; ++k) v[k] = a[k];
+      for (int k = 0; k < 5; ++k) v[k] = a[k];
     }
 #pragma unroll
-    for (int k = 0; k < 4; ++k)
+    for (int k = 0; k < 5; ++k)
       for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_xor(v[k], off);
     if (lane == 0)
-      for (int k = 0; k < 4; ++k) lds.tot[k] = v[k];
+      for (int k = 0; k < 5; ++k) lds.tot[k] = v[k];
   }
   __syncthreads();
   const unsigned long long nmem = lds.tot[0];
   const bool nonempty = nmem != 0ull;
   const bool heavy = lds.tot[2] > S.heavy_slots;
   const bool dhubs = lds.tot[3] != 0ull;
+  const bool dmids = lds.tot[4] != 0ull;
   if (blockIdx.x == 0) {
     // round r's counts into the trace; round r+1's sums start at zero
     if (threadIdx.x == 0) {
@@ -754,7 +782,8 @@ __device__ __forceinline__ void round_c(const LoopState& S, int r, LoopLds& lds)
       S.ctl->last_nc = (int32_t)nmem;
       if (!nonempty) S.ctl->done_round = r;
     }
-    if (threadIdx.x < kAcc * 4) S.acc[(size_t)((r + 1) & 1) * kAcc * 4 + threadIdx.x] = 0ull;
+    for (int i = threadIdx.x; i < kAcc * kAccW; i += kBlock)
+      S.acc[(size_t)((r + 1) & 1) * kAcc * kAccW + i] = 0ull;
   }
   // L_r, D_{r-1} and R_{r-1} are no longer read
   clear_words(S.B[(r + 1) & 1], S.words);
@@ -770,20 +799,19 @@ __device__ __forceinline__ void round_c(const LoopState& S, int r, LoopLds& lds)
     pull_all(S, cb, xb, rn, lds, pslots, rows);
     propagate_changed(S, db, lds);
   } else {
-  // short rows of D_r by 8-lane groups, mid rows by the wave (hubs go by the
-  // chunk table below); a row whose owner changed colour then updates its
-  // neighbours' caches
+  // rows of D_r without a histogram (fewer than kSmallLen slots) by 8-lane
+  // groups; the longer ones go by the histogram-row table and the hub chunk
+  // table below.  A row whose owner changed colour then updates its
+  // neighbours' caches.
   walk_bits(S.words, list, [&](uint32_t i) { return db[i]; }, [&](int32_t* lst, uint32_t total) {
     const int grp = lane >> 3, sub = lane & 7;
-    uint32_t nw = 0;
     for (uint32_t base = 0; base < total; base += 8) {
       const uint32_t i = base + grp;
       const bool valid = i < total;
       const int32_t x = valid ? lst[i] : 0;
-      const uint2 rw = valid ? G.row[x] : make_uint2(0u, 0u);
-      const bool small = valid && rw.y < kSmallLen;
-      const bool mid = valid && rw.y >= kSmallLen && rw.y < kMidLen;
+      const bool small = valid && G.hidx[x] < 0;
       if (small) {
+        const uint2 rw = G.row[x];
         const DState d = load_d(S, cb, x);
         uint32_t cnt = count_slots<4>(S, d, rw.x, sub, rw.y, kGroup, cb, db, xb, rn, slots);
         cnt += __shfl_xor(cnt, 1);
@@ -795,17 +823,10 @@ __device__ __forceinline__ void round_c(const LoopState& S, int r, LoopLds& lds)
         }
         if (d.chg) propagate_slots(G, d.sn, d.so, rw.x, sub, rw.y, kGroup);
       }
-      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      const unsigned long long mm = __ballot(mid && sub == 0);
-      if (mid && sub == 0) lst[nw + lanes_below(mm)] = x;
-      nw += (uint32_t)__popcll(mm);
-      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-      __builtin_amdgcn_wave_barrier();
     }
-    for (uint32_t q = 0; q < nw; ++q) {
-      const int32_t x = lst[q];
-      const uint2 rw = G.row[x];
+  });
+  if (dmids) {
+    each_mid_row(S, lds, [&](int32_t v) { return has_bit(db, v); }, [&](int32_t x, uint2 rw) {
       const DState d = load_d(S, cb, x);
       uint32_t cnt = count_slots<8>(S, d, rw.x, lane, rw.y, 64, cb, db, xb, rn, slots);
       for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off);
@@ -814,8 +835,8 @@ __device__ __forceinline__ void round_c(const LoopState& S, int r, LoopLds& lds)
         ++rows;
       }
       if (d.chg) propagate_slots(G, d.sn, d.so, rw.x, lane, rw.y, 64);
-    }
-  });
+    });
+  }
   // hub rows of D_r, one wave per kChunk-slot chunk (K[x] was zeroed in A).
   // A hub's chunks are adjacent in the table; lane l of wave w looks at entry
   // l * nwaves + w, so they go to different waves.
@@ -903,7 +924,7 @@ static int inc_alloc(Graph* g, IncState* s) {
   S.Rb[1] = s->bits + 3 * (size_t)S.words;
   S.Xb[0] = s->bits + 4 * (size_t)S.words;
   S.Xb[1] = s->bits + 5 * (size_t)S.words;
-  DC_HIP(hipMalloc(&S.acc, sizeof(unsigned long long) * 2 * kAcc * 4));
+  DC_HIP(hipMalloc(&S.acc, sizeof(unsigned long long) * 2 * kAcc * kAccW));
   DC_HIP(hipMalloc(&S.bstat, sizeof(unsigned long long) * 4 * kMaxBlocks));
   DC_HIP(hipMalloc(&S.ctl, sizeof(LoopCtl)));
   DC_HIP(hipHostMalloc(&s->hctl, sizeof(LoopCtl), hipHostMallocDefault));
@@ -1005,16 +1026,18 @@ int loop_hi(Graph* g, Frontier L0, int32_t nl0, bool persistent, LoopResult* res
   S.L0 = L0;
   S.hch = g->hch;
   S.n_hch = g->n_hch;
-  // rounds whose D rows hold more than half of all slots recount by pull
-  // (measured on R-MAT 22: the delta pass is cheaper below that)
-  S.heavy_slots = std::max<unsigned long long>(g->rows_end / 2, 1ull << 16);
+  S.hrowv = g->hrowv;
+  S.n_hrows = g->n_hrows;
+  // rounds whose D rows hold more than 1/8 of all slots recount by pull
+  // (measured on R-MAT 22 against 1/2 and 1/32)
+  S.heavy_slots = std::max<unsigned long long>(g->rows_end / 8, 1ull << 16);
 
   *s->hctl = LoopCtl{};
   s->hctl->done_round = INT_MAX;
   s->hctl->nl0 = nl0;
   DC_HIP(hipMemcpyAsync(S.ctl, s->hctl, sizeof(LoopCtl), hipMemcpyHostToDevice, g->stream));
   DC_HIP(hipMemsetAsync(S.bstat, 0, sizeof(unsigned long long) * 4 * kMaxBlocks, g->stream));
-  DC_HIP(hipMemsetAsync(S.acc, 0, sizeof(unsigned long long) * 2 * kAcc * 4, g->stream));
+  DC_HIP(hipMemsetAsync(S.acc, 0, sizeof(unsigned long long) * 2 * kAcc * kAccW, g->stream));
   const dim3 blk(kBlock);
   hipLaunchKernelGGL(k_loop_init, dim3(kMaxBlocks), blk, 0, g->stream, S);
   DC_HIP(hipGetLastError());
