@@ -64,7 +64,7 @@ constexpr int kAcc = 32;           // phase A sums: workgroup b adds into slot b
 constexpr int kAccW = 8;           // words per slot (five used)
 constexpr int kBitmaps = 6;        // Db, Rb, Xb (two each) besides the C/L pair
 constexpr int32_t kLightC = 4096;  // rounds after one with fewer members use the light grid
-constexpr int kWaveList = 64 * 32; // set bits of 64 bitmap words
+constexpr int kWaveList = 64 * 16; // set bits of 32 bitmap words (16 per lane)
 
 struct LoopCtl {
   int32_t done_round;   // the round whose C was empty (INT_MAX while running)
@@ -104,15 +104,27 @@ struct LoopState {
 };
 
 // LDS of a loop workgroup, shared by both phases
+// (20 KiB, so that eight workgroups fit in a CU's LDS.)  The mex windows of
+// phase A share their space with the workgroup sums, which are written only
+// after a barrier that follows every window use; the table-slice list of
+// the count pass shares the first wave list, which it writes only after a
+// barrier that follows every list use.
 struct LoopLds {
-  int32_t lists[kBlock / 64][kWaveList];
-  int2 chunks[kBlock];            // active hub chunks of one table slice
-  uint32_t nchunks[kBlock / 64];
-  unsigned long long wins[kBlock / 64][kWinWords];
-  uint32_t red[kBlock / 64];
-  unsigned long long red64[kBlock / 64];
-  unsigned long long tot[5];
+  union {
+    int32_t lists[kBlock / 64][kWaveList];
+    int2 chunks[kBlock];            // active table entries of one slice
+  };
+  union {
+    unsigned long long wins[kBlock / 64][kWinWords];
+    struct {
+      uint32_t red[kBlock / 64];
+      unsigned long long red64[kBlock / 64];
+      unsigned long long tot[5];
+      uint32_t nchunks[kBlock / 64];
+    };
+  };
 };
+static_assert(sizeof(LoopLds) <= 20480, "loop LDS exceeds 1/8 of a CU");
 
 struct IncState {
   LoopState S{};
@@ -136,16 +148,18 @@ __device__ __forceinline__ bool marks(int32_t v, int32_t cv, int32_t pv, int32_t
 }
 
 // Walk the set bits of a bitmap given word by word (word(i) -> bits).  Each
-// wave takes 64 words strided over the bitmap, compacts their set bits into
-// its LDS list and calls fn(list, count) with the whole wave.
+// wave takes 32 words per step, strided over the bitmap (lanes 2q and 2q+1
+// take the low and high halves of word q), compacts their set bits into its
+// LDS list and calls fn(list, count) with the whole wave.
 template <class WordFn, class Fn>
 __device__ __forceinline__ void walk_bits(uint32_t words, int32_t* list, WordFn word, Fn fn) {
   const int lane = lane_id();
   const uint32_t gwave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
-  for (uint64_t base = 0; base < words; base += 64ull * nwaves) {
-    const uint64_t wi = base + (uint64_t)lane * nwaves + gwave;
-    uint32_t bits = wi < words ? word((uint32_t)wi) : 0u;
+  const uint32_t half = (uint32_t)(lane & 1) * 16u;
+  for (uint64_t base = 0; base < words; base += 32ull * nwaves) {
+    const uint64_t wi = base + (uint64_t)(lane >> 1) * nwaves + gwave;
+    uint32_t bits = wi < words ? (word((uint32_t)wi) >> half) & 0xffffu : 0u;
     const uint32_t cnt = (uint32_t)__popc(bits);
     uint32_t x = cnt;
     for (int off = 1; off < 64; off <<= 1) {
@@ -158,7 +172,7 @@ __device__ __forceinline__ void walk_bits(uint32_t words, int32_t* list, WordFn 
     while (bits) {
       const int b = __ffs(bits) - 1;
       bits &= bits - 1u;
-      list[pos++] = (int32_t)(wi * 32 + b);
+      list[pos++] = (int32_t)(wi * 32 + half + b);
     }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
