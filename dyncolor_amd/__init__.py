@@ -43,7 +43,7 @@ EXPORTS = [
 class Options(C.Structure):
     _fields_ = [("slack_frac", C.c_float), ("slack_min", C.c_int32), ("max_rounds", C.c_int32),
                 ("driver", C.c_int32), ("device", C.c_int32), ("spare_slots", C.c_int32),
-                ("reserved", C.c_int32 * 2)]
+                ("exact_rounds", C.c_int32), ("reserved", C.c_int32 * 1)]
 
 
 class Stats(C.Structure):
@@ -55,7 +55,7 @@ class Stats(C.Structure):
                 ("ms_recolor", C.c_double), ("ms_recolor_kernel", C.c_double),
                 ("recolor_launches", C.c_int32), ("regrows", C.c_int32),
                 ("ms_count_pass", C.c_double), ("count_pass_bytes", C.c_int64),
-                ("count_pass_launches", C.c_int32), ("reserved", C.c_int32)]
+                ("count_pass_launches", C.c_int32), ("loop_settled", C.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}

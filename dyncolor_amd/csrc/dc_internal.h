@@ -218,6 +218,7 @@ struct LoopResult {
   double count_ms = 0.0;        // summed count-pass launch durations (profiling on)
   int64_t count_launches = 0;
   uint64_t count_bytes = 0;     // algorithmic bytes of those launches
+  bool settled = false;         // stopped at settled colours (see loop_hi)
 };
 int loop_hi(Graph* g, Frontier L0, int32_t nl0, bool persistent, LoopResult* res);
 void loop_free(Graph* g);

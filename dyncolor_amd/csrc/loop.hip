@@ -70,7 +70,7 @@ struct LoopCtl {
   int32_t done_round;   // the round whose C was empty (INT_MAX while running)
   int32_t nl0;          // |L_0|
   int32_t last_nc;      // |C| of the latest round
-  int32_t pad;
+  int32_t last_changed; // colour changes of the latest round
 };
 
 struct LoopState {
@@ -794,6 +794,7 @@ __device__ __forceinline__ void round_c(const LoopState& S, int r, LoopLds& lds)
       S.trace[r].y = (int32_t)nmem;
       S.trace[r].z = (int32_t)lds.tot[1];
       S.ctl->last_nc = (int32_t)nmem;
+      S.ctl->last_changed = (int32_t)lds.tot[1];
       if (!nonempty) S.ctl->done_round = r;
     }
     for (int i = threadIdx.x; i < kAcc * kAccW; i += kBlock)
@@ -907,6 +908,55 @@ __global__ __launch_bounds__(kBlock) void k_loop_persistent(LoopState S, int r0,
     grid.sync();
     if (S.ctl->done_round != INT_MAX) break;
   }
+}
+
+// Settle check, between rounds: does every owned vertex hold the mex of its
+// neighbours' colours?  If so, and the round just run changed no colour,
+// every later round of the reference recolours each member to the colour it
+// has (its mex over an unchanged snapshot), so no colour changes again and
+// the loop's result is the current colouring.  A cached mex is taken as it
+// is; a stale one is recomputed (histogram or short-row scan, by 8-lane
+// groups) and cached.  A mex past the histogram counts as a mismatch: the
+// check only errs towards "not settled".  *unsettled is set, never cleared.
+__global__ __launch_bounds__(kBlock) void k_settle_check(LoopState S, int32_t* unsettled) {
+  __shared__ int32_t stale_list[kBlock / 64][64];
+  const GraphView& G = S.G;
+  const int lane = lane_id();
+  const int grp = lane >> 3, sub = lane & 7;
+  int32_t* lst = stale_list[threadIdx.x >> 6];
+  const int64_t gwave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  bool bad = false;
+  for (int64_t base = gwave * 64; base < G.n_owned; base += nwaves * 64) {
+    const int64_t v = base + lane;
+    const bool valid = v < G.n_owned;
+    const int32_t m = valid ? G.mexc[v] : 0;
+    if (valid && m >= 0 && m != S.col0[v]) bad = true;
+    const bool stale = valid && m < 0;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    const unsigned long long sm = __ballot(stale);
+    if (stale) lst[lanes_below(sm)] = (int32_t)v;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    const uint32_t ns = (uint32_t)__popcll(sm);
+    for (uint32_t b8 = 0; b8 < ns; b8 += 8) {
+      const uint32_t q = b8 + grp;
+      const bool ok = q < ns;
+      const int32_t w = ok ? lst[q] : 0;
+      int32_t mw = -1;
+      if (ok) {
+        const int32_t h = G.hidx[w];
+        mw = h >= 0 ? group_hist_mex(G.hist + (uint64_t)h * kHistW, sub)
+                    : group_small_mex(G, G.row[w], sub);
+      }
+      if (ok && sub == 0) {
+        if (mw < 0 || mw != S.col0[w]) bad = true;
+        if (mw >= 0) G.mexc[w] = mw;
+      }
+    }
+  }
+  if (__ballot(bad) != 0ull && lane == 0) *unsettled = 1;
 }
 
 // K and the loop's bitmaps are zero between batches.
@@ -1072,7 +1122,12 @@ int loop_hi(Graph* g, Frontier L0, int32_t nl0, bool persistent, LoopResult* res
   double kms = 0.0, cms = 0.0;
   int64_t claunches = 0;
   hipEvent_t e0 = g->ev[2], e1 = g->ev[3];
-  int32_t rounds = -1, r = 0, block = persistent ? 4096 : 8;
+  // settle checks: at most two per batch, at rounds r with r + 4 a power of
+  // two (block ends of both drivers, so both stop at the same round), after
+  // a round without a change
+  int settle_left = g->opt.exact_rounds ? 0 : 2;
+  bool settled = false;
+  int32_t rounds = -1, r = 0, block = 4;
   while (r < max_rounds && rounds < 0) {
     const int32_t nb = std::min(block, max_rounds - r);
     if ((rc = inc_trace(g, s, r + nb))) return rc;
@@ -1118,7 +1173,25 @@ int loop_hi(Graph* g, Frontier L0, int32_t nl0, bool persistent, LoopResult* res
       }
     }
     r += nb;
-    if (done != INT_MAX) rounds = done + 1;
+    if (done != INT_MAX) {
+      rounds = done + 1;
+    } else if (settle_left > 0 && ((r + 4) & (r + 3)) == 0 && s->hctl->last_changed == 0) {
+      // rounds 0..r-1 ran in full; the last one changed nothing
+      --settle_left;
+      int32_t* flag = reinterpret_cast<int32_t*>(g->aux);
+      DC_HIP(hipMemsetAsync(flag, 0, sizeof(int32_t), g->stream));
+      hipLaunchKernelGGL(k_settle_check, dim3(kMaxBlocks), blk, 0, g->stream, S, flag);
+      DC_HIP(hipGetLastError());
+      ++launches;
+      int32_t h = 1;
+      DC_HIP(hipMemcpyAsync(&h, flag, sizeof(h), hipMemcpyDeviceToHost, g->stream));
+      DC_HIP(hipStreamSynchronize(g->stream));
+      if (h == 0) {
+        settled = true;
+        rounds = r;
+        if ((rc = loop_reset(g, s))) return rc;
+      }
+    }
     round_grid = s->hctl->last_nc < kLightC ? light_grid : grid;
     block = std::min(block * 2, persistent ? 4096 : 32);
   }
@@ -1135,6 +1208,7 @@ int loop_hi(Graph* g, Frontier L0, int32_t nl0, bool persistent, LoopResult* res
                         hipMemcpyDeviceToHost, g->stream));
   DC_HIP(hipStreamSynchronize(g->stream));
   res->rounds = rounds;
+  res->settled = settled;
   for (int32_t i = 0; i < rounds; ++i) {
     g->trace.insert(g->trace.end(),
                     {(int64_t)10, (int64_t)tr[i].x, (int64_t)tr[i].y, (int64_t)tr[i].z});
@@ -1142,6 +1216,7 @@ int loop_hi(Graph* g, Frontier L0, int32_t nl0, bool persistent, LoopResult* res
     res->evaluated += (uint64_t)tr[i].y;
     res->changed += (uint64_t)tr[i].z;
   }
+  if (settled) g->trace.insert(g->trace.end(), {(int64_t)11, (int64_t)rounds, (int64_t)0, (int64_t)0});
   uint64_t pull = 0, cnt = 0, rowv = 0;
   for (int b = 0; b < kMaxBlocks; ++b) {
     res->rslots += s->hstat[4 * b + 0];

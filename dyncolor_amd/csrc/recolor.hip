@@ -414,6 +414,7 @@ int recolor_plain(Graph* g, int rule, dc_stats* st, bool persistent_loop) {
       st->recolor_bytes = (int64_t)recolor_bytes;
       st->phases = (int32_t)(g->trace.size() / 4);
       st->loop_rounds = lr.rounds;
+      st->loop_settled = lr.settled ? 1 : 0;
       st->evaluated = (int64_t)evaluated;
       st->changed = (int64_t)changed_total;
       st->frontier_sum = (int64_t)frontier_sum;

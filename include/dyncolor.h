@@ -58,7 +58,13 @@ typedef struct {
   int32_t device;      /* HIP device ordinal (default 0)                              */
   int32_t spare_slots; /* spare tail for rows that outgrow their capacity; 0 = half
                           the row slots, at least 2^20 (default 0)                    */
-  int32_t reserved[2];
+  int32_t exact_rounds; /* 1: the independent-rule loop runs every round the reference
+                          runs; 0 (default): it stops once the colours are settled
+                          (a round without a change after which every owned vertex
+                          holds the mex of its neighbours' colours: the reference's
+                          remaining rounds change no colour).  Colours are the same
+                          either way; only the trace and round counts differ.         */
+  int32_t reserved[1];
 } dc_options;
 
 /* Per-batch counters.  Byte counts follow the algorithmic model: each visited
@@ -84,7 +90,7 @@ typedef struct {
                                count-pass launches (profiling on, plain driver) */
   int64_t count_pass_bytes; /* algorithmic bytes of those launches            */
   int32_t count_pass_launches;
-  int32_t reserved;
+  int32_t loop_settled;     /* 1 if the loop stopped at settled colours       */
 } dc_stats;
 
 /* ---- graph lifetime ------------------------------------------------------ */
@@ -162,7 +168,9 @@ int dc_validate(dc_graph* g, int64_t* monochromatic, int32_t* max_color);
 /* Phase trace of the last dc_recolor: 4 int64 per phase (kind, a, b, c).
  * kinds: 1 deletion round (targets, changed), 2 insertion round (targets,
  * changed), 3 LDF change-edge round (endpoints, changed), 4 cross-deletion
- * round, 5 cross-insertion round, 10 loop round (|L|, |C|, changed). */
+ * round, 5 cross-insertion round, 10 loop round (|L|, |C|, changed), 11 the
+ * loop stopped at settled colours (rounds run, 0, 0): the reference runs
+ * more rounds, none of which changes a color. */
 int dc_trace_get(dc_graph* g, int64_t* trace, int32_t cap, int32_t* n_phases);
 
 /* Live neighbours of v (multi-edges repeated), for tests and tools. */

@@ -9,6 +9,7 @@ import pytest
 
 import brute
 import dyncolor_amd as d
+from traces import same_trace
 
 pytestmark = pytest.mark.gpu
 
@@ -57,7 +58,7 @@ def run_both(oracle_mod, n, u, v, batches, rule, colors=None, opts=None, n_owned
         if not (got == og.color).all():
             bad = np.nonzero(got != og.color)[0]
             raise AssertionError(f"batch {bi}: {bad.size} vertices differ, first {bad[:10]}")
-        assert tr.tolist() == ph.tolist(), f"batch {bi}: trace differs"
+        assert same_trace(tr, ph), f"batch {bi}: trace differs"
         # slot colour cache and reverse-slot links stay exact
         assert g.check_consistency() == 0, f"batch {bi}: stale slot colour / twin"
         stats.append(st)
@@ -75,7 +76,7 @@ def test_sample_graph(sample, rule, key):
     g.process_batch(ch[:, 0], ch[:, 1], ch[:, 2].astype(np.uint8), rule)
     exp = sample["expected"][key]
     assert g.colors().tolist() == exp["colors"]
-    assert g.trace().tolist() == exp["trace"]
+    assert same_trace(g.trace(), exp["trace"])
     assert g.validate() == (0, max(exp["colors"]))
     if "prop" in exp:
         assert g.prop().tolist() == exp["prop"]
@@ -241,7 +242,7 @@ def test_compaction_keeps_graph(oracle_mod):
         og.apply_batch(bu, bv, bi)
         ph = og.recolor(0)
         assert (g.colors() == og.color).all()
-        assert g.trace().tolist() == ph.tolist()
+        assert same_trace(g.trace(), ph)
 
 
 def test_determinism_repeat(oracle_mod):
@@ -279,3 +280,19 @@ def test_error_paths_on_device():
     g.static_color(0)
     g.process_batch([0], [1], [1], 0)
     assert g.validate()[0] == 0
+
+
+def test_settle_and_exact_rounds(oracle_mod):
+    # exact_rounds=1: every round the reference runs, trace equal to the
+    # oracle's; default: the loop stops at settled colours in some batches,
+    # with the same colours (run_both checks both, and that every skipped
+    # oracle round changed nothing)
+    say("settle vs exact rounds")
+    u, v = d.gen_rmat(15, 16, seed=3)
+    n = 1 << 15
+    batches = d.gen_batches(n, u, v, 4, 6000, 0.5, 21)
+    _, _, st_exact = run_both(oracle_mod, n, u, v, batches, 0, opts=d.options(exact_rounds=1))
+    assert all(s.loop_settled == 0 for s in st_exact)
+    _, _, st = run_both(oracle_mod, n, u, v, batches, 0)
+    assert any(s.loop_settled for s in st)
+    assert sum(s.loop_rounds for s in st) < sum(s.loop_rounds for s in st_exact)

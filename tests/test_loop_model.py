@@ -124,3 +124,154 @@ def test_incremental_loop_equals_direct(pull_frac):
         got = incremental(n, n_owned, adj, col, prev, L0, pull_frac)
         assert got[2] == want[2], trial
         assert got[0] == want[0] and got[1] == want[1], trial
+
+
+# ---------------------------------------------------------------- settle --
+# The library stops the loop once the colours are settled: a round changed no
+# colour and every owned vertex holds the mex of its neighbours' colours.
+# Then each later round recolours every member to the colour it has (its mex
+# over an unchanged snapshot), so the result is the current colouring.  The
+# check below runs whole batches with the reference's rules (deletion round,
+# insertion round on edges equal-coloured at batch start, loop;
+# HI/Color_main.cu:146-202) on random multigraphs with frozen ghosts, and
+# asserts that no round after a settled one changes a colour, and that the
+# rule fires on a fair share of batches.
+
+def _settle_batches(rng, n, m, nb, bsz, n_owned):
+    edges = []
+    for _ in range(m):
+        a, b = rng.randrange(n), rng.randrange(n)
+        if a != b and not (a >= n_owned and b >= n_owned):
+            edges.append((a, b))
+
+    def adjacency():
+        adj = [[] for _ in range(n)]
+        for a, b in edges:
+            adj[a].append(b)
+            adj[b].append(a)
+        return adj
+
+    adj = adjacency()
+    c = [0] * n
+    for v in range(n):   # greedy in ID order; ghosts keep fixed colours
+        if v >= n_owned:
+            c[v] = rng.randrange(4)
+            continue
+        used = {c[w] for w in adj[v] if w < v or w >= n_owned}
+        k = 0
+        while k in used:
+            k += 1
+        c[v] = k
+    fired = violations = 0
+    for _ in range(nb):
+        dels, ins = [], []
+        for _ in range(bsz):
+            if rng.random() < 0.5 and edges:
+                dels.append(edges[rng.randrange(len(edges))])
+            else:
+                a, b = rng.randrange(n), rng.randrange(n)
+                if a != b and not (a >= n_owned and b >= n_owned):
+                    ins.append((a, b))
+        eq0 = [c[a] == c[b] for a, b in ins]
+        gone = {(min(a, b), max(a, b)) for a, b in dels}
+        edges = [e for e in edges if (min(e), max(e)) not in gone] + ins
+        adj = adjacency()
+        p = [-1] * n
+        marked, change = [], False
+        targets = []
+        for a, b in dels:
+            t = a if c[a] > c[b] else b
+            if t < n_owned and t not in targets:
+                targets.append(t)
+        new = [mex(c, adj[t]) for t in targets]
+        for t, m_ in zip(targets, new):
+            if m_ != c[t]:
+                p[t], c[t] = c[t], m_
+                marked.append(t)
+                change = True
+        targets = []
+        for (a, b), e0 in zip(ins, eq0):
+            if not e0 or c[a] != c[b]:
+                continue
+            ka = len({c[w] for w in adj[a]}) if a < n_owned else 1 << 30
+            kb = len({c[w] for w in adj[b]}) if b < n_owned else 1 << 30
+            t = a if ka < kb else b
+            if t < n_owned and t not in targets:
+                targets.append(t)
+        new = [mex(c, adj[t]) for t in targets]
+        for t, m_ in zip(targets, new):
+            p[t], c[t] = c[t], m_
+            marked.append(t)
+        change = change or bool(targets)
+        if not change:
+            continue
+        L, rounds = sorted(set(marked)), []
+        while True:
+            C = sorted({w for v in L for w in adj[v]
+                        if w < n_owned and marks(v, c[v], p[v], w, c[w])})
+            new = {w: mex(c, adj[w]) for w in C}
+            changed = sum(new[w] != c[w] for w in C)
+            for w in C:
+                p[w], c[w] = c[w], new[w]
+            settled = changed == 0 and all(mex(c, adj[v]) == c[v] for v in range(n_owned))
+            rounds.append((changed, settled))
+            if not C:
+                break
+            L = C
+        for r, (_, settled) in enumerate(rounds):
+            if settled:
+                fired += 1
+                violations += sum(ch for ch, _ in rounds[r + 1:])
+                break
+    return fired, violations
+
+
+def test_settle_rule():
+    rng = random.Random(5)
+    fired = 0
+    for _ in range(12):
+        for n, m, nb, bsz, own in [(60, 300, 5, 30, 60), (200, 1500, 4, 80, 200),
+                                   (120, 900, 4, 60, 90)]:
+            f, viol = _settle_batches(rng, n, m, nb, bsz, own)
+            assert viol == 0
+            fired += f
+    assert fired >= 20
+
+
+def test_settle_rule_arbitrary_states():
+    # loops started from arbitrary colourings (unstable vertices anywhere):
+    # the rule must look at every owned vertex, not only at the members
+    rng = random.Random(7)
+    fired = 0
+    for trial in range(600):
+        n = rng.randint(4, 30)
+        n_owned = n if rng.random() < 0.7 else rng.randint(2, n)
+        adj = [[] for _ in range(n)]
+        pr = rng.choice([0.15, 0.3, 0.5])
+        for a in range(n):
+            for b in range(a + 1, n):
+                if (a < n_owned or b < n_owned) and rng.random() < pr:
+                    adj[a].append(b)
+                    adj[b].append(a)
+        c = [rng.randint(0, 6) for _ in range(n)]
+        p = [rng.choice([-1] + list(range(7))) for _ in range(n)]
+        L = sorted(v for v in range(n_owned) if rng.random() < 0.2)
+        rounds = []
+        while True:
+            C = sorted({w for v in L for w in adj[v]
+                        if w < n_owned and marks(v, c[v], p[v], w, c[w])})
+            new = {w: mex(c, adj[w]) for w in C}
+            changed = sum(new[w] != c[w] for w in C)
+            for w in C:
+                p[w], c[w] = c[w], new[w]
+            settled = changed == 0 and all(mex(c, adj[v]) == c[v] for v in range(n_owned))
+            rounds.append((changed, settled))
+            if not C or len(rounds) > 200:
+                break
+            L = C
+        for r, (_, settled) in enumerate(rounds):
+            if settled:
+                fired += 1
+                assert sum(ch for ch, _ in rounds[r + 1:]) == 0, trial
+                break
+    assert fired >= 50
