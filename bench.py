@@ -115,7 +115,7 @@ def run_stream(args, d, rule, n, u, v, workload):
 
     lat = []
     tot = {"bytes": 0, "kms": 0.0, "launches": 0, "rounds": 0, "changed": 0, "evaluated": 0,
-           "slots": 0, "regrows": 0}
+           "slots": 0, "regrows": 0, "settled": 0}
     t0 = time.perf_counter()
     for i in range(args.warmup, args.warmup + args.steps):
         tb = time.perf_counter()
@@ -130,6 +130,7 @@ def run_stream(args, d, rule, n, u, v, workload):
         tot["evaluated"] += st.evaluated
         tot["slots"] += st.slots_scanned
         tot["regrows"] += st.regrows
+        tot["settled"] += st.loop_settled
     g.synchronize()
     elapsed = time.perf_counter() - t0
     # untimed: the same stream's next batches with a hipEvent pair around every
@@ -168,7 +169,8 @@ def run_stream(args, d, rule, n, u, v, workload):
     out["roofline"]["traffic"] = traffic_for(out["roofline"]["kernel"])
     out["work"] = {"loop_rounds": tot["rounds"], "evaluated": tot["evaluated"],
                    "changed": tot["changed"], "row_slots_scanned": tot["slots"],
-                   "batches_with_row_growth": tot["regrows"]}
+                   "batches_with_row_growth": tot["regrows"],
+                   "batches_settled": tot["settled"]}
     out["cpu_baseline"] = None
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(n, u, v, batches[0], rule, colors_after_first)

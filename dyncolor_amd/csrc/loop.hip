@@ -1068,6 +1068,25 @@ static int persistent_grid(Graph* g, int* grid) {
   return DC_OK;
 }
 
+// Host blocks of rounds (the host reads the control word after each): rounds
+// 0-2, then one at a time up to 8 (a batch's changes usually end by round 4,
+// and the settle check wants the first round without a change), 8-11, then
+// blocks ending at 2^k - 4 (28, 60, 124, ...), the plain driver's at most 32
+// rounds long.
+static int32_t block_end(int32_t r, bool persistent) {
+  if (r < 3) return 3;
+  if (r < 8) return r + 1;
+  if (r < 12) return 12;
+  int32_t e = 28;
+  while (e <= r) e = 2 * e + 4;
+  return persistent ? e : std::min(e, r + 32);
+}
+
+// rounds after which a settle check may run (block ends of both drivers)
+static bool settle_point(int32_t r) {
+  return (r >= 3 && r <= 8) || r == 12 || (r >= 28 && ((r + 4) & (r + 3)) == 0);
+}
+
 int loop_hi(Graph* g, Frontier L0, int32_t nl0, bool persistent, LoopResult* res) {
   IncState* s = inc_state(g);
   if (!s) {
@@ -1122,14 +1141,14 @@ int loop_hi(Graph* g, Frontier L0, int32_t nl0, bool persistent, LoopResult* res
   double kms = 0.0, cms = 0.0;
   int64_t claunches = 0;
   hipEvent_t e0 = g->ev[2], e1 = g->ev[3];
-  // settle checks: at most two per batch, at rounds r with r + 4 a power of
-  // two (block ends of both drivers, so both stop at the same round), after
-  // a round without a change
-  int settle_left = g->opt.exact_rounds ? 0 : 2;
+  // settle checks: at most three per batch, after a round without a change,
+  // at the check rounds of settle_point (block ends of both drivers, so both
+  // stop at the same round)
+  int settle_left = g->opt.exact_rounds ? 0 : 3;
   bool settled = false;
-  int32_t rounds = -1, r = 0, block = 4;
+  int32_t rounds = -1, r = 0;
   while (r < max_rounds && rounds < 0) {
-    const int32_t nb = std::min(block, max_rounds - r);
+    const int32_t nb = std::min(block_end(r, persistent), max_rounds) - r;
     if ((rc = inc_trace(g, s, r + nb))) return rc;
     if (g->profiling) {
       while (s->cev.size() < 2 * (size_t)nb) {
@@ -1175,7 +1194,7 @@ int loop_hi(Graph* g, Frontier L0, int32_t nl0, bool persistent, LoopResult* res
     r += nb;
     if (done != INT_MAX) {
       rounds = done + 1;
-    } else if (settle_left > 0 && ((r + 4) & (r + 3)) == 0 && s->hctl->last_changed == 0) {
+    } else if (settle_left > 0 && settle_point(r) && s->hctl->last_changed == 0) {
       // rounds 0..r-1 ran in full; the last one changed nothing
       --settle_left;
       int32_t* flag = reinterpret_cast<int32_t*>(g->aux);
@@ -1193,7 +1212,6 @@ int loop_hi(Graph* g, Frontier L0, int32_t nl0, bool persistent, LoopResult* res
       }
     }
     round_grid = s->hctl->last_nc < kLightC ? light_grid : grid;
-    block = std::min(block * 2, persistent ? 4096 : 32);
   }
   if (rounds < 0) {
     int rc2 = loop_reset(g, s);
