@@ -23,15 +23,17 @@
 // histogram, or a scan of a short row.  Colours, traces and round counts are
 // those of the full re-derivation.
 //
-// Sets are bitmaps (one bit per vertex, small enough to stay in L2): C_r in
-// B[r & 1] (so L_r = C_{r-1} is the other one), D_r in Db[r & 1], R_r in
-// Rb[r & 1].  A phase finds its vertices by scanning bitmap words; each wave
-// compacts the set bits of its 64 words (spread over the whole bitmap, so the
-// dense low-ID region of a power-law graph lands on many waves) into a list
-// in LDS.  There are no global list counters: on this GPU a few thousand
-// atomics on one address serialise into tens of microseconds, so every update
-// here is a bit set, a count change at a scattered vertex, or a
-// per-workgroup partial sum.
+// Sets are state arrays, one nibble per vertex (4 MB for both parities at
+// 4.2 M vertices): round r's C_r, D_r, R_r and X_r (members whose colour
+// changed) are flags in St[r & 1], so L_r = C_{r-1} and the previous round's
+// D and R are in the other array, and the count pass learns a neighbour's
+// C, D, R and X membership from one gather.  A phase finds its vertices by
+// scanning state words; each wave compacts the vertices of its 128 words
+// (spread over the whole array, so the dense low-ID region of a power-law
+// graph lands on many waves) into a list in LDS.  There are no global list
+// counters: on this GPU a few thousand atomics on one address serialise into
+// tens of microseconds, so every update here is a flag set, a count change
+// at a scattered vertex, or a per-workgroup partial sum.
 //
 // One round = two phases:
 //   A(r): candidates = L_r | D_{r-1} | R_{r-1}.  K > 0 makes a member of C_r,
@@ -40,14 +42,15 @@
 //         members of L_r with K == 0 leave and join D_r.
 //   C(r): rows of D_r: count changes, recounts, and for vertices whose colour
 //         changed the slot cache / mex cache / histograms of their neighbours.
-//         Hub rows go by the static hub chunk table.  The previous round's
-//         bitmaps are cleared.
+//         Rows of 64-511 slots go by the static histogram-row table, hub rows
+//         by the static hub chunk table.  The previous round's state array
+//         is cleared.
 // The plain driver launches one kernel per phase (the host enqueues rounds in
 // blocks and reads one control word per block; rounds past the last one
 // return at once); the persistent driver runs the rounds in one cooperative
 // launch with grid barriers between phases.
 //
-// Outside the loop K is zero everywhere and so are the bitmaps.
+// Outside the loop K is zero everywhere and so are the state arrays.
 #include <hip/hip_runtime.h>
 #include <hip/hip_cooperative_groups.h>
 
@@ -62,9 +65,8 @@ namespace dc {
 constexpr int kMaxBlocks = 2048;   // grid of the loop kernels (per-block partials)
 constexpr int kAcc = 32;           // phase A sums: workgroup b adds into slot b % kAcc
 constexpr int kAccW = 8;           // words per slot (five used)
-constexpr int kBitmaps = 6;        // Db, Rb, Xb (two each) besides the C/L pair
 constexpr int32_t kLightC = 4096;  // rounds after one with fewer members use the light grid
-constexpr int kWaveList = 64 * 16; // set bits of 32 bitmap words (16 per lane)
+constexpr int kWaveList = 64 * 16; // vertices of 128 state words (16 per lane)
 
 struct LoopCtl {
   int32_t done_round;   // the round whose C was empty (INT_MAX while running)
@@ -80,11 +82,10 @@ struct LoopState {
   int32_t* prev;
   int32_t* K;
   int2* dold;          // D members: {colour, (prev + 1) | in_L << 31} before the round
-  uint32_t* B[2];      // C_r members in B[r & 1]
-  uint32_t* Db[2];     // D_r in Db[r & 1]
-  uint32_t* Rb[2];     // R_r in Rb[r & 1]
-  uint32_t* Xb[2];     // members of C_r whose colour changed in round r, in Xb[r & 1]
-  uint32_t words;      // words per bitmap
+  // set membership of round r in St[r & 1], one nibble per vertex (8 per
+  // word): kFC C_r, kFD D_r, kFR R_r, kFX members of C_r whose colour changed
+  uint32_t* St[2];
+  uint32_t swords;     // words per state array
   Frontier L0;         // the marked set
   const int2* hch;     // hub chunk table {vertex, chunk}
   uint32_t n_hch;
@@ -129,17 +130,24 @@ static_assert(sizeof(LoopLds) <= 20480, "loop LDS exceeds 1/8 of a CU");
 struct IncState {
   LoopState S{};
   std::vector<hipEvent_t> cev;   // profiling: start/stop around each count-pass launch
-  uint32_t* bits = nullptr;   // Db[0..1], Rb[0..1], Xb[0..1]
+  uint32_t* bits = nullptr;   // St[0], St[1]
   int32_t trace_cap = 0;
   LoopCtl* hctl = nullptr;
   unsigned long long* hstat = nullptr;
 };
 
-__device__ __forceinline__ bool has_bit(const uint32_t* b, int32_t v) {
-  return (b[v >> 5] >> (v & 31)) & 1u;
+// Membership flags of one vertex in a state array (one gather tells a
+// neighbour's C, D, R and X membership at once).
+constexpr uint32_t kFC = 1u, kFD = 2u, kFR = 4u, kFX = 8u;
+
+__device__ __forceinline__ uint32_t st_flags(uint32_t word, int32_t v) {
+  return (word >> ((v & 7) << 2)) & 15u;
 }
-__device__ __forceinline__ void set_bit(uint32_t* b, int32_t v) {
-  atomicOr(&b[v >> 5], 1u << (v & 31));
+__device__ __forceinline__ bool st_has(const uint32_t* st, int32_t v, uint32_t f) {
+  return (st[v >> 3] >> ((v & 7) << 2)) & f;
+}
+__device__ __forceinline__ void st_set(uint32_t* st, int32_t v, uint32_t f) {
+  atomicOr(&st[v >> 3], f << ((v & 7) << 2));
 }
 
 // The marking predicate of findEligibleNeighbors for a slot (v -> w).
@@ -147,20 +155,24 @@ __device__ __forceinline__ bool marks(int32_t v, int32_t cv, int32_t pv, int32_t
   return (cv == cw && v < w) || pv < cw;
 }
 
-// Walk the set bits of a bitmap given word by word (word(i) -> bits).  Each
-// wave takes 32 words per step, strided over the bitmap (lanes 2q and 2q+1
-// take the low and high halves of word q), compacts their set bits into its
-// LDS list and calls fn(list, count) with the whole wave.
-template <class WordFn, class Fn>
-__device__ __forceinline__ void walk_bits(uint32_t words, int32_t* list, WordFn word, Fn fn) {
+// Walk the vertices of a state array with any of the flags f set.  Each
+// wave takes 128 words (1024 vertices) per step, two per lane strided over
+// the array, compacts the vertices into its LDS list and calls fn(list,
+// count) with the whole wave.
+template <class Fn>
+__device__ __forceinline__ void walk_states(const uint32_t* st, uint32_t swords, uint32_t f,
+                                            int32_t* list, Fn fn) {
   const int lane = lane_id();
   const uint32_t gwave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
-  const uint32_t half = (uint32_t)(lane & 1) * 16u;
-  for (uint64_t base = 0; base < words; base += 32ull * nwaves) {
-    const uint64_t wi = base + (uint64_t)(lane >> 1) * nwaves + gwave;
-    uint32_t bits = wi < words ? (word((uint32_t)wi) >> half) & 0xffffu : 0u;
-    const uint32_t cnt = (uint32_t)__popc(bits);
+  const uint32_t fm = f * 0x11111111u;
+  auto nibbles = [](uint32_t x) { return (x | (x >> 1) | (x >> 2) | (x >> 3)) & 0x11111111u; };
+  for (uint64_t base = 0; base < swords; base += 128ull * nwaves) {
+    const uint64_t w0 = base + (uint64_t)lane * nwaves + gwave;
+    const uint64_t w1 = w0 + 64ull * nwaves;
+    uint32_t b0 = w0 < swords ? nibbles(st[w0] & fm) : 0u;
+    uint32_t b1 = w1 < swords ? nibbles(st[w1] & fm) : 0u;
+    const uint32_t cnt = (uint32_t)(__popc(b0) + __popc(b1));
     uint32_t x = cnt;
     for (int off = 1; off < 64; off <<= 1) {
       const uint32_t y = __shfl_up(x, off);
@@ -169,10 +181,15 @@ __device__ __forceinline__ void walk_bits(uint32_t words, int32_t* list, WordFn 
     const uint32_t total = __shfl(x, 63);
     if (total == 0) continue;
     uint32_t pos = x - cnt;
-    while (bits) {
-      const int b = __ffs(bits) - 1;
-      bits &= bits - 1u;
-      list[pos++] = (int32_t)(wi * 32 + half + b);
+    while (b0) {
+      const int b = __ffs(b0) - 1;
+      b0 &= b0 - 1u;
+      list[pos++] = (int32_t)(w0 * 8 + (b >> 2));
+    }
+    while (b1) {
+      const int b = __ffs(b1) - 1;
+      b1 &= b1 - 1u;
+      list[pos++] = (int32_t)(w1 * 8 + (b >> 2));
     }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -294,7 +311,7 @@ __global__ __launch_bounds__(kBlock) void k_loop_init(LoopState S) {
   const int64_t gwave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
   const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
   auto row = [&](int32_t v, uint2 r, uint32_t lo, uint32_t hi, uint32_t stride, bool head) {
-    if (head) set_bit(S.B[1], v);
+    if (head) st_set(S.St[1], v, kFC);
     const int32_t cv = S.col0[v];
     const int32_t pv = S.prev[v];
     for (uint32_t j = lo; j < hi; j += stride) {
@@ -302,7 +319,7 @@ __global__ __launch_bounds__(kBlock) void k_loop_init(LoopState S) {
       const int32_t cw = G.ccol[(uint64_t)r.x + j];
       if (w >= 0 && w < G.n_owned && marks(v, cv, pv, w, cw)) {
         atomicAdd(&S.K[w], 1);
-        set_bit(S.Rb[1], w);
+        st_set(S.St[1], w, kFR);
       }
     }
   };
@@ -329,24 +346,22 @@ __global__ __launch_bounds__(kBlock) void k_loop_init(LoopState S) {
 
 // Commit w's new colour m (one lane); w joins D when its (in_L, colour,
 // prev) state changed.
-__device__ __forceinline__ bool commit_member(const LoopState& S, uint32_t* cb, uint32_t* dn,
-                                              uint32_t* xb, int32_t w, int32_t cm, int32_t so,
-                                              int32_t po, bool inl, int32_t m, uint32_t& changed) {
-  set_bit(cb, w);
+__device__ __forceinline__ bool commit_member(const LoopState& S, uint32_t* sn, int32_t w,
+                                              int32_t cm, int32_t so, int32_t po, bool inl,
+                                              int32_t m, uint32_t& changed) {
   if (cm < 0) S.G.mexc[w] = m;
   S.col0[w] = m;
   S.col1[w] = m;
   S.prev[w] = so;
   const bool chv = m != so;
   changed += chv ? 1u : 0u;
-  if (chv) set_bit(xb, w);
-  if (!inl || po != so || chv) {
+  const bool dv = !inl || po != so || chv;
+  if (dv) {
     S.dold[w] = make_int2(so, (po + 1) | (inl ? (int32_t)0x80000000 : 0));
     S.K[w] = 0;
-    set_bit(dn, w);
-    return true;
   }
-  return false;
+  st_set(sn, w, kFC | (chv ? kFX : 0u) | (dv ? kFD : 0u));
+  return dv;
 }
 
 // mex of a row with fewer than 64 slots by an 8-lane group (lane `sub` reads
@@ -396,12 +411,8 @@ __device__ __forceinline__ int32_t group_hist_mex(const uint32_t* hrow, int sub)
 __device__ __forceinline__ void round_a(const LoopState& S, int r, LoopLds& lds) {
   uint32_t* red = lds.red;
   const GraphView& G = S.G;
-  const uint32_t* Lb = S.B[(r + 1) & 1];
-  uint32_t* Cb = S.B[r & 1];
-  const uint32_t* Dp = S.Db[(r + 1) & 1];
-  uint32_t* Dn = S.Db[r & 1];
-  const uint32_t* Rp = S.Rb[(r + 1) & 1];
-  uint32_t* Xn = S.Xb[r & 1];
+  const uint32_t* sp = S.St[(r + 1) & 1];   // L_r = C_{r-1}, D_{r-1}, R_{r-1}
+  uint32_t* sn = S.St[r & 1];               // C_r, D_r, X_r
   const int lane = lane_id();
   const int wib = threadIdx.x >> 6;
   int32_t* list = lds.lists[wib];
@@ -419,8 +430,7 @@ __device__ __forceinline__ void round_a(const LoopState& S, int r, LoopLds& lds)
     dmids += len < kMidLen && h >= 0 ? 1u : 0u;
   };
 
-  walk_bits(S.words, list, [&](uint32_t i) { return Lb[i] | Dp[i] | Rp[i]; },
-            [&](int32_t* lst, uint32_t total) {
+  walk_states(sp, S.swords, kFC | kFD | kFR, list, [&](int32_t* lst, uint32_t total) {
     // members with a cached mex commit at once; the others (stale mex) are
     // compacted to the front of the list and evaluated by 8-lane groups
     uint32_t nd = 0;
@@ -435,16 +445,16 @@ __device__ __forceinline__ void round_a(const LoopState& S, int r, LoopLds& lds)
         cm = G.mexc[w];
         so = S.col0[w];
         po = S.prev[w];
-        inl = has_bit(Lb, w);
+        inl = st_has(sp, w, kFC);
       }
       const bool member = valid && k > 0;
       const bool leaving = valid && !member && inl;
       bool dv = leaving;
       if (leaving) {
         S.dold[w] = make_int2(so, (po + 1) | (int32_t)0x80000000);
-        set_bit(Dn, w);
+        st_set(sn, w, kFD);
       }
-      if (member && cm >= 0) dv = commit_member(S, Cb, Dn, Xn, w, cm, so, po, inl, cm, changed);
+      if (member && cm >= 0) dv = commit_member(S, sn, w, cm, so, po, inl, cm, changed);
       if (dv) d_row(w);
       members += member ? 1u : 0u;
       const bool defer = member && cm < 0;
@@ -477,7 +487,7 @@ __device__ __forceinline__ void round_a(const LoopState& S, int r, LoopLds& lds)
       }
       const bool wst = valid && m < 0;
       if (valid && !wst && sub == 0 &&
-          commit_member(S, Cb, Dn, Xn, w, -1, S.col0[w], S.prev[w], has_bit(Lb, w), m, changed))
+          commit_member(S, sn, w, -1, S.col0[w], S.prev[w], st_has(sp, w, kFC), m, changed))
         d_row(w);
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
       __builtin_amdgcn_wave_barrier();
@@ -492,7 +502,7 @@ __device__ __forceinline__ void round_a(const LoopState& S, int r, LoopLds& lds)
       const int32_t w = lst[q];
       const int32_t m = wave_row_mex<M_LOOP_HI>(RA, w, win);
       if (lane == 0 &&
-          commit_member(S, Cb, Dn, Xn, w, -1, S.col0[w], S.prev[w], has_bit(Lb, w), m, changed))
+          commit_member(S, sn, w, -1, S.col0[w], S.prev[w], st_has(sp, w, kFC), m, changed))
         d_row(w);
     }
   });
@@ -527,7 +537,7 @@ struct DState {
   bool inl, inc, chg;
 };
 
-__device__ __forceinline__ DState load_d(const LoopState& S, const uint32_t* cb, int32_t x) {
+__device__ __forceinline__ DState load_d(const LoopState& S, const uint32_t* st, int32_t x) {
   DState d;
   d.x = x;
   const int2 o = S.dold[x];
@@ -536,23 +546,21 @@ __device__ __forceinline__ DState load_d(const LoopState& S, const uint32_t* cb,
   d.inl = o.y < 0;
   d.sn = S.col0[x];
   d.pn = S.prev[x];
-  d.inc = has_bit(cb, x);
+  d.inc = st_has(st, x, kFC);
   d.chg = d.sn != d.so;
   return d;
 }
 
 // One lane's share of a count-pass visit of x's row: slots lo, lo+stride, ...
 // Returns this lane's part of x's recount.  U slots at a time in four
-// stages (slots; their neighbours' bitmap words; count changes and the state
+// stages (slots; their neighbours' state words; count changes and the state
 // of neighbours that changed colour; the recount), each stage's gathers all
 // in flight before the next stage reads them.  x's own colour change goes
 // to the neighbours separately (propagate_slots).
 template <int U>
 __device__ __forceinline__ uint32_t count_slots(const LoopState& S, const DState& d, uint64_t rbase,
                                                 uint32_t lo, uint32_t hi, uint32_t stride,
-                                                const uint32_t* cb, const uint32_t* db,
-                                                const uint32_t* xb, uint32_t* rn,
-                                                unsigned long long& slots) {
+                                                uint32_t* st, unsigned long long& slots) {
   const GraphView& G = S.G;
   uint32_t cnt = 0;
   for (uint32_t j = lo; j < hi; j += U * stride) {
@@ -565,37 +573,33 @@ __device__ __forceinline__ uint32_t count_slots(const LoopState& S, const DState
       c[k] = in ? G.ccol[rbase + jj] : -1;
       slots += in ? 1u : 0u;
     }
-    uint32_t wd[U], wc[U], wx[U], wr[U];
+    uint32_t fl[U];
 #pragma unroll
     for (int k = 0; k < U; ++k) {
       // tombstones and frozen ghosts read word 0 and are masked out below
       const bool own = y[k] >= 0 && y[k] < G.n_owned;
-      const uint32_t wi = own ? ((uint32_t)y[k] >> 5) : 0u;
-      wd[k] = db[wi];
-      wc[k] = cb[wi];
-      wx[k] = xb[wi];
-      wr[k] = rn[wi];   // R_r only gains bits in this phase: a set bit stays set
+      fl[k] = own ? st_flags(st[(uint32_t)y[k] >> 3], y[k]) : 0u;
     }
     int32_t gc[U], gp[U];
     bool mem[U];
 #pragma unroll
     for (int k = 0; k < U; ++k) {
       const bool own = y[k] >= 0 && y[k] < G.n_owned;
-      const uint32_t bit = 1u << (y[k] & 31);
-      if (own && !(wd[k] & bit)) {
+      if (own && !(fl[k] & kFD)) {
         const bool to = d.inl && marks(d.x, d.so, d.po, y[k], c[k]);
         const bool tn = d.inc && marks(d.x, d.sn, d.pn, y[k], c[k]);
         if (to != tn) {
           atomicAdd(&S.K[y[k]], tn ? 1 : -1);
-          if (tn && !(wr[k] & bit)) set_bit(rn, y[k]);
+          // R_r only gains members in this phase: a flag seen set stays set
+          if (tn && !(fl[k] & kFR)) st_set(st, y[k], kFR);
         }
       }
-      mem[k] = own && (wc[k] & bit);
+      mem[k] = own && (fl[k] & kFC);
       // a member that kept its colour has prev == colour == the slot value;
       // one that changed is gathered (its slots may not be updated yet)
       gc[k] = c[k];
       gp[k] = c[k];
-      if (mem[k] && (wx[k] & bit)) {
+      if (mem[k] && (fl[k] & kFX)) {
         gc[k] = S.col0[y[k]];
         gp[k] = S.prev[y[k]];
       }
@@ -618,8 +622,8 @@ __device__ __forceinline__ void clear_words(uint32_t* b, uint32_t words) {
 template <int U>
 __device__ __forceinline__ uint32_t pull_slots(const LoopState& S, int32_t y, int32_t cy,
                                                uint64_t rbase, uint32_t lo, uint32_t hi,
-                                               uint32_t stride, const uint32_t* cb,
-                                               const uint32_t* xb, unsigned long long& slots) {
+                                               uint32_t stride, const uint32_t* st,
+                                               unsigned long long& slots) {
   const GraphView& G = S.G;
   uint32_t cnt = 0;
   for (uint32_t j = lo; j < hi; j += U * stride) {
@@ -632,23 +636,20 @@ __device__ __forceinline__ uint32_t pull_slots(const LoopState& S, int32_t y, in
       c[k] = in ? G.ccol[rbase + jj] : -1;
       slots += in ? 1u : 0u;
     }
-    uint32_t wc[U], wx[U];
+    uint32_t fl[U];
 #pragma unroll
     for (int k = 0; k < U; ++k) {
       const bool own = x[k] >= 0 && x[k] < G.n_owned;
-      const uint32_t wi = own ? ((uint32_t)x[k] >> 5) : 0u;
-      wc[k] = cb[wi];
-      wx[k] = xb[wi];
+      fl[k] = own ? st_flags(st[(uint32_t)x[k] >> 3], x[k]) : 0u;
     }
     int32_t gc[U], gp[U];
     bool mem[U];
 #pragma unroll
     for (int k = 0; k < U; ++k) {
-      const uint32_t bit = 1u << (x[k] & 31);
-      mem[k] = x[k] >= 0 && x[k] < G.n_owned && (wc[k] & bit);
+      mem[k] = (fl[k] & kFC) != 0u;
       gc[k] = c[k];
       gp[k] = c[k];
-      if (mem[k] && (wx[k] & bit)) {
+      if (mem[k] && (fl[k] & kFX)) {
         gc[k] = S.col0[x[k]];
         gp[k] = S.prev[x[k]];
       }
@@ -661,13 +662,12 @@ __device__ __forceinline__ uint32_t pull_slots(const LoopState& S, int32_t y, in
 }
 
 __device__ __forceinline__ void set_count(const LoopState& S, int32_t y, uint32_t cnt,
-                                          const uint32_t* cb, uint32_t* rn) {
+                                          uint32_t* st) {
   S.K[y] = (int32_t)cnt;
-  if (cnt && !has_bit(cb, y)) set_bit(rn, y);
+  if (cnt && !st_has(st, y, kFC)) st_set(st, y, kFR);
 }
 
-__device__ __forceinline__ void pull_all(const LoopState& S, const uint32_t* cb,
-                                         const uint32_t* xb, uint32_t* rn, LoopLds& lds,
+__device__ __forceinline__ void pull_all(const LoopState& S, uint32_t* st, LoopLds& lds,
                                          unsigned long long& slots, unsigned long long& rows) {
   const GraphView& G = S.G;
   const int lane = lane_id();
@@ -682,21 +682,21 @@ __device__ __forceinline__ void pull_all(const LoopState& S, const uint32_t* cb,
     const int32_t y = (int32_t)vi;
     const bool small = vi < nv && G.hidx[y] < 0;
     const uint2 rw = small ? G.row[y] : make_uint2(0u, 0u);
-    uint32_t cnt = small ? pull_slots<8>(S, y, S.col0[y], rw.x, sub, rw.y, kGroup, cb, xb, slots) : 0u;
+    uint32_t cnt = small ? pull_slots<8>(S, y, S.col0[y], rw.x, sub, rw.y, kGroup, st, slots) : 0u;
     cnt += __shfl_xor(cnt, 1);
     cnt += __shfl_xor(cnt, 2);
     cnt += __shfl_xor(cnt, 4);
     if (small && sub == 0) {
-      set_count(S, y, cnt, cb, rn);
+      set_count(S, y, cnt, st);
       ++rows;
     }
   }
   // rows with a histogram and fewer than kMidLen slots, one per wave
   each_mid_row(S, lds, [](int32_t) { return true; }, [&](int32_t x, uint2 rw) {
-    uint32_t c2 = pull_slots<8>(S, x, S.col0[x], rw.x, lane, rw.y, 64, cb, xb, slots);
+    uint32_t c2 = pull_slots<8>(S, x, S.col0[x], rw.x, lane, rw.y, 64, st, slots);
     for (int off = 32; off > 0; off >>= 1) c2 += __shfl_xor(c2, off);
     if (lane == 0) {
-      set_count(S, x, c2, cb, rn);
+      set_count(S, x, c2, st);
       ++rows;
     }
   });
@@ -707,10 +707,10 @@ __device__ __forceinline__ void pull_all(const LoopState& S, const uint32_t* cb,
     const int32_t y = e.x;
     const uint2 rw = G.row[y];
     if (rw.y < kMidLen) continue;  // went by the short/mid pass (uniform)
-    const uint32_t cnt = pull_slots<8>(S, y, S.col0[y], rw.x, threadIdx.x, rw.y, kBlock, cb, xb, slots);
+    const uint32_t cnt = pull_slots<8>(S, y, S.col0[y], rw.x, threadIdx.x, rw.y, kBlock, st, slots);
     const uint32_t tot = block_sum(cnt, lds.red);
     if (threadIdx.x == 0) {
-      set_count(S, y, tot, cb, rn);
+      set_count(S, y, tot, st);
       ++rows;
     }
   }
@@ -718,24 +718,24 @@ __device__ __forceinline__ void pull_all(const LoopState& S, const uint32_t* cb,
 
 // The new colours of D_r's members that changed into the slot cache, mex
 // cache and histograms of their neighbours.
-__device__ __forceinline__ void propagate_changed(const LoopState& S, uint32_t* db, LoopLds& lds) {
+__device__ __forceinline__ void propagate_changed(const LoopState& S, const uint32_t* st,
+                                                  LoopLds& lds) {
   const GraphView& G = S.G;
   const int lane = lane_id();
   int32_t* list = lds.lists[threadIdx.x >> 6];
-  walk_bits(S.words, list, [&](uint32_t i) { return db[i]; }, [&](int32_t* lst, uint32_t total) {
+  // the members that changed colour (X_r)
+  walk_states(st, S.swords, kFX, list, [&](int32_t* lst, uint32_t total) {
     uint32_t nw = 0;
     for (uint32_t base = 0; base < total; base += 64) {
       const uint32_t i = base + lane;
-      const bool valid = i < total;
-      const int32_t x = valid ? lst[i] : 0;
-      bool chg = false;
+      const bool chg = i < total;
+      const int32_t x = chg ? lst[i] : 0;
       int32_t so = 0, sn = 0;
       uint2 rw = make_uint2(0u, 0u);
-      if (valid) {
+      if (chg) {
         so = S.dold[x].x;
         sn = S.col0[x];
-        chg = so != sn;
-        if (chg) rw = G.row[x];
+        rw = G.row[x];
       }
       if (chg && rw.y < kSmallLen) propagate_slots(G, sn, so, rw.x, 0, rw.y, 1);
       const bool mid = chg && rw.y >= kSmallLen && rw.y < kMidLen;
@@ -753,7 +753,7 @@ __device__ __forceinline__ void propagate_changed(const LoopState& S, uint32_t* 
       propagate_slots(G, S.col0[x], S.dold[x].x, rw.x, lane, rw.y, 64);
     }
   });
-  each_hub_chunk(S, lds, [&](int2 e) { return has_bit(db, e.x) && S.col0[e.x] != S.dold[e.x].x; },
+  each_hub_chunk(S, lds, [&](int2 e) { return st_has(st, e.x, kFX); },
                  [&](int32_t x, uint2 rw, uint32_t lo, uint32_t hi) {
     propagate_slots(G, S.col0[x], S.dold[x].x, rw.x, lo + lane, hi, 64);
   });
@@ -761,10 +761,7 @@ __device__ __forceinline__ void propagate_changed(const LoopState& S, uint32_t* 
 
 __device__ __forceinline__ void round_c(const LoopState& S, int r, LoopLds& lds) {
   const GraphView& G = S.G;
-  const uint32_t* cb = S.B[r & 1];
-  uint32_t* db = S.Db[r & 1];
-  uint32_t* rn = S.Rb[r & 1];
-  const uint32_t* xb = S.Xb[r & 1];
+  uint32_t* st = S.St[r & 1];   // C_r, D_r, X_r; R_r is filled here
   const int lane = lane_id();
   int32_t* list = lds.lists[threadIdx.x >> 6];
   // phase A's totals (every workgroup reads the same slots: uniform choices)
@@ -800,25 +797,22 @@ __device__ __forceinline__ void round_c(const LoopState& S, int r, LoopLds& lds)
     for (int i = threadIdx.x; i < kAcc * kAccW; i += kBlock)
       S.acc[(size_t)((r + 1) & 1) * kAcc * kAccW + i] = 0ull;
   }
-  // L_r, D_{r-1} and R_{r-1} are no longer read
-  clear_words(S.B[(r + 1) & 1], S.words);
-  clear_words(S.Db[(r + 1) & 1], S.words);
-  clear_words(S.Rb[(r + 1) & 1], S.words);
-  clear_words(S.Xb[(r + 1) & 1], S.words);
+  // L_r, D_{r-1}, R_{r-1} and X_{r-1} are no longer read
+  clear_words(S.St[(r + 1) & 1], S.swords);
   if (!nonempty) {
-    clear_words(db, S.words);
+    clear_words(st, S.swords);   // D_r's leaving members
     return;
   }
   unsigned long long slots = 0, pslots = 0, rows = 0;
   if (heavy) {
-    pull_all(S, cb, xb, rn, lds, pslots, rows);
-    propagate_changed(S, db, lds);
+    pull_all(S, st, lds, pslots, rows);
+    propagate_changed(S, st, lds);
   } else {
   // rows of D_r without a histogram (fewer than kSmallLen slots) by 8-lane
   // groups; the longer ones go by the histogram-row table and the hub chunk
   // table below.  A row whose owner changed colour then updates its
   // neighbours' caches.
-  walk_bits(S.words, list, [&](uint32_t i) { return db[i]; }, [&](int32_t* lst, uint32_t total) {
+  walk_states(st, S.swords, kFD, list, [&](int32_t* lst, uint32_t total) {
     const int grp = lane >> 3, sub = lane & 7;
     for (uint32_t base = 0; base < total; base += 8) {
       const uint32_t i = base + grp;
@@ -827,8 +821,8 @@ __device__ __forceinline__ void round_c(const LoopState& S, int r, LoopLds& lds)
       const bool small = valid && G.hidx[x] < 0;
       if (small) {
         const uint2 rw = G.row[x];
-        const DState d = load_d(S, cb, x);
-        uint32_t cnt = count_slots<4>(S, d, rw.x, sub, rw.y, kGroup, cb, db, xb, rn, slots);
+        const DState d = load_d(S, st, x);
+        uint32_t cnt = count_slots<4>(S, d, rw.x, sub, rw.y, kGroup, st, slots);
         cnt += __shfl_xor(cnt, 1);
         cnt += __shfl_xor(cnt, 2);
         cnt += __shfl_xor(cnt, 4);
@@ -841,9 +835,9 @@ __device__ __forceinline__ void round_c(const LoopState& S, int r, LoopLds& lds)
     }
   });
   if (dmids) {
-    each_mid_row(S, lds, [&](int32_t v) { return has_bit(db, v); }, [&](int32_t x, uint2 rw) {
-      const DState d = load_d(S, cb, x);
-      uint32_t cnt = count_slots<8>(S, d, rw.x, lane, rw.y, 64, cb, db, xb, rn, slots);
+    each_mid_row(S, lds, [&](int32_t v) { return st_has(st, v, kFD); }, [&](int32_t x, uint2 rw) {
+      const DState d = load_d(S, st, x);
+      uint32_t cnt = count_slots<8>(S, d, rw.x, lane, rw.y, 64, st, slots);
       for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off);
       if (lane == 0) {
         S.K[x] = (int32_t)cnt;
@@ -852,14 +846,13 @@ __device__ __forceinline__ void round_c(const LoopState& S, int r, LoopLds& lds)
       if (d.chg) propagate_slots(G, d.sn, d.so, rw.x, lane, rw.y, 64);
     });
   }
-  // hub rows of D_r, one wave per kChunk-slot chunk (K[x] was zeroed in A).
-  // A hub's chunks are adjacent in the table; lane l of wave w looks at entry
-  // l * nwaves + w, so they go to different waves.
+  // hub rows of D_r by their kChunk-slot chunks, a quarter chunk per wave
+  // (K[x] was zeroed in A; the chunks add up)
   if (dhubs) {
-    each_hub_chunk(S, lds, [&](int2 e) { return has_bit(db, e.x); },
+    each_hub_chunk(S, lds, [&](int2 e) { return st_has(st, e.x, kFD); },
                    [&](int32_t x, uint2 rw, uint32_t lo, uint32_t hi) {
-      const DState d = load_d(S, cb, x);
-      uint32_t cnt = count_slots<4>(S, d, rw.x, lo + lane, hi, 64, cb, db, xb, rn, slots);
+      const DState d = load_d(S, st, x);
+      uint32_t cnt = count_slots<4>(S, d, rw.x, lo + lane, hi, 64, st, slots);
       for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off);
       if (lane == 0) {
         if (cnt) atomicAdd(&S.K[x], (int32_t)cnt);
@@ -959,7 +952,7 @@ __global__ __launch_bounds__(kBlock) void k_settle_check(LoopState S, int32_t* u
   if (__ballot(bad) != 0ull && lane == 0) *unsettled = 1;
 }
 
-// K and the loop's bitmaps are zero between batches.
+// K and the loop's state arrays are zero between batches.
 __global__ void k_zero_check(const int32_t* K, size_t nk, const uint32_t* bits, size_t words,
                              unsigned long long* bad) {
   unsigned long long cnt = 0;
@@ -978,23 +971,19 @@ static IncState* inc_state(Graph* g) { return static_cast<IncState*>(g->inc); }
 static int inc_alloc(Graph* g, IncState* s) {
   const size_t n = (size_t)g->n + 1;
   LoopState& S = s->S;
-  S.words = (uint32_t)g->cbits_words;
+  S.swords = (uint32_t)(((size_t)g->n + 8) / 8);
   DC_HIP(hipMalloc(&S.K, sizeof(int32_t) * n));
   DC_HIP(hipMalloc(&S.dold, sizeof(int2) * n));
-  DC_HIP(hipMalloc(&s->bits, sizeof(uint32_t) * kBitmaps * (size_t)S.words));
-  S.Db[0] = s->bits;
-  S.Db[1] = s->bits + S.words;
-  S.Rb[0] = s->bits + 2 * (size_t)S.words;
-  S.Rb[1] = s->bits + 3 * (size_t)S.words;
-  S.Xb[0] = s->bits + 4 * (size_t)S.words;
-  S.Xb[1] = s->bits + 5 * (size_t)S.words;
+  DC_HIP(hipMalloc(&s->bits, sizeof(uint32_t) * 2 * (size_t)S.swords));
+  S.St[0] = s->bits;
+  S.St[1] = s->bits + S.swords;
   DC_HIP(hipMalloc(&S.acc, sizeof(unsigned long long) * 2 * kAcc * kAccW));
   DC_HIP(hipMalloc(&S.bstat, sizeof(unsigned long long) * 4 * kMaxBlocks));
   DC_HIP(hipMalloc(&S.ctl, sizeof(LoopCtl)));
   DC_HIP(hipHostMalloc(&s->hctl, sizeof(LoopCtl), hipHostMallocDefault));
   s->hstat = new unsigned long long[4 * kMaxBlocks];
   DC_HIP(hipMemsetAsync(S.K, 0, sizeof(int32_t) * n, g->stream));
-  DC_HIP(hipMemsetAsync(s->bits, 0, sizeof(uint32_t) * kBitmaps * (size_t)S.words, g->stream));
+  DC_HIP(hipMemsetAsync(s->bits, 0, sizeof(uint32_t) * 2 * (size_t)S.swords, g->stream));
 
   return DC_OK;
 }
@@ -1039,7 +1028,7 @@ int loop_check(Graph* g, int64_t* bad) {
   IncState* s = inc_state(g);
   if (s)
     hipLaunchKernelGGL(k_zero_check, dim3(1024), dim3(kBlock), 0, g->stream, s->S.K,
-                       (size_t)g->n, s->bits, kBitmaps * (size_t)s->S.words, cnt);
+                       (size_t)g->n, s->bits, 2 * (size_t)s->S.swords, cnt);
   DC_HIP(hipGetLastError());
   unsigned long long h = 0;
   DC_HIP(hipMemcpyAsync(&h, cnt, sizeof(h), hipMemcpyDeviceToHost, g->stream));
@@ -1050,8 +1039,7 @@ int loop_check(Graph* g, int64_t* bad) {
 
 static int loop_reset(Graph* g, IncState* s) {
   DC_HIP(hipMemsetAsync(s->S.K, 0, sizeof(int32_t) * ((size_t)g->n + 1), g->stream));
-  DC_HIP(hipMemsetAsync(s->bits, 0, sizeof(uint32_t) * kBitmaps * (size_t)s->S.words, g->stream));
-  DC_HIP(hipMemsetAsync(g->cbits[0], 0, sizeof(uint32_t) * 2 * g->cbits_words, g->stream));
+  DC_HIP(hipMemsetAsync(s->bits, 0, sizeof(uint32_t) * 2 * (size_t)s->S.swords, g->stream));
   return DC_OK;
 }
 
@@ -1104,8 +1092,6 @@ int loop_hi(Graph* g, Frontier L0, int32_t nl0, bool persistent, LoopResult* res
   S.col0 = g->color[g->cur];
   S.col1 = g->color[g->cur ^ 1];
   S.prev = g->prev;
-  S.B[0] = g->cbits[0];
-  S.B[1] = g->cbits[1];
   S.L0 = L0;
   S.hch = g->hch;
   S.n_hch = g->n_hch;
@@ -1245,13 +1231,13 @@ int loop_hi(Graph* g, Frontier L0, int32_t nl0, bool persistent, LoopResult* res
   res->slots = pull + cnt;
   res->kernel_ms = kms;
   res->launches = launches;
-  // Algorithmic bytes of the count pass: per round the four bitmaps it clears
-  // or walks (4 B per word each); per count-pass slot col + ccol (8 B); per
-  // pulled slot col (4 B); per row or chunk visit its bounds and count or
-  // saved state (16 B).
+  // Algorithmic bytes of a round's phases over the state arrays: phase A
+  // walks one, the count pass clears one and walks one (4 B per word each);
+  // per count-pass slot col + ccol (8 B); per pulled slot col (4 B); per row
+  // or chunk visit its bounds and count or saved state (16 B).
   res->count_ms = cms;
   res->count_launches = claunches;
-  res->count_bytes = (uint64_t)rounds * 16ull * S.words + 8ull * cnt + 4ull * pull + 16ull * rowv;
+  res->count_bytes = (uint64_t)rounds * 12ull * S.swords + 8ull * cnt + 4ull * pull + 16ull * rowv;
   return DC_OK;
 }
 
