@@ -111,6 +111,7 @@ struct LoopState {
 // the count pass shares the first wave list, which it writes only after a
 // barrier that follows every list use.
 struct LoopLds {
+  int32_t pick[kBlock / 64][64];    // a wave's compacted entries of one batch of 64
   union {
     int32_t lists[kBlock / 64][kWaveList];
     int2 chunks[kBlock];            // active table entries of one slice
@@ -125,7 +126,7 @@ struct LoopLds {
     };
   };
 };
-static_assert(sizeof(LoopLds) <= 20480, "loop LDS exceeds 1/8 of a CU");
+static_assert(sizeof(LoopLds) <= 24576, "loop LDS exceeds 3/20 of a CU");
 
 struct IncState {
   LoopState S{};
@@ -673,23 +674,45 @@ __device__ __forceinline__ void pull_all(const LoopState& S, uint32_t* st, LoopL
   const int lane = lane_id();
   const uint32_t gwave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
-  // rows without a histogram (fewer than kSmallLen slots) by 8-lane groups;
-  // consecutive groups of 8 vertices go to different waves
+  // rows without a histogram (fewer than kSmallLen slots) by 8-lane groups:
+  // a wave loads 64 consecutive vertices' rows at once (lane l vertex l),
+  // compacts the non-empty short ones and takes them 8 at a time.  An empty
+  // row's count is 0 and stays 0.
   const int grp = lane >> 3, sub = lane & 7;
+  int32_t* pick = lds.pick[threadIdx.x >> 6];
   const uint64_t nv = (uint64_t)G.n_owned;
-  for (uint64_t base = (uint64_t)gwave * 8; base < nv; base += 8ull * nwaves) {
-    const uint64_t vi = base + grp;
-    const int32_t y = (int32_t)vi;
-    const bool small = vi < nv && G.hidx[y] < 0;
-    const uint2 rw = small ? G.row[y] : make_uint2(0u, 0u);
-    uint32_t cnt = small ? pull_slots<8>(S, y, S.col0[y], rw.x, sub, rw.y, kGroup, st, slots) : 0u;
-    cnt += __shfl_xor(cnt, 1);
-    cnt += __shfl_xor(cnt, 2);
-    cnt += __shfl_xor(cnt, 4);
-    if (small && sub == 0) {
-      set_count(S, y, cnt, st);
-      ++rows;
+  for (uint64_t base = (uint64_t)gwave * 64; base < nv; base += 64ull * nwaves) {
+    const uint64_t vi = base + lane;
+    const bool valid = vi < nv;
+    const int32_t y = valid ? (int32_t)vi : 0;
+    const int32_t h = valid ? G.hidx[y] : 0;
+    const uint2 rw = valid ? G.row[y] : make_uint2(0u, 0u);
+    const int32_t cy = S.col0[y];
+    const bool small = valid && h < 0 && rw.y > 0;
+    const unsigned long long sm = __ballot(small);
+    if (small) pick[lanes_below(sm)] = lane;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    const uint32_t ns = (uint32_t)__popcll(sm);
+    for (uint32_t t = 0; t < ns; t += 8) {
+      const bool mine = t + grp < ns;
+      const int src = mine ? pick[t + grp] : lane;
+      const int32_t yy = __shfl(y, src);
+      const int32_t cyy = __shfl(cy, src);
+      const uint32_t rx = (uint32_t)__shfl((int)rw.x, src);
+      const uint32_t ry_all = (uint32_t)__shfl((int)rw.y, src);   // every lane shuffles
+      const uint32_t ry = mine ? ry_all : 0u;
+      uint32_t cnt = pull_slots<8>(S, yy, cyy, rx, sub, ry, kGroup, st, slots);
+      cnt += __shfl_xor(cnt, 1);
+      cnt += __shfl_xor(cnt, 2);
+      cnt += __shfl_xor(cnt, 4);
+      if (mine && sub == 0) {
+        set_count(S, yy, cnt, st);
+        ++rows;
+      }
     }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
   }
   // rows with a histogram and fewer than kMidLen slots, one per wave
   each_mid_row(S, lds, [](int32_t) { return true; }, [&](int32_t x, uint2 rw) {
@@ -814,24 +837,52 @@ __device__ __forceinline__ void round_c(const LoopState& S, int r, LoopLds& lds)
   // neighbours' caches.
   walk_states(st, S.swords, kFD, list, [&](int32_t* lst, uint32_t total) {
     const int grp = lane >> 3, sub = lane & 7;
-    for (uint32_t base = 0; base < total; base += 8) {
-      const uint32_t i = base + grp;
+    int32_t* pick = lds.pick[threadIdx.x >> 6];
+    // 64 entries at a time: every lane loads its entry's row and state at
+    // once, the short rows are compacted, then 8-lane groups take them
+    for (uint32_t c0 = 0; c0 < total; c0 += 64) {
+      const uint32_t i = c0 + lane;
       const bool valid = i < total;
       const int32_t x = valid ? lst[i] : 0;
-      const bool small = valid && G.hidx[x] < 0;
-      if (small) {
-        const uint2 rw = G.row[x];
-        const DState d = load_d(S, st, x);
-        uint32_t cnt = count_slots<4>(S, d, rw.x, sub, rw.y, kGroup, st, slots);
+      const int32_t h = valid ? G.hidx[x] : 0;
+      const uint2 rw = valid ? G.row[x] : make_uint2(0u, 0u);
+      const DState d = load_d(S, st, x);
+      const bool small = valid && h < 0;
+      const unsigned long long sm = __ballot(small);
+      if (small) pick[lanes_below(sm)] = lane;
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      const uint32_t ns = (uint32_t)__popcll(sm);
+      for (uint32_t t = 0; t < ns; t += 8) {
+        const bool mine = t + grp < ns;
+        const int src = mine ? pick[t + grp] : lane;
+        DState e;
+        e.x = __shfl(d.x, src);
+        e.so = __shfl(d.so, src);
+        e.po = __shfl(d.po, src);
+        e.sn = __shfl(d.sn, src);
+        e.pn = __shfl(d.pn, src);
+        const int fl = __shfl((d.inl ? 1 : 0) | (d.inc ? 2 : 0) | (d.chg ? 4 : 0), src);
+        e.inl = fl & 1;
+        e.inc = fl & 2;
+        e.chg = fl & 4;
+        const uint32_t rx = (uint32_t)__shfl((int)rw.x, src);
+        // every lane takes part in the shuffle (a lane reading from a lane
+        // that sits out would get an undefined value)
+        const uint32_t ry_all = (uint32_t)__shfl((int)rw.y, src);
+        const uint32_t ry = mine ? ry_all : 0u;
+        uint32_t cnt = count_slots<4>(S, e, rx, sub, ry, kGroup, st, slots);
         cnt += __shfl_xor(cnt, 1);
         cnt += __shfl_xor(cnt, 2);
         cnt += __shfl_xor(cnt, 4);
-        if (sub == 0) {
-          S.K[x] = (int32_t)cnt;
+        if (mine && sub == 0) {
+          S.K[e.x] = (int32_t)cnt;
           ++rows;
         }
-        if (d.chg) propagate_slots(G, d.sn, d.so, rw.x, sub, rw.y, kGroup);
+        if (mine && e.chg) propagate_slots(G, e.sn, e.so, rx, sub, ry, kGroup);
       }
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
     }
   });
   if (dmids) {
