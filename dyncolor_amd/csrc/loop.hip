@@ -226,14 +226,31 @@ __device__ __forceinline__ unsigned long long block_sum64(unsigned long long v,
   return t;
 }
 
+// A per-lane value from lane src, word by word (a shuffle every lane takes
+// part in).
+template <class P>
+__device__ __forceinline__ P bcast(const P& p, int src) {
+  static_assert(sizeof(P) % 4 == 0, "shuffled in 4-byte words");
+  union W {
+    P p;
+    int a[sizeof(P) / 4];
+  } u, o;
+  u.p = p;
+#pragma unroll
+  for (int i = 0; i < (int)(sizeof(P) / 4); ++i) o.a[i] = __shfl(u.a[i], src);
+  return o.p;
+}
+
 // Hub chunks whose entry passes act(e): the workgroup takes the table entries
 // blockIdx, blockIdx + gridDim, ... (consecutive chunks of one hub go to
 // different workgroups), compacts the active ones in LDS, and runs each chunk
-// with all its waves: wave q of the workgroup calls fn(x, row, lo, hi) on
-// the q-th quarter of the chunk's slots.  Every thread must call this (it
-// synchronises the workgroup).
-template <class Act, class Fn>
-__device__ __forceinline__ void each_hub_chunk(const LoopState& S, LoopLds& lds, Act act, Fn fn) {
+// with all its waves: wave q of the workgroup calls fn(x, row, lo, hi, p) on
+// the q-th quarter of the chunk's slots, p = pre(x).  The rows and pre() of
+// up to 64 chunks are loaded at once, one per lane.  Every thread must call
+// this (it synchronises the workgroup).
+template <class Act, class Pre, class Fn>
+__device__ __forceinline__ void each_hub_chunk(const LoopState& S, LoopLds& lds, Act act, Pre pre,
+                                               Fn fn) {
   const GraphView& G = S.G;
   const int lane = lane_id();
   const int wib = threadIdx.x >> 6;
@@ -253,13 +270,21 @@ __device__ __forceinline__ void each_hub_chunk(const LoopState& S, LoopLds& lds,
     }
     if (a) lds.chunks[off + lanes_below(am)] = e;
     __syncthreads();
-    for (uint32_t q = 0; q < n; ++q) {
-      const int2 ce = lds.chunks[q];
-      const uint2 rw = G.row[ce.x];
-      const uint32_t lo = (uint32_t)ce.y * kChunk + (uint32_t)wib * kQuarter;
-      // rows now shorter than kMidLen went by the short/mid path
-      if (rw.y < kMidLen || lo >= rw.y) continue;
-      fn(ce.x, rw, lo, min(rw.y, lo + kQuarter));
+    for (uint32_t c0 = 0; c0 < n; c0 += 64) {
+      const uint32_t m = min(64u, n - c0);
+      const int2 ce = (uint32_t)lane < m ? lds.chunks[c0 + lane] : make_int2(0, 0);
+      const uint2 rw = (uint32_t)lane < m ? G.row[ce.x] : make_uint2(0u, 0u);
+      const auto p = pre(ce.x);
+      for (uint32_t q = 0; q < m; ++q) {
+        const int32_t x = __shfl(ce.x, (int)q);
+        const uint32_t c = (uint32_t)__shfl(ce.y, (int)q);
+        const uint2 rq = bcast(rw, (int)q);
+        const auto pq = bcast(p, (int)q);
+        const uint32_t lo = c * kChunk + (uint32_t)wib * kQuarter;
+        // rows now shorter than kMidLen went by the short/mid path
+        if (rq.y < kMidLen || lo >= rq.y) continue;
+        fn(x, rq, lo, min(rq.y, lo + kQuarter), pq);
+      }
     }
     __syncthreads();
   }
@@ -268,15 +293,18 @@ __device__ __forceinline__ void each_hub_chunk(const LoopState& S, LoopLds& lds,
 // Rows of the histogram-row table (every row of kSmallLen or more slots of
 // capacity) that pass act(v) and are shorter than kMidLen (longer ones go by
 // the hub chunk table): the workgroup takes entries blockIdx, blockIdx +
-// gridDim, ..., compacts the active ones in LDS and its waves take them in
-// turn, one row per wave: fn(x, row).  The table spreads these rows evenly
-// over the workgroups whatever their vertex IDs.  Every thread must call
-// this (it synchronises the workgroup).
-template <class Act, class Fn>
-__device__ __forceinline__ void each_mid_row(const LoopState& S, LoopLds& lds, Act act, Fn fn) {
+// gridDim, ..., compacts the active ones in LDS, and wave w takes entries w,
+// w + 4, ...: its lanes load those rows and pre(x) at once, then the wave
+// runs them in turn, fn(x, row, p).  The table spreads these rows evenly over
+// the workgroups whatever their vertex IDs.  Every thread must call this (it
+// synchronises the workgroup).
+template <class Act, class Pre, class Fn>
+__device__ __forceinline__ void each_mid_row(const LoopState& S, LoopLds& lds, Act act, Pre pre,
+                                             Fn fn) {
   const GraphView& G = S.G;
   const int lane = lane_id();
   const int wib = threadIdx.x >> 6;
+  constexpr uint32_t kWaves = kBlock / 64;
   for (uint64_t base = 0; base < S.n_hrows; base += (uint64_t)kBlock * gridDim.x) {
     const uint64_t i = base + (uint64_t)threadIdx.x * gridDim.x + blockIdx.x;
     const int32_t v = i < S.n_hrows ? S.hrowv[i] : -1;
@@ -292,11 +320,15 @@ __device__ __forceinline__ void each_mid_row(const LoopState& S, LoopLds& lds, A
     }
     if (a) lds.chunks[off + lanes_below(am)] = make_int2(v, 0);
     __syncthreads();
-    for (uint32_t q = (uint32_t)wib; q < n; q += kBlock / 64) {
-      const int32_t x = lds.chunks[q].x;
-      const uint2 rw = G.row[x];
-      if (rw.y >= kMidLen) continue;
-      fn(x, rw);
+    const uint32_t mine = n > (uint32_t)wib ? (n - (uint32_t)wib + kWaves - 1) / kWaves : 0u;
+    const bool has = (uint32_t)lane < mine;
+    const int32_t x = has ? lds.chunks[(uint32_t)wib + kWaves * (uint32_t)lane].x : 0;
+    const uint2 rw = has ? G.row[x] : make_uint2(0u, 0u);
+    const auto p = pre(x);
+    for (uint32_t q = 0; q < mine; ++q) {
+      const uint2 rq = bcast(rw, (int)q);
+      if (rq.y >= kMidLen) continue;
+      fn(__shfl(x, (int)q), rq, bcast(p, (int)q));
     }
     __syncthreads();
   }
@@ -535,7 +567,7 @@ __device__ __forceinline__ void round_a(const LoopState& S, int r, LoopLds& lds)
 // neighbours are in D_r.
 struct DState {
   int32_t x, so, po, sn, pn;
-  bool inl, inc, chg;
+  bool inl, inc, chg, pad;
 };
 
 __device__ __forceinline__ DState load_d(const LoopState& S, const uint32_t* st, int32_t x) {
@@ -549,6 +581,7 @@ __device__ __forceinline__ DState load_d(const LoopState& S, const uint32_t* st,
   d.pn = S.prev[x];
   d.inc = st_has(st, x, kFC);
   d.chg = d.sn != d.so;
+  d.pad = false;
   return d;
 }
 
@@ -715,8 +748,9 @@ __device__ __forceinline__ void pull_all(const LoopState& S, uint32_t* st, LoopL
     __builtin_amdgcn_wave_barrier();
   }
   // rows with a histogram and fewer than kMidLen slots, one per wave
-  each_mid_row(S, lds, [](int32_t) { return true; }, [&](int32_t x, uint2 rw) {
-    uint32_t c2 = pull_slots<8>(S, x, S.col0[x], rw.x, lane, rw.y, 64, st, slots);
+  each_mid_row(S, lds, [](int32_t) { return true; }, [&](int32_t x) { return S.col0[x]; },
+               [&](int32_t x, uint2 rw, int32_t cx) {
+    uint32_t c2 = pull_slots<8>(S, x, cx, rw.x, lane, rw.y, 64, st, slots);
     for (int off = 32; off > 0; off >>= 1) c2 += __shfl_xor(c2, off);
     if (lane == 0) {
       set_count(S, x, c2, st);
@@ -777,8 +811,9 @@ __device__ __forceinline__ void propagate_changed(const LoopState& S, const uint
     }
   });
   each_hub_chunk(S, lds, [&](int2 e) { return st_has(st, e.x, kFX); },
-                 [&](int32_t x, uint2 rw, uint32_t lo, uint32_t hi) {
-    propagate_slots(G, S.col0[x], S.dold[x].x, rw.x, lo + lane, hi, 64);
+                 [&](int32_t x) { return make_int2(S.col0[x], S.dold[x].x); },
+                 [&](int32_t x, uint2 rw, uint32_t lo, uint32_t hi, int2 cs) {
+    propagate_slots(G, cs.x, cs.y, rw.x, lo + lane, hi, 64);
   });
 }
 
@@ -856,16 +891,7 @@ __device__ __forceinline__ void round_c(const LoopState& S, int r, LoopLds& lds)
       for (uint32_t t = 0; t < ns; t += 8) {
         const bool mine = t + grp < ns;
         const int src = mine ? pick[t + grp] : lane;
-        DState e;
-        e.x = __shfl(d.x, src);
-        e.so = __shfl(d.so, src);
-        e.po = __shfl(d.po, src);
-        e.sn = __shfl(d.sn, src);
-        e.pn = __shfl(d.pn, src);
-        const int fl = __shfl((d.inl ? 1 : 0) | (d.inc ? 2 : 0) | (d.chg ? 4 : 0), src);
-        e.inl = fl & 1;
-        e.inc = fl & 2;
-        e.chg = fl & 4;
+        const DState e = bcast(d, src);
         const uint32_t rx = (uint32_t)__shfl((int)rw.x, src);
         // every lane takes part in the shuffle (a lane reading from a lane
         // that sits out would get an undefined value)
@@ -886,8 +912,9 @@ __device__ __forceinline__ void round_c(const LoopState& S, int r, LoopLds& lds)
     }
   });
   if (dmids) {
-    each_mid_row(S, lds, [&](int32_t v) { return st_has(st, v, kFD); }, [&](int32_t x, uint2 rw) {
-      const DState d = load_d(S, st, x);
+    each_mid_row(S, lds, [&](int32_t v) { return st_has(st, v, kFD); },
+                 [&](int32_t x) { return load_d(S, st, x); },
+                 [&](int32_t x, uint2 rw, const DState& d) {
       uint32_t cnt = count_slots<8>(S, d, rw.x, lane, rw.y, 64, st, slots);
       for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off);
       if (lane == 0) {
@@ -901,8 +928,8 @@ __device__ __forceinline__ void round_c(const LoopState& S, int r, LoopLds& lds)
   // (K[x] was zeroed in A; the chunks add up)
   if (dhubs) {
     each_hub_chunk(S, lds, [&](int2 e) { return st_has(st, e.x, kFD); },
-                   [&](int32_t x, uint2 rw, uint32_t lo, uint32_t hi) {
-      const DState d = load_d(S, st, x);
+                   [&](int32_t x) { return load_d(S, st, x); },
+                   [&](int32_t x, uint2 rw, uint32_t lo, uint32_t hi, const DState& d) {
       uint32_t cnt = count_slots<4>(S, d, rw.x, lo + lane, hi, 64, st, slots);
       for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off);
       if (lane == 0) {
