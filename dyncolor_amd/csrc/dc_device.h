@@ -32,6 +32,22 @@ __device__ __forceinline__ void wave_append(bool p, T val, T* list, uint32_t* co
   if (p) list[base + lanes_below(m)] = val;
 }
 
+// wave_append into a list of `cap` entries: the counter counts every append,
+// entries past cap are not written (a count above cap marks the list as
+// incomplete).
+template <typename T>
+__device__ __forceinline__ void wave_append_capped(bool p, T val, T* list, uint32_t* counter,
+                                                   uint32_t cap) {
+  unsigned long long m = __ballot(p);
+  if (m == 0ull) return;
+  int leader = __ffsll((long long)m) - 1;
+  uint32_t base = 0;
+  if (lane_id() == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
+  base = __shfl(base, leader);
+  const uint32_t pos = base + lanes_below(m);
+  if (p && pos < cap) list[pos] = val;
+}
+
 // Wave-aggregated add of a per-lane value into a 64-bit counter.
 __device__ __forceinline__ void wave_add_u64(unsigned long long v, unsigned long long* ctr) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);

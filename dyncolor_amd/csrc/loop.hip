@@ -28,12 +28,20 @@
 // changed) are flags in St[r & 1], so L_r = C_{r-1} and the previous round's
 // D and R are in the other array, and the count pass learns a neighbour's
 // C, D, R and X membership from one gather.  A phase finds its vertices by
-// scanning state words; each wave compacts the vertices of its 128 words
-// (spread over the whole array, so the dense low-ID region of a power-law
-// graph lands on many waves) into a list in LDS.  There are no global list
-// counters: on this GPU a few thousand atomics on one address serialise into
-// tens of microseconds, so every update here is a flag set, a count change
-// at a scattered vertex, or a per-workgroup partial sum.
+// scanning state words (each wave compacts the vertices of its 128 words,
+// spread over the whole array so that the dense low-ID region of a
+// power-law graph lands on many waves, into a list in LDS) or, in rounds
+// whose sets are small, from worklists: a flag set on a vertex that had none
+// in the round appends the vertex to the round's touched list, and phase A
+// appends D's rows to three lists by count-pass path.  The next phase A then
+// takes the touched list instead of scanning, the count pass takes D's lists
+// instead of the state array and the row tables, and state arrays are
+// cleared entry by entry.  A scan costs the same at any set size (2 MB of
+// state words and a few hundred thousand table entries at 4.2 M vertices);
+// a list costs what the set holds.  Appends are wave-aggregated atomics on
+// one counter per list, and on this GPU a few thousand atomics on one address
+// serialise into tens of microseconds, so rounds with large sets (more than
+// kListC vertices in L, or more than kListSlots slots in D's rows) scan.
 //
 // One round = two phases:
 //   A(r): candidates = L_r | D_{r-1} | R_{r-1}.  K > 0 makes a member of C_r,
@@ -43,8 +51,8 @@
 //   C(r): rows of D_r: count changes, recounts, and for vertices whose colour
 //         changed the slot cache / mex cache / histograms of their neighbours.
 //         Rows of 64-511 slots go by the static histogram-row table, hub rows
-//         by the static hub chunk table.  The previous round's state array
-//         is cleared.
+//         by the static hub chunk table (or all three kinds by D's lists).
+//         The previous round's state array is cleared.
 // The plain driver launches one kernel per phase (the host enqueues rounds in
 // blocks and reads one control word per block; rounds past the last one
 // return at once); the persistent driver runs the rounds in one cooperative
@@ -67,6 +75,16 @@ constexpr int kAcc = 32;           // phase A sums: workgroup b adds into slot b
 constexpr int kAccW = 8;           // words per slot (five used)
 constexpr int32_t kLightC = 4096;  // rounds after one with fewer members use the light grid
 constexpr int kWaveList = 64 * 16; // vertices of 128 state words (16 per lane)
+// Worklists: a round lists its sets when L has at most list_c vertices (the
+// smaller of kListC and 1/64 of the words that the scans pass: below that an
+// append's atomic costs less than the scan words it saves), and R too when
+// D's rows hold at most kListSlots slots (and the round is not a pull round).
+constexpr int32_t kListC = 1 << 14;
+constexpr unsigned long long kListSlots = 1ull << 20;
+constexpr uint32_t kQInvalid = 0xffffffffu;  // touched count of a round that did not list
+constexpr int kQT = 0;      // qc[kQT + (r & 3)]: entries of round r's touched list
+constexpr int kQD = 4;      // qc[kQD + 3 * (r & 1) + k]: D_r's short rows, mid rows, hub chunks
+constexpr int kQWords = 10;
 
 struct LoopCtl {
   int32_t done_round;   // the round whose C was empty (INT_MAX while running)
@@ -102,6 +120,19 @@ struct LoopState {
   unsigned long long* bstat;  // [kMaxBlocks][4]
   int4* trace;         // per round {|L|, |C|, changed, 0}
   LoopCtl* ctl;
+  // worklists: round r's vertices with a flag in St[r & 1] in tq[r & 1]
+  // (count qc[kQT + (r & 3)], kQInvalid when the round did not list them
+  // all); D_r's rows without a histogram, histogram rows below kMidLen and
+  // hub chunks {vertex, chunk} in dq_s, dq_m, dq_h
+  int32_t* tq[2];
+  uint32_t tq_cap;
+  int32_t* dq_s;
+  int32_t* dq_m;
+  int2* dq_h;
+  uint32_t dq_s_cap, dq_m_cap, dq_h_cap;
+  uint32_t* qc;               // [kQWords]
+  int32_t list_c;             // rounds with |L| up to this list their sets
+  unsigned long long* wstat;  // 4-byte words of state arrays and lists passed by count passes
 };
 
 // LDS of a loop workgroup, shared by both phases
@@ -135,6 +166,7 @@ struct IncState {
   int32_t trace_cap = 0;
   LoopCtl* hctl = nullptr;
   unsigned long long* hstat = nullptr;
+  int32_t* tqbuf = nullptr;     // tq[0], tq[1]
 };
 
 // Membership flags of one vertex in a state array (one gather tells a
@@ -149,6 +181,11 @@ __device__ __forceinline__ bool st_has(const uint32_t* st, int32_t v, uint32_t f
 }
 __device__ __forceinline__ void st_set(uint32_t* st, int32_t v, uint32_t f) {
   atomicOr(&st[v >> 3], f << ((v & 7) << 2));
+}
+// st_set that tells whether v had no flag before
+__device__ __forceinline__ bool st_set_first(uint32_t* st, int32_t v, uint32_t f) {
+  const uint32_t sh = ((uint32_t)v & 7u) << 2;
+  return ((atomicOr(&st[(uint32_t)v >> 3], f << sh) >> sh) & 15u) == 0u;
 }
 
 // The marking predicate of findEligibleNeighbors for a slot (v -> w).
@@ -200,6 +237,59 @@ __device__ __forceinline__ void walk_states(const uint32_t* st, uint32_t swords,
   }
 }
 
+// The entries of a worklist by the same contract: each wave takes 64 entries
+// at a time (waves strided over the list), copies them into its LDS list and
+// calls fn(list, count) with the whole wave.
+template <class Fn>
+__device__ __forceinline__ void walk_list(const int32_t* q, uint32_t n, int32_t* list, Fn fn) {
+  const int lane = lane_id();
+  const uint32_t gwave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
+  for (uint64_t base = (uint64_t)gwave * 64; base < n; base += 64ull * nwaves) {
+    const uint64_t i = base + lane;
+    if (i < n) list[lane] = q[i];
+    const uint32_t total = (uint32_t)min<uint64_t>(64ull, (uint64_t)n - base);
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    fn(list, total);
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+  }
+}
+
+// Append the vertices v[k] with p[k] of the executing lanes to a list of cap
+// entries with one atomic for all U of them (entries past cap are counted,
+// not written).
+template <int U>
+__device__ __forceinline__ void wave_append_many(const bool (&p)[U], const int32_t (&v)[U],
+                                                 int32_t* list, uint32_t* counter, uint32_t cap) {
+  unsigned long long m[U];
+  uint32_t tot = 0;
+#pragma unroll
+  for (int k = 0; k < U; ++k) {
+    m[k] = __ballot(p[k]);
+    tot += (uint32_t)__popcll(m[k]);
+  }
+  if (tot == 0u) return;
+  const int leader = __ffsll((long long)__ballot(true)) - 1;
+  uint32_t base = 0;
+  if (lane_id() == leader) base = atomicAdd(counter, tot);
+  base = __shfl(base, leader);
+#pragma unroll
+  for (int k = 0; k < U; ++k) {
+    const uint32_t pos = base + lanes_below(m[k]);
+    if (p[k] && pos < cap) list[pos] = v[k];
+    base += (uint32_t)__popcll(m[k]);
+  }
+}
+
+// A Lister appends vertices to a worklist (off when q is null).
+struct Lister {
+  int32_t* q;
+  uint32_t* cnt;
+  uint32_t cap;
+};
+
 // Sum over the workgroup (every thread gets the total).
 __device__ __forceinline__ uint32_t block_sum(uint32_t v, uint32_t* scratch) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
@@ -241,7 +331,8 @@ __device__ __forceinline__ P bcast(const P& p, int src) {
   return o.p;
 }
 
-// Hub chunks whose entry passes act(e): the workgroup takes the table entries
+// Hub chunks of a chunk table (the static one or a list) whose entry passes
+// act(e): the workgroup takes the entries
 // blockIdx, blockIdx + gridDim, ... (consecutive chunks of one hub go to
 // different workgroups), compacts the active ones in LDS, and runs each chunk
 // with all its waves: wave q of the workgroup calls fn(x, row, lo, hi, p) on
@@ -249,15 +340,15 @@ __device__ __forceinline__ P bcast(const P& p, int src) {
 // up to 64 chunks are loaded at once, one per lane.  Every thread must call
 // this (it synchronises the workgroup).
 template <class Act, class Pre, class Fn>
-__device__ __forceinline__ void each_hub_chunk(const LoopState& S, LoopLds& lds, Act act, Pre pre,
-                                               Fn fn) {
+__device__ __forceinline__ void each_hub_chunk(const LoopState& S, LoopLds& lds, const int2* tab,
+                                               uint32_t ntab, Act act, Pre pre, Fn fn) {
   const GraphView& G = S.G;
   const int lane = lane_id();
   const int wib = threadIdx.x >> 6;
   constexpr uint32_t kQuarter = kChunk / (kBlock / 64);
-  for (uint64_t base = 0; base < S.n_hch; base += (uint64_t)kBlock * gridDim.x) {
+  for (uint64_t base = 0; base < ntab; base += (uint64_t)kBlock * gridDim.x) {
     const uint64_t i = base + (uint64_t)threadIdx.x * gridDim.x + blockIdx.x;
-    const int2 e = i < S.n_hch ? S.hch[i] : make_int2(0, -1);
+    const int2 e = i < ntab ? tab[i] : make_int2(0, -1);
     const bool a = e.y >= 0 && act(e);
     const unsigned long long am = __ballot(a);
     if (lane == 0) lds.nchunks[wib] = (uint32_t)__popcll(am);
@@ -290,8 +381,9 @@ __device__ __forceinline__ void each_hub_chunk(const LoopState& S, LoopLds& lds,
   }
 }
 
-// Rows of the histogram-row table (every row of kSmallLen or more slots of
-// capacity) that pass act(v) and are shorter than kMidLen (longer ones go by
+// Rows of a histogram-row table (the static one of every row of kSmallLen or
+// more slots of capacity, or a list) that pass act(v) and are shorter than
+// kMidLen (longer ones go by
 // the hub chunk table): the workgroup takes entries blockIdx, blockIdx +
 // gridDim, ..., compacts the active ones in LDS, and wave w takes entries w,
 // w + 4, ...: its lanes load those rows and pre(x) at once, then the wave
@@ -299,15 +391,15 @@ __device__ __forceinline__ void each_hub_chunk(const LoopState& S, LoopLds& lds,
 // the workgroups whatever their vertex IDs.  Every thread must call this (it
 // synchronises the workgroup).
 template <class Act, class Pre, class Fn>
-__device__ __forceinline__ void each_mid_row(const LoopState& S, LoopLds& lds, Act act, Pre pre,
-                                             Fn fn) {
+__device__ __forceinline__ void each_mid_row(const LoopState& S, LoopLds& lds, const int32_t* tab,
+                                             uint32_t ntab, Act act, Pre pre, Fn fn) {
   const GraphView& G = S.G;
   const int lane = lane_id();
   const int wib = threadIdx.x >> 6;
   constexpr uint32_t kWaves = kBlock / 64;
-  for (uint64_t base = 0; base < S.n_hrows; base += (uint64_t)kBlock * gridDim.x) {
+  for (uint64_t base = 0; base < ntab; base += (uint64_t)kBlock * gridDim.x) {
     const uint64_t i = base + (uint64_t)threadIdx.x * gridDim.x + blockIdx.x;
-    const int32_t v = i < S.n_hrows ? S.hrowv[i] : -1;
+    const int32_t v = i < ntab ? tab[i] : -1;
     const bool a = v >= 0 && act(v);
     const unsigned long long am = __ballot(a);
     if (lane == 0) lds.nchunks[wib] = (uint32_t)__popcll(am);
@@ -341,6 +433,10 @@ __global__ __launch_bounds__(kBlock) void k_loop_init(LoopState S) {
   const GraphView& G = S.G;
   const Frontier F = S.L0;
   const int lane = lane_id();
+  // list counts: round 0's start at zero; round -1 (the marked set) has no
+  // touched list
+  if (blockIdx.x == 0 && threadIdx.x < kQWords)
+    S.qc[threadIdx.x] = threadIdx.x == kQT + 3 ? kQInvalid : 0u;
   const int64_t gwave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
   const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
   auto row = [&](int32_t v, uint2 r, uint32_t lo, uint32_t hi, uint32_t stride, bool head) {
@@ -454,16 +550,32 @@ __device__ __forceinline__ void round_a(const LoopState& S, int r, LoopLds& lds)
   RA.G = G;
   uint32_t members = 0, changed = 0, dhubs = 0, dmids = 0;
   unsigned long long slots = 0, dsum = 0;
+  // rounds with a small L list their sets (uniform: the previous round's |C|)
+  const int32_t nl = r == 0 ? S.ctl->nl0 : S.trace[r - 1].y;
+  const bool listing = nl <= S.list_c;
+  uint32_t* qd = S.qc + kQD + 3 * (r & 1);
   // a row put into D: its slots, and which count-pass path takes it
   auto d_row = [&](int32_t w) {
     const uint32_t len = G.row[w].y;
     const int32_t h = G.hidx[w];
+    const bool hub = len >= kMidLen;
+    const bool mid = !hub && h >= 0;
     dsum += len;
-    dhubs += len >= kMidLen ? 1u : 0u;
-    dmids += len < kMidLen && h >= 0 ? 1u : 0u;
+    dhubs += hub ? 1u : 0u;
+    dmids += mid ? 1u : 0u;
+    if (listing) {
+      wave_append_capped(!hub && !mid, w, S.dq_s, qd + 0, S.dq_s_cap);
+      wave_append_capped(mid, w, S.dq_m, qd + 1, S.dq_m_cap);
+      if (hub) {
+        const uint32_t nch = (len + kChunk - 1) / kChunk;
+        const uint32_t b = atomicAdd(qd + 2, nch);
+        for (uint32_t c = 0; c < nch && b + c < S.dq_h_cap; ++c)
+          S.dq_h[b + c] = make_int2(w, (int32_t)c);
+      }
+    }
   };
 
-  walk_states(sp, S.swords, kFC | kFD | kFR, list, [&](int32_t* lst, uint32_t total) {
+  auto visit = [&](int32_t* lst, uint32_t total) {
     // members with a cached mex commit at once; the others (stale mex) are
     // compacted to the front of the list and evaluated by 8-lane groups
     uint32_t nd = 0;
@@ -482,6 +594,9 @@ __device__ __forceinline__ void round_a(const LoopState& S, int r, LoopLds& lds)
       }
       const bool member = valid && k > 0;
       const bool leaving = valid && !member && inl;
+      // w gets its first flag of the round (C or D) in this phase
+      if (listing)
+        wave_append_capped(member || leaving, w, S.tq[r & 1], S.qc + kQT + (r & 3), S.tq_cap);
       bool dv = leaving;
       if (leaving) {
         S.dold[w] = make_int2(so, (po + 1) | (int32_t)0x80000000);
@@ -538,7 +653,13 @@ __device__ __forceinline__ void round_a(const LoopState& S, int r, LoopLds& lds)
           commit_member(S, sn, w, -1, S.col0[w], S.prev[w], st_has(sp, w, kFC), m, changed))
         d_row(w);
     }
-  });
+  };
+  // the previous round's touched list when it has one, else the state array
+  const uint32_t tprev = S.qc[kQT + ((r + 3) & 3)];
+  if (tprev <= S.tq_cap)
+    walk_list(S.tq[(r + 1) & 1], tprev, list, visit);
+  else
+    walk_states(sp, S.swords, kFC | kFD | kFR, list, visit);
 
   const uint32_t bm = block_sum(members, red);
   const uint32_t bc = block_sum(changed, red);
@@ -594,7 +715,8 @@ __device__ __forceinline__ DState load_d(const LoopState& S, const uint32_t* st,
 template <int U>
 __device__ __forceinline__ uint32_t count_slots(const LoopState& S, const DState& d, uint64_t rbase,
                                                 uint32_t lo, uint32_t hi, uint32_t stride,
-                                                uint32_t* st, unsigned long long& slots) {
+                                                uint32_t* st, const Lister& tl,
+                                                unsigned long long& slots) {
   const GraphView& G = S.G;
   uint32_t cnt = 0;
   for (uint32_t j = lo; j < hi; j += U * stride) {
@@ -615,17 +737,24 @@ __device__ __forceinline__ uint32_t count_slots(const LoopState& S, const DState
       fl[k] = own ? st_flags(st[(uint32_t)y[k] >> 3], y[k]) : 0u;
     }
     int32_t gc[U], gp[U];
-    bool mem[U];
+    bool mem[U], fresh[U];
 #pragma unroll
     for (int k = 0; k < U; ++k) {
       const bool own = y[k] >= 0 && y[k] < G.n_owned;
+      fresh[k] = false;
       if (own && !(fl[k] & kFD)) {
         const bool to = d.inl && marks(d.x, d.so, d.po, y[k], c[k]);
         const bool tn = d.inc && marks(d.x, d.sn, d.pn, y[k], c[k]);
         if (to != tn) {
           atomicAdd(&S.K[y[k]], tn ? 1 : -1);
-          // R_r only gains members in this phase: a flag seen set stays set
-          if (tn && !(fl[k] & kFR)) st_set(st, y[k], kFR);
+          // R_r only gains members in this phase: a flag seen set stays set.
+          // A vertex that had no flag goes on the touched list.
+          if (tn && !(fl[k] & kFR)) {
+            if (tl.q)
+              fresh[k] = st_set_first(st, y[k], kFR);
+            else
+              st_set(st, y[k], kFR);
+          }
         }
       }
       mem[k] = own && (fl[k] & kFC);
@@ -638,6 +767,7 @@ __device__ __forceinline__ uint32_t count_slots(const LoopState& S, const DState
         gp[k] = S.prev[y[k]];
       }
     }
+    if (tl.q) wave_append_many<U>(fresh, y, tl.q, tl.cnt, tl.cap);
 #pragma unroll
     for (int k = 0; k < U; ++k)
       if (mem[k] && marks(y[k], gc[k], gp[k], d.x, d.sn)) ++cnt;
@@ -648,6 +778,19 @@ __device__ __forceinline__ uint32_t count_slots(const LoopState& S, const DState
 __device__ __forceinline__ void clear_words(uint32_t* b, uint32_t words) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < words; i += gridDim.x * blockDim.x)
     b[i] = 0u;
+}
+
+// Clear a state array: the words of a round's touched list when it has one
+// (n <= cap), else every word.  Returns the 4-byte words passed.
+__device__ __forceinline__ uint64_t clear_state(uint32_t* b, uint32_t words, const int32_t* q,
+                                                uint32_t n, uint32_t cap) {
+  if (n > cap) {
+    clear_words(b, words);
+    return words;
+  }
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    b[(uint32_t)q[i] >> 3] = 0u;
+  return 2ull * n;
 }
 
 // ---- heavy rounds: every K recounted by pull, no count atomics ------------
@@ -748,7 +891,8 @@ __device__ __forceinline__ void pull_all(const LoopState& S, uint32_t* st, LoopL
     __builtin_amdgcn_wave_barrier();
   }
   // rows with a histogram and fewer than kMidLen slots, one per wave
-  each_mid_row(S, lds, [](int32_t) { return true; }, [&](int32_t x) { return S.col0[x]; },
+  each_mid_row(S, lds, S.hrowv, S.n_hrows, [](int32_t) { return true; },
+               [&](int32_t x) { return S.col0[x]; },
                [&](int32_t x, uint2 rw, int32_t cx) {
     uint32_t c2 = pull_slots<8>(S, x, cx, rw.x, lane, rw.y, 64, st, slots);
     for (int off = 32; off > 0; off >>= 1) c2 += __shfl_xor(c2, off);
@@ -810,7 +954,7 @@ __device__ __forceinline__ void propagate_changed(const LoopState& S, const uint
       propagate_slots(G, S.col0[x], S.dold[x].x, rw.x, lane, rw.y, 64);
     }
   });
-  each_hub_chunk(S, lds, [&](int2 e) { return st_has(st, e.x, kFX); },
+  each_hub_chunk(S, lds, S.hch, S.n_hch, [&](int2 e) { return st_has(st, e.x, kFX); },
                  [&](int32_t x) { return make_int2(S.col0[x], S.dold[x].x); },
                  [&](int32_t x, uint2 rw, uint32_t lo, uint32_t hi, int2 cs) {
     propagate_slots(G, cs.x, cs.y, rw.x, lo + lane, hi, 64);
@@ -842,35 +986,58 @@ __device__ __forceinline__ void round_c(const LoopState& S, int r, LoopLds& lds)
   const bool heavy = lds.tot[2] > S.heavy_slots;
   const bool dhubs = lds.tot[3] != 0ull;
   const bool dmids = lds.tot[4] != 0ull;
+  // phase A listed round r's sets (the same test as there); R_r joins the
+  // touched list in light rounds with few D slots
+  const int32_t nl = r == 0 ? S.ctl->nl0 : S.trace[r - 1].y;
+  const bool listed = nl <= S.list_c;
+  const bool listing = listed && !heavy && lds.tot[2] <= kListSlots;
+  uint64_t words = 0;   // state-array and list words passed (block 0's count)
   if (blockIdx.x == 0) {
     // round r's counts into the trace; round r+1's sums start at zero
     if (threadIdx.x == 0) {
-      S.trace[r].x = r == 0 ? S.ctl->nl0 : S.trace[r - 1].y;
+      S.trace[r].x = nl;
       S.trace[r].y = (int32_t)nmem;
       S.trace[r].z = (int32_t)lds.tot[1];
       S.ctl->last_nc = (int32_t)nmem;
       S.ctl->last_changed = (int32_t)lds.tot[1];
       if (!nonempty) S.ctl->done_round = r;
+      // round r+1's list counts start at zero; a round that does not list
+      // R_r has no usable touched list (no block reads that count here)
+      S.qc[kQT + ((r + 1) & 3)] = 0u;
+      for (int k = 0; k < 3; ++k) S.qc[kQD + 3 * ((r + 1) & 1) + k] = 0u;
+      if (nonempty && !listing) S.qc[kQT + (r & 3)] = kQInvalid;
     }
     for (int i = threadIdx.x; i < kAcc * kAccW; i += kBlock)
       S.acc[(size_t)((r + 1) & 1) * kAcc * kAccW + i] = 0ull;
   }
   // L_r, D_{r-1}, R_{r-1} and X_{r-1} are no longer read
-  clear_words(S.St[(r + 1) & 1], S.swords);
+  words += clear_state(S.St[(r + 1) & 1], S.swords, S.tq[(r + 1) & 1], S.qc[kQT + ((r + 3) & 3)],
+                       S.tq_cap);
   if (!nonempty) {
-    clear_words(st, S.swords);   // D_r's leaving members
+    // D_r's leaving members
+    words += clear_state(st, S.swords, S.tq[r & 1], listed ? S.qc[kQT + (r & 3)] : kQInvalid,
+                         S.tq_cap);
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(S.wstat, (unsigned long long)words);
     return;
   }
   unsigned long long slots = 0, pslots = 0, rows = 0;
   if (heavy) {
     pull_all(S, st, lds, pslots, rows);
     propagate_changed(S, st, lds);
+    words += S.swords;
   } else {
+  // D_r's lists when phase A made them (and none overflowed)
+  const uint32_t* qd = S.qc + kQD + 3 * (r & 1);
+  const uint32_t nds = qd[0], ndm = qd[1], ndh = qd[2];
+  const bool dl = listed && nds <= S.dq_s_cap && ndm <= S.dq_m_cap && ndh <= S.dq_h_cap;
+  const Lister tl = listing ? Lister{S.tq[r & 1], S.qc + kQT + (r & 3), S.tq_cap}
+                            : Lister{nullptr, nullptr, 0u};
+  words += dl ? (uint64_t)nds + ndm + 2ull * ndh : (uint64_t)S.swords;
   // rows of D_r without a histogram (fewer than kSmallLen slots) by 8-lane
   // groups; the longer ones go by the histogram-row table and the hub chunk
   // table below.  A row whose owner changed colour then updates its
   // neighbours' caches.
-  walk_states(st, S.swords, kFD, list, [&](int32_t* lst, uint32_t total) {
+  auto short_rows = [&](int32_t* lst, uint32_t total) {
     const int grp = lane >> 3, sub = lane & 7;
     int32_t* pick = lds.pick[threadIdx.x >> 6];
     // 64 entries at a time: every lane loads its entry's row and state at
@@ -897,7 +1064,7 @@ __device__ __forceinline__ void round_c(const LoopState& S, int r, LoopLds& lds)
         // that sits out would get an undefined value)
         const uint32_t ry_all = (uint32_t)__shfl((int)rw.y, src);
         const uint32_t ry = mine ? ry_all : 0u;
-        uint32_t cnt = count_slots<4>(S, e, rx, sub, ry, kGroup, st, slots);
+        uint32_t cnt = count_slots<4>(S, e, rx, sub, ry, kGroup, st, tl, slots);
         cnt += __shfl_xor(cnt, 1);
         cnt += __shfl_xor(cnt, 2);
         cnt += __shfl_xor(cnt, 4);
@@ -910,12 +1077,17 @@ __device__ __forceinline__ void round_c(const LoopState& S, int r, LoopLds& lds)
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
       __builtin_amdgcn_wave_barrier();
     }
-  });
+  };
+  if (dl)
+    walk_list(S.dq_s, nds, list, short_rows);
+  else
+    walk_states(st, S.swords, kFD, list, short_rows);
   if (dmids) {
-    each_mid_row(S, lds, [&](int32_t v) { return st_has(st, v, kFD); },
+    each_mid_row(S, lds, dl ? S.dq_m : S.hrowv, dl ? ndm : S.n_hrows,
+                 [&](int32_t v) { return dl || st_has(st, v, kFD); },
                  [&](int32_t x) { return load_d(S, st, x); },
                  [&](int32_t x, uint2 rw, const DState& d) {
-      uint32_t cnt = count_slots<8>(S, d, rw.x, lane, rw.y, 64, st, slots);
+      uint32_t cnt = count_slots<8>(S, d, rw.x, lane, rw.y, 64, st, tl, slots);
       for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off);
       if (lane == 0) {
         S.K[x] = (int32_t)cnt;
@@ -927,10 +1099,11 @@ __device__ __forceinline__ void round_c(const LoopState& S, int r, LoopLds& lds)
   // hub rows of D_r by their kChunk-slot chunks, a quarter chunk per wave
   // (K[x] was zeroed in A; the chunks add up)
   if (dhubs) {
-    each_hub_chunk(S, lds, [&](int2 e) { return st_has(st, e.x, kFD); },
+    each_hub_chunk(S, lds, dl ? S.dq_h : S.hch, dl ? ndh : S.n_hch,
+                   [&](int2 e) { return dl || st_has(st, e.x, kFD); },
                    [&](int32_t x) { return load_d(S, st, x); },
                    [&](int32_t x, uint2 rw, uint32_t lo, uint32_t hi, const DState& d) {
-      uint32_t cnt = count_slots<4>(S, d, rw.x, lo + lane, hi, 64, st, slots);
+      uint32_t cnt = count_slots<4>(S, d, rw.x, lo + lane, hi, 64, st, tl, slots);
       for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off);
       if (lane == 0) {
         if (cnt) atomicAdd(&S.K[x], (int32_t)cnt);
@@ -950,6 +1123,7 @@ __device__ __forceinline__ void round_c(const LoopState& S, int r, LoopLds& lds)
     if (slots) atomicAdd(&S.bstat[blockIdx.x * 4 + 2], slots);
     if (rows) atomicAdd(&S.bstat[blockIdx.x * 4 + 3], rows);
   }
+  if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(S.wstat, (unsigned long long)words);
 }
 
 // ---------------------------------------------------------------------------
@@ -1056,10 +1230,22 @@ static int inc_alloc(Graph* g, IncState* s) {
   S.St[0] = s->bits;
   S.St[1] = s->bits + S.swords;
   DC_HIP(hipMalloc(&S.acc, sizeof(unsigned long long) * 2 * kAcc * kAccW));
-  DC_HIP(hipMalloc(&S.bstat, sizeof(unsigned long long) * 4 * kMaxBlocks));
+  // per-workgroup stats, then the count passes' word count (wstat)
+  DC_HIP(hipMalloc(&S.bstat, sizeof(unsigned long long) * (4 * kMaxBlocks + 1)));
+  S.wstat = S.bstat + 4 * kMaxBlocks;
   DC_HIP(hipMalloc(&S.ctl, sizeof(LoopCtl)));
+  DC_HIP(hipMalloc(&s->tqbuf, sizeof(int32_t) * 2 * n));
+  S.tq[0] = s->tqbuf;
+  S.tq[1] = s->tqbuf + n;
+  S.tq_cap = (uint32_t)n;
+  DC_HIP(hipMalloc(&S.dq_s, sizeof(int32_t) * n));
+  S.dq_s_cap = (uint32_t)n;
+  S.dq_m = nullptr;
+  S.dq_h = nullptr;
+  S.dq_m_cap = S.dq_h_cap = 0;
+  DC_HIP(hipMalloc(&S.qc, sizeof(uint32_t) * kQWords));
   DC_HIP(hipHostMalloc(&s->hctl, sizeof(LoopCtl), hipHostMallocDefault));
-  s->hstat = new unsigned long long[4 * kMaxBlocks];
+  s->hstat = new unsigned long long[4 * kMaxBlocks + 1];
   DC_HIP(hipMemsetAsync(S.K, 0, sizeof(int32_t) * n, g->stream));
   DC_HIP(hipMemsetAsync(s->bits, 0, sizeof(uint32_t) * 2 * (size_t)S.swords, g->stream));
 
@@ -1090,7 +1276,7 @@ void loop_free(Graph* g) {
     if (p) (void)hipFree(p);
   };
   f(s->S.K); f(s->S.dold); f(s->bits); f(s->S.acc); f(s->S.bstat); f(s->S.trace);
-  f(s->S.ctl);
+  f(s->S.ctl); f(s->tqbuf); f(s->S.dq_s); f(s->S.dq_m); f(s->S.dq_h); f(s->S.qc);
   if (s->hctl) (void)hipHostFree(s->hctl);
   for (hipEvent_t e : s->cev) (void)hipEventDestroy(e);
   delete[] s->hstat;
@@ -1175,15 +1361,32 @@ int loop_hi(Graph* g, Frontier L0, int32_t nl0, bool persistent, LoopResult* res
   S.n_hch = g->n_hch;
   S.hrowv = g->hrowv;
   S.n_hrows = g->n_hrows;
+  // D's mid-row and hub-chunk lists hold at most the static tables' entries
+  if (S.dq_m_cap < g->n_hrows) {
+    if (S.dq_m) DC_HIP(hipFree(S.dq_m));
+    S.dq_m = nullptr;
+    S.dq_m_cap = 0;
+    DC_HIP(hipMalloc(&S.dq_m, sizeof(int32_t) * ((size_t)g->hist_cap + 1)));
+    S.dq_m_cap = g->hist_cap + 1;
+  }
+  if (S.dq_h_cap < g->n_hch) {
+    if (S.dq_h) DC_HIP(hipFree(S.dq_h));
+    S.dq_h = nullptr;
+    S.dq_h_cap = 0;
+    DC_HIP(hipMalloc(&S.dq_h, sizeof(int2) * ((size_t)g->hch_cap + 1)));
+    S.dq_h_cap = g->hch_cap + 1;
+  }
   // rounds whose D rows hold more than 1/8 of all slots recount by pull
   // (measured on R-MAT 22 against 1/2 and 1/32)
   S.heavy_slots = std::max<unsigned long long>(g->rows_end / 8, 1ull << 16);
+  S.list_c = (int32_t)std::min<uint64_t>(
+      kListC, ((uint64_t)S.swords + g->n_hrows + g->n_hch) / 64);
 
   *s->hctl = LoopCtl{};
   s->hctl->done_round = INT_MAX;
   s->hctl->nl0 = nl0;
   DC_HIP(hipMemcpyAsync(S.ctl, s->hctl, sizeof(LoopCtl), hipMemcpyHostToDevice, g->stream));
-  DC_HIP(hipMemsetAsync(S.bstat, 0, sizeof(unsigned long long) * 4 * kMaxBlocks, g->stream));
+  DC_HIP(hipMemsetAsync(S.bstat, 0, sizeof(unsigned long long) * (4 * kMaxBlocks + 1), g->stream));
   DC_HIP(hipMemsetAsync(S.acc, 0, sizeof(unsigned long long) * 2 * kAcc * kAccW, g->stream));
   const dim3 blk(kBlock);
   hipLaunchKernelGGL(k_loop_init, dim3(kMaxBlocks), blk, 0, g->stream, S);
@@ -1286,7 +1489,7 @@ int loop_hi(Graph* g, Frontier L0, int32_t nl0, bool persistent, LoopResult* res
   std::vector<int4> tr((size_t)rounds);
   DC_HIP(hipMemcpyAsync(tr.data(), S.trace, sizeof(int4) * (size_t)rounds, hipMemcpyDeviceToHost,
                         g->stream));
-  DC_HIP(hipMemcpyAsync(s->hstat, S.bstat, sizeof(unsigned long long) * 4 * kMaxBlocks,
+  DC_HIP(hipMemcpyAsync(s->hstat, S.bstat, sizeof(unsigned long long) * (4 * kMaxBlocks + 1),
                         hipMemcpyDeviceToHost, g->stream));
   DC_HIP(hipStreamSynchronize(g->stream));
   res->rounds = rounds;
@@ -1309,13 +1512,13 @@ int loop_hi(Graph* g, Frontier L0, int32_t nl0, bool persistent, LoopResult* res
   res->slots = pull + cnt;
   res->kernel_ms = kms;
   res->launches = launches;
-  // Algorithmic bytes of a round's phases over the state arrays: phase A
-  // walks one, the count pass clears one and walks one (4 B per word each);
-  // per count-pass slot col + ccol (8 B); per pulled slot col (4 B); per row
-  // or chunk visit its bounds and count or saved state (16 B).
+  // Algorithmic bytes of the count passes: the state-array words they clear
+  // and walk and the list entries they read (4 B each, counted on the
+  // device); per count-pass slot col + ccol (8 B); per pulled slot col (4 B);
+  // per row or chunk visit its bounds and count or saved state (16 B).
   res->count_ms = cms;
   res->count_launches = claunches;
-  res->count_bytes = (uint64_t)rounds * 12ull * S.swords + 8ull * cnt + 4ull * pull + 16ull * rowv;
+  res->count_bytes = 4ull * s->hstat[4 * kMaxBlocks] + 8ull * cnt + 4ull * pull + 16ull * rowv;
   return DC_OK;
 }
 
