@@ -720,14 +720,17 @@ __global__ void k_apply(int64_t k, const int32_t* bu, const int32_t* bv, const u
 // (deleteEdgeFromAdj, HI/gpuFunctions_undir.cuh:118-142, scans both rows; the
 // twin links make one scan of min(deg) slots enough).  A ghost endpoint has
 // no row: the owned row is scanned.
+// One 8-lane group per line (most scanned rows are short: a wave would leave
+// most lanes idle and keep one line's chain of dependent loads in flight
+// instead of eight).
 __global__ void k_tombstone(const int2* del, uint32_t ndel, const int2* xdel, uint32_t nxdel,
                             GraphView G, Counters* ctr) {
-  const int lane = lane_id();
-  const int64_t wave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
-  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  const int sub = lane_id() & (kGroup - 1);
+  const int64_t grp = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / kGroup;
+  const int64_t ngrp = ((int64_t)gridDim.x * blockDim.x) / kGroup;
   const int64_t items = (int64_t)ndel + nxdel;
   unsigned long long tomb = 0;
-  for (int64_t it = wave; it < items; it += nwaves) {
+  for (int64_t it = grp; it < items; it += ngrp) {
     const bool cross = it >= (int64_t)ndel;
     const int2 e = cross ? xdel[it - ndel] : del[it];
     int32_t x = e.x, y = e.y;   // cross lines are (owned, ghost)
@@ -748,18 +751,18 @@ __global__ void k_tombstone(const int2* del, uint32_t ndel, const int2* xdel, ui
     // (the same edge can come twice, in either orientation), which also
     // takes its colour out of the row's histogram.
     const int32_t hx = G.hidx[x], hy = cross ? -1 : G.hidx[y];
-    for (uint32_t j0 = lane; j0 < r.y; j0 += 8 * 64) {
+    for (uint32_t j0 = sub; j0 < r.y; j0 += 8 * kGroup) {
       // 8 loads in flight per lane; matches are rare
       int32_t cv[8];
 #pragma unroll
       for (int k = 0; k < 8; ++k) {
-        const uint32_t jj = j0 + k * 64;
+        const uint32_t jj = j0 + k * kGroup;
         cv[k] = jj < r.y ? G.col[(uint64_t)r.x + jj] : -2;
       }
 #pragma unroll
       for (int k = 0; k < 8; ++k) {
         if (cv[k] != y) continue;
-        const uint64_t s = (uint64_t)r.x + j0 + k * 64;
+        const uint64_t s = (uint64_t)r.x + j0 + k * kGroup;
         if (atomicExch(&G.col[s], -1) == y) {
           const int32_t c = atomicExch(&G.ccol[s], -1);
           if (hx >= 0 && c >= 0 && c < kHistW) atomicSub(&G.hist[(uint64_t)hx * kHistW + c], 1u);
@@ -976,7 +979,7 @@ int csr_apply(Graph* g, const Batch& b, int rule) {
   }
   uint32_t items = c.n_del + c.n_xdel;
   if (items > 0) {
-    int64_t threads = (int64_t)items * 64;
+    int64_t threads = (int64_t)items * kGroup;
     hipLaunchKernelGGL(k_tombstone, dim3(grid_for(threads)), dim3(kBlock), 0, g->stream, g->del,
                        c.n_del, g->xdel, c.n_xdel, g->view(), g->ctr);
     DC_HIP(hipGetLastError());
