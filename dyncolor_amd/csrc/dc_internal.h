@@ -51,7 +51,8 @@ struct Counters {
   unsigned long long mono;    // validate
   int32_t maxc;
   int32_t window_full;    // a hub mex window was full -> slow path used
-  uint32_t pad1[2];
+  int32_t cmax;           // highest colour (insertion round: are histograms complete?)
+  uint32_t pad1;
 };
 
 // Everything a kernel needs about one graph, passed by value.

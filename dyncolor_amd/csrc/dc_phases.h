@@ -720,26 +720,25 @@ __device__ __forceinline__ void targets_ldf_body(const int2* del, const uint32_t
   }
 }
 
-// Number of distinct colors among v's live neighbours (saturation size, the
-// popcount of computeSC's mask, HI/gpuFunctions_undir.cuh:203-207), one wave,
-// windows of 8192 colors in LDS.
-__device__ inline uint32_t wave_distinct_colors(const GraphView& G, const int32_t* cur, int32_t v,
-                                                unsigned long long* ww) {
-  const int lane = lane_id();
-  const uint2 r = G.row[v];
+// Number of distinct colors among a row's live neighbours (saturation size,
+// the popcount of computeSC's mask, HI/gpuFunctions_undir.cuh:203-207) by an
+// 8-lane group: lane `sub` reads slots sub, sub + 8, ...; windows of kHistW
+// colours in the group's 8 LDS words (one pass per window up to the row's
+// highest colour).
+__device__ __forceinline__ uint32_t group_distinct_colors(const GraphView& G, uint2 r, int sub,
+                                                          unsigned long long* gw) {
   uint32_t total = 0;
   int32_t rowmax = 0;
-  for (int32_t base = 0; base <= rowmax; base += kWinBits) {
-    ww[lane] = 0ull;
-    ww[lane + 64] = 0ull;
+  for (int32_t base = 0; base <= rowmax; base += kHistW) {
+    gw[sub] = 0ull;
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
     int32_t lmax = 0;
-    for (uint32_t j0 = lane; j0 < r.y; j0 += 8 * 64) {
+    for (uint32_t j0 = sub; j0 < r.y; j0 += 8 * kGroup) {
       int32_t cc[8];
 #pragma unroll
       for (int k = 0; k < 8; ++k) {
-        const uint32_t jj = j0 + k * 64;
+        const uint32_t jj = j0 + k * kGroup;
         cc[k] = jj < r.y ? G.ccol[(uint64_t)r.x + jj] : -1;
       }
 #pragma unroll
@@ -748,46 +747,140 @@ __device__ inline uint32_t wave_distinct_colors(const GraphView& G, const int32_
         if (c < 0) continue;
         lmax = max(lmax, c);
         const int32_t cw = c - base;
-        if (cw >= 0 && cw < kWinBits) atomicOr(&ww[cw >> 6], 1ull << (cw & 63));
+        if (cw >= 0 && cw < kHistW) atomicOr(&gw[cw >> 6], 1ull << (cw & 63));
       }
     }
-    for (int off = 32; off > 0; off >>= 1) lmax = max(lmax, __shfl_xor(lmax, off));
+    lmax = max(lmax, __shfl_xor(lmax, 1));
+    lmax = max(lmax, __shfl_xor(lmax, 2));
+    lmax = max(lmax, __shfl_xor(lmax, 4));
     if (base == 0) rowmax = lmax;
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
-    uint32_t pc = (uint32_t)(__popcll(ww[lane]) + __popcll(ww[lane + 64]));
-    for (int off = 32; off > 0; off >>= 1) pc += __shfl_xor(pc, off);
+    uint32_t pc = (uint32_t)__popcll(gw[sub]);
+    pc += __shfl_xor(pc, 1);
+    pc += __shfl_xor(pc, 2);
+    pc += __shfl_xor(pc, 4);
     total += pc;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
   }
   return total;
+}
+
+// The same count from a row's colour histogram (nonzero bins) by an 8-lane
+// group.  Exact when no colour reaches kHistW.
+__device__ __forceinline__ uint32_t group_hist_distinct(const uint32_t* hrow, int sub) {
+  const uint4* h4 = reinterpret_cast<const uint4*>(hrow);
+  uint32_t c = 0;
+#pragma unroll
+  for (int k = 0; k < kHistW / 4 / kGroup; ++k) {
+    const uint4 b = h4[k * kGroup + sub];
+    c += (b.x != 0u) + (b.y != 0u) + (b.z != 0u) + (b.w != 0u);
+  }
+  c += __shfl_xor(c, 1);
+  c += __shfl_xor(c, 2);
+  c += __shfl_xor(c, 4);
+  return c;
+}
+
+// highest colour of cur[0..n) into *cmax (zeroed beforehand): one atomic per
+// workgroup
+__device__ __forceinline__ void colour_max_body(const int32_t* cur, int32_t n, int32_t* cmax) {
+  __shared__ int32_t wmax[kBlock / 64];
+  int32_t m = 0;
+  for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    m = max(m, cur[i]);
+  for (int off = 32; off > 0; off >>= 1) m = max(m, __shfl_xor(m, off));
+  if (lane_id() == 0) wmax[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int q = 1; q < kBlock / 64; ++q) m = max(m, wmax[q]);
+    if (m > 0) atomicMax(cmax, m);
+  }
+}
+
+// frontier_push for a whole workgroup (every thread calls it): one atomic
+// per list and workgroup instead of one per wave.
+__device__ __forceinline__ void block_frontier_push(bool p, int32_t v, const GraphView& G,
+                                                    Frontier F, uint32_t hub_cap,
+                                                    uint32_t (*cnt)[2], uint32_t* base) {
+  const int lane = lane_id();
+  const int wib = threadIdx.x >> 6;
+  const uint32_t len = p ? G.row[v].y : 0u;
+  const bool is_s = p && len < kSmallLen;
+  const bool is_m = p && len >= kSmallLen && len < kMidLen;
+  const bool is_h = p && len >= kMidLen;
+  const unsigned long long ms = __ballot(is_s), mm = __ballot(is_m);
+  if (lane == 0) {
+    cnt[wib][0] = (uint32_t)__popcll(ms);
+    cnt[wib][1] = (uint32_t)__popcll(mm);
+  }
+  __syncthreads();
+  if (threadIdx.x < 2) {
+    uint32_t tot = 0;
+    for (int q = 0; q < kBlock / 64; ++q) tot += cnt[q][threadIdx.x];
+    base[threadIdx.x] = tot ? atomicAdd(&F.cnt[threadIdx.x], tot) : 0u;
+  }
+  __syncthreads();
+  uint32_t os = base[0], om = base[1];
+  for (int q = 0; q < wib; ++q) {
+    os += cnt[q][0];
+    om += cnt[q][1];
+  }
+  if (is_s) F.s[os + lanes_below(ms)] = v;
+  if (is_m) F.m[om + lanes_below(mm)] = v;
+  if (is_h) {
+    const uint32_t nch = (len + kChunk - 1) / kChunk;
+    const uint32_t slot = atomicAdd(&F.cnt[3], 1u);
+    if (slot < hub_cap) {
+      const uint32_t b = atomicAdd(&F.cnt[2], nch);
+      for (uint32_t c = 0; c < nch; ++c) F.h[b + c] = make_int4(v, (int)c, (int)slot, (int)nch);
+    }
+  }
+  __syncthreads();
 }
 
 // insertion round: for an edge whose endpoints share a snapshot color, the
 // endpoint with fewer distinct neighbour colors is the target (tie -> node2,
 // HI/gpuFunctions_undir.cuh:186-209); a ghost endpoint counts as +inf (its
 // owner's size is not available on this GPU, MG/gpuFunctions_undir.cuh:
-// 260-263).  One wave per edge.
+// 260-263).  One 8-lane group per edge (most endpoints have a handful of
+// neighbours); when no colour reaches kHistW (*cmax; no test when cmax is
+// null), a row with a colour histogram counts its nonzero bins instead of
+// scanning its slots.  Targets go into the frontier a workgroup at a time.
 __device__ __forceinline__ void targets_ins_body(const int2* ins, const uint32_t* nins_p, GraphView G,
-                                                 const int32_t* cur, uint32_t* tmark, uint32_t epoch,
-                                                 Frontier T, uint32_t hub_cap) {
-  __shared__ unsigned long long win[kBlock / 64][kWinWords];
-  const int lane = lane_id();
-  const int64_t gwave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
-  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
-  unsigned long long* ww = win[threadIdx.x >> 6];
+                                                 const int32_t* cur, const int32_t* cmax,
+                                                 uint32_t* tmark, uint32_t epoch, Frontier T,
+                                                 uint32_t hub_cap) {
+  constexpr int kGroups = kBlock / kGroup;
+  __shared__ unsigned long long gwin[kGroups][kHistW / 64];
+  __shared__ uint32_t pcnt[kBlock / 64][2];
+  __shared__ uint32_t pbase[2];
+  const int sub = lane_id() & (kGroup - 1);
+  const int gib = threadIdx.x / kGroup;
+  unsigned long long* gw = gwin[gib];
   const uint32_t ni = *nins_p;
-  for (int64_t i = gwave; i < ni; i += nwaves) {
-    const int2 e = ins[i];
+  const bool hist_ok = cmax != nullptr && *cmax < kHistW;
+  auto distinct = [&](int32_t v) -> uint32_t {
+    if (v >= G.n_owned) return 0xffffffffu;
+    const int32_t h = hist_ok ? G.hidx[v] : -1;
+    return h >= 0 ? group_hist_distinct(G.hist + (uint64_t)h * kHistW, sub)
+                  : group_distinct_colors(G, G.row[v], sub, gw);
+  };
+  for (uint64_t b0 = (uint64_t)blockIdx.x * kGroups; b0 < ni; b0 += (uint64_t)gridDim.x * kGroups) {
+    const uint64_t i = b0 + gib;
     bool push = false;
     int32_t t = 0;
-    if (cur[e.x] == cur[e.y]) {
-      const uint32_t ka = e.x >= G.n_owned ? 0xffffffffu : wave_distinct_colors(G, cur, e.x, ww);
-      const uint32_t kb = e.y >= G.n_owned ? 0xffffffffu : wave_distinct_colors(G, cur, e.y, ww);
-      t = ka < kb ? e.x : e.y;
-      if (lane == 0) push = t < G.n_owned && claim(tmark, t, epoch);
+    if (i < ni) {
+      const int2 e = ins[i];
+      if (cur[e.x] == cur[e.y]) {
+        const uint32_t ka = distinct(e.x);
+        const uint32_t kb = distinct(e.y);
+        t = ka < kb ? e.x : e.y;
+        if (sub == 0) push = t < G.n_owned && claim(tmark, t, epoch);
+      }
     }
-    frontier_push(push, t, G, T, hub_cap);
+    block_frontier_push(push, t, G, T, hub_cap, pcnt, pbase);
   }
 }
 

@@ -170,7 +170,7 @@ __global__ __launch_bounds__(kBlock) void k_fused(FusedArgs A) {
     }
     if (ni > 0) {
       catchup_body(A.Mk, G, color[cur], color[cur ^ 1]);
-      targets_ins_body(A.ins, &A.ctr->n_ins, G, color[cur], A.tmark, ++epoch, A.T2, A.hub_cap);
+      targets_ins_body(A.ins, &A.ctr->n_ins, G, color[cur], nullptr, A.tmark, ++epoch, A.T2, A.hub_cap);
       grid.sync();
       recolor_phase<M_INS>(grid, rargs(A.T2, &A.chg[2]));
       uint32_t nt = fcount(A.T2), ch = A.chg[2];
@@ -182,7 +182,7 @@ __global__ __launch_bounds__(kBlock) void k_fused(FusedArgs A) {
     }
     if (nxi > 0) {
       catchup_body(A.Mk, G, color[cur], color[cur ^ 1]);
-      targets_ins_body(A.xins, &A.ctr->n_xins, G, color[cur], A.tmark, ++epoch, A.T3, A.hub_cap);
+      targets_ins_body(A.xins, &A.ctr->n_xins, G, color[cur], nullptr, A.tmark, ++epoch, A.T3, A.hub_cap);
       grid.sync();
       recolor_phase<M_INS>(grid, rargs(A.T3, &A.chg[3]));
       uint32_t nt = fcount(A.T3), ch = A.chg[3];

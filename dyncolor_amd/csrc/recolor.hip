@@ -56,9 +56,14 @@ __global__ void k_targets_ldf(const int2* del, const uint32_t* ndel_p, const int
 
 __global__ __launch_bounds__(kBlock) void k_targets_ins(const int2* ins, const uint32_t* nins_p,
                                                         GraphView G, const int32_t* cur,
-                                                        uint32_t* tmark, uint32_t epoch, Frontier T,
+                                                        const int32_t* cmax, uint32_t* tmark,
+                                                        uint32_t epoch, Frontier T,
                                                         uint32_t hub_cap) {
-  targets_ins_body(ins, nins_p, G, cur, tmark, epoch, T, hub_cap);
+  targets_ins_body(ins, nins_p, G, cur, cmax, tmark, epoch, T, hub_cap);
+}
+
+__global__ __launch_bounds__(kBlock) void k_colour_max(const int32_t* cur, int32_t n, int32_t* cmax) {
+  colour_max_body(cur, n, cmax);
 }
 
 __global__ void k_catchup(Frontier F, GraphView G, const int32_t* cur, int32_t* nxt) {
@@ -341,6 +346,12 @@ int recolor_plain(Graph* g, int rule, dc_stats* st, bool persistent_loop) {
   };
 
   const int fg = frontier_grid(g);
+  // highest current colour, for the insertion targets' histogram test
+  auto launch_colour_max = [&]() {
+    (void)hipMemsetAsync(&g->ctr->cmax, 0, sizeof(int32_t), g->stream);
+    hipLaunchKernelGGL(k_colour_max, dim3(grid_threads(g->n) < 256 ? grid_threads(g->n) : 256),
+                       blk, 0, g->stream, g->color[g->cur], g->n, &g->ctr->cmax);
+  };
   if (rule == DC_RULE_INDEPENDENT) {
     if (staged.n_del > 0) {
       rc = run_targets_round(1, M_DEL, [&](uint32_t e) {
@@ -360,17 +371,19 @@ int recolor_plain(Graph* g, int rule, dc_stats* st, bool persistent_loop) {
     }
     if (staged.n_ins > 0) {
       rc = run_targets_round(2, M_INS, [&](uint32_t e) {
-        hipLaunchKernelGGL(k_targets_ins, dim3(grid_waves(staged.n_ins)), blk, 0, g->stream,
-                           g->ins, d_nins, g->view(), g->color[g->cur], g->tmark, e, g->T,
-                           g->hub_cap);
+        launch_colour_max();
+        hipLaunchKernelGGL(k_targets_ins, dim3(grid_threads((int64_t)staged.n_ins * kGroup)), blk, 0, g->stream,
+                           g->ins, d_nins, g->view(), g->color[g->cur], &g->ctr->cmax, g->tmark, e,
+                           g->T, g->hub_cap);
       });
       if (rc) return rc;
     }
     if (staged.n_xins > 0) {
       rc = run_targets_round(5, M_INS, [&](uint32_t e) {
-        hipLaunchKernelGGL(k_targets_ins, dim3(grid_waves(staged.n_xins)), blk, 0, g->stream,
-                           g->xins, d_nxins, g->view(), g->color[g->cur], g->tmark, e, g->T,
-                           g->hub_cap);
+        launch_colour_max();
+        hipLaunchKernelGGL(k_targets_ins, dim3(grid_threads((int64_t)staged.n_xins * kGroup)), blk, 0, g->stream,
+                           g->xins, d_nxins, g->view(), g->color[g->cur], &g->ctr->cmax, g->tmark,
+                           e, g->T, g->hub_cap);
       });
       if (rc) return rc;
     }
