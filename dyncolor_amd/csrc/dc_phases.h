@@ -783,12 +783,22 @@ __device__ __forceinline__ uint32_t group_hist_distinct(const uint32_t* hrow, in
   return c;
 }
 
-// highest colour of cur[0..n) into *cmax (zeroed beforehand): one atomic per
-// workgroup
+// highest colour of cur[0..n) into *cmax (zeroed beforehand): 16-byte loads,
+// four in flight per lane, one atomic per workgroup
 __device__ __forceinline__ void colour_max_body(const int32_t* cur, int32_t n, int32_t* cmax) {
   __shared__ int32_t wmax[kBlock / 64];
   int32_t m = 0;
-  for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+  const int4* c4 = reinterpret_cast<const int4*>(cur);
+  const uint32_t n4 = (uint32_t)n / 4;
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += 4 * stride) {
+    int4 x[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) x[k] = i + k * stride < n4 ? c4[i + k * stride] : make_int4(0, 0, 0, 0);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) m = max(m, max(max(x[k].x, x[k].y), max(x[k].z, x[k].w)));
+  }
+  for (uint32_t i = 4 * n4 + blockIdx.x * blockDim.x + threadIdx.x; i < (uint32_t)n; i += stride)
     m = max(m, cur[i]);
   for (int off = 32; off > 0; off >>= 1) m = max(m, __shfl_xor(m, off));
   if (lane_id() == 0) wmax[threadIdx.x >> 6] = m;
