@@ -43,7 +43,7 @@ EXPORTS = [
 class Options(C.Structure):
     _fields_ = [("slack_frac", C.c_float), ("slack_min", C.c_int32), ("max_rounds", C.c_int32),
                 ("driver", C.c_int32), ("device", C.c_int32), ("spare_slots", C.c_int32),
-                ("exact_rounds", C.c_int32), ("reserved", C.c_int32 * 1)]
+                ("exact_rounds", C.c_int32), ("list_limit", C.c_int32)]
 
 
 class Stats(C.Structure):

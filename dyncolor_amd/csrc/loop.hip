@@ -132,6 +132,7 @@ struct LoopState {
   uint32_t dq_s_cap, dq_m_cap, dq_h_cap;
   uint32_t* qc;               // [kQWords]
   int32_t list_c;             // rounds with |L| up to this list their sets
+  unsigned long long list_slots;  // ... and R too when D's rows hold at most this many slots
   unsigned long long* wstat;  // 4-byte words of state arrays and lists passed by count passes
 };
 
@@ -995,7 +996,7 @@ __device__ __forceinline__ void round_c(const LoopState& S, int r, LoopLds& lds)
   // touched list in light rounds with few D slots
   const int32_t nl = r == 0 ? S.ctl->nl0 : S.trace[r - 1].y;
   const bool listed = nl <= S.list_c;
-  const bool listing = listed && !heavy && lds.tot[2] <= kListSlots;
+  const bool listing = listed && !heavy && lds.tot[2] <= S.list_slots;
   uint64_t words = 0;   // state-array and list words passed (block 0's count)
   if (blockIdx.x == 0) {
     // round r's counts into the trace; round r+1's sums start at zero
@@ -1386,6 +1387,11 @@ int loop_hi(Graph* g, Frontier L0, int32_t nl0, bool persistent, LoopResult* res
   S.heavy_slots = std::max<unsigned long long>(g->rows_end / 8, 1ull << 16);
   S.list_c = (int32_t)std::min<uint64_t>(
       kListC, ((uint64_t)S.swords + g->n_hrows + g->n_hch) / 64);
+  S.list_slots = kListSlots;
+  if (g->opt.list_limit != 0) {
+    S.list_c = g->opt.list_limit < 0 ? -1 : g->opt.list_limit;
+    S.list_slots = ~0ull;
+  }
 
   *s->hctl = LoopCtl{};
   s->hctl->done_round = INT_MAX;

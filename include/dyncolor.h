@@ -64,7 +64,12 @@ typedef struct {
                           holds the mex of its neighbours' colours: the reference's
                           remaining rounds change no colour).  Colours are the same
                           either way; only the trace and round counts differ.         */
-  int32_t reserved[1];
+  int32_t list_limit;  /* independent-rule loop rounds with at most this many vertices
+                          in L take their sets from worklists instead of scanning the
+                          per-vertex state arrays; 0 (default): automatic (at most
+                          16384, and fewer on small graphs); -1: never; > 0 also lists
+                          R in every round that is not a pull round.  Results do not
+                          depend on it.                                             */
 } dc_options;
 
 /* Per-batch counters.  Byte counts follow the algorithmic model: each visited

@@ -296,3 +296,34 @@ def test_settle_and_exact_rounds(oracle_mod):
     _, _, st = run_both(oracle_mod, n, u, v, batches, 0)
     assert any(s.loop_settled for s in st)
     assert sum(s.loop_rounds for s in st) < sum(s.loop_rounds for s in st_exact)
+
+
+@pytest.mark.parametrize("limit", [-1, 1 << 30])
+def test_worklist_modes(oracle_mod, limit):
+    # the loop's sets by state-array scans only (-1) or from worklists in
+    # every round that is not a pull round (1 << 30): same colours and
+    # traces as the oracle either way, and run_both's consistency check finds
+    # the state arrays cleared between batches
+    say("worklists", limit)
+    u, v = d.gen_rmat(15, 16, seed=3)
+    n = 1 << 15
+    batches = d.gen_batches(n, u, v, 4, 6000, 0.5, 21)
+    run_both(oracle_mod, n, u, v, batches, 0, opts=d.options(exact_rounds=1, list_limit=limit))
+    run_both(oracle_mod, n, u, v, batches, 0, opts=d.options(list_limit=limit), device_batches=True)
+    rng = random.Random(29)
+    for trial in range(4):
+        nn = rng.randint(200, 3000)
+        n_owned = rng.randint(100, nn - 5)
+        uu, vv = d.gen_er(nn, 5 * nn, 300 + trial)
+        keep = (uu < n_owned) | (vv < n_owned)
+        uu, vv = uu[keep], vv[keep]
+        ghost = np.array([rng.randint(0, 6) for _ in range(nn - n_owned)], np.int32)
+        lines = []
+        while len(lines) < nn // 5:
+            a, b = rng.sample(range(nn), 2)
+            if a >= n_owned and b >= n_owned:
+                continue
+            lines.append((a, b, rng.randint(0, 1)))
+        la = np.array(lines, np.int32)
+        run_both(oracle_mod, nn, uu, vv, [(la[:, 0], la[:, 1], la[:, 2].astype(np.uint8))], 0,
+                 opts=d.options(list_limit=limit), n_owned=n_owned, ghost_colors=ghost)

@@ -117,3 +117,14 @@ def test_fused_equals_plain_larger(oracle_mod):
         assert g.validate()[0] == 0
     assert (outs[0][0] == outs[1][0]).all()
     assert outs[0][1] == outs[1][1]
+
+
+def test_fused_worklists_every_round(oracle_mod):
+    # the persistent driver with worklists in every round that is not a pull
+    # round: colours and traces equal to the oracle's
+    say("fused worklists")
+    u, v = d.gen_rmat(14, 16, seed=5)
+    n = 1 << 14
+    batches = d.gen_batches(n, u, v, 3, 4000, 0.5, 33)
+    run_both(oracle_mod, n, u, v, batches, 0, opts=d.options(exact_rounds=1, list_limit=1 << 30),
+             driver=d.DRIVER_FUSED)
