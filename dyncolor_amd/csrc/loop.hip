@@ -152,6 +152,7 @@ struct LoopLds {
     unsigned long long wins[kBlock / 64][kWinWords];
     struct {
       uint32_t red[kBlock / 64];
+      uint32_t red4[kBlock / 64][4];
       unsigned long long red64[kBlock / 64];
       unsigned long long tot[5];
       uint32_t nchunks[kBlock / 64];
@@ -539,7 +540,6 @@ __device__ __forceinline__ int32_t group_hist_mex(const uint32_t* hrow, int sub)
 }
 
 __device__ __forceinline__ void round_a(const LoopState& S, int r, LoopLds& lds) {
-  uint32_t* red = lds.red;
   const GraphView& G = S.G;
   const uint32_t* sp = S.St[(r + 1) & 1];   // L_r = C_{r-1}, D_{r-1}, R_{r-1}
   uint32_t* sn = S.St[r & 1];               // C_r, D_r, X_r
@@ -662,13 +662,35 @@ __device__ __forceinline__ void round_a(const LoopState& S, int r, LoopLds& lds)
   else
     walk_states(sp, S.swords, kFC | kFD | kFR, list, visit);
 
-  const uint32_t bm = block_sum(members, red);
-  const uint32_t bc = block_sum(changed, red);
-  const uint32_t bh = block_sum(dhubs, red);
-  const uint32_t bmid = block_sum(dmids, red);
-  const unsigned long long bd = block_sum64(dsum, lds.red64);
-  for (int off = 32; off > 0; off >>= 1) slots += __shfl_xor(slots, off);
+  // the workgroup's sums in one reduction (after the barrier every wave is
+  // done with its mex window, which shares LDS with red4 and red64)
+  for (int off = 32; off > 0; off >>= 1) {
+    members += __shfl_xor(members, off);
+    changed += __shfl_xor(changed, off);
+    dhubs += __shfl_xor(dhubs, off);
+    dmids += __shfl_xor(dmids, off);
+    dsum += __shfl_xor(dsum, off);
+    slots += __shfl_xor(slots, off);
+  }
+  __syncthreads();
+  if (lane == 0) {
+    lds.red4[wib][0] = members;
+    lds.red4[wib][1] = changed;
+    lds.red4[wib][2] = dhubs;
+    lds.red4[wib][3] = dmids;
+    lds.red64[wib] = dsum;
+  }
+  __syncthreads();
   if (threadIdx.x == 0) {
+    uint32_t bm = 0, bc = 0, bh = 0, bmid = 0;
+    unsigned long long bd = 0;
+    for (int q = 0; q < kBlock / 64; ++q) {
+      bm += lds.red4[q][0];
+      bc += lds.red4[q][1];
+      bh += lds.red4[q][2];
+      bmid += lds.red4[q][3];
+      bd += lds.red64[q];
+    }
     unsigned long long* a = S.acc + ((size_t)(r & 1) * kAcc + blockIdx.x % kAcc) * kAccW;
     if (bm) atomicAdd(&a[0], (unsigned long long)bm);
     if (bc) atomicAdd(&a[1], (unsigned long long)bc);
