@@ -994,6 +994,14 @@ __device__ __forceinline__ void round_c(const LoopState& S, int r, LoopLds& lds)
   uint32_t* st = S.St[r & 1];   // C_r, D_r, X_r; R_r is filled here
   const int lane = lane_id();
   int32_t* list = lds.lists[threadIdx.x >> 6];
+  uint64_t words = 0;   // state-array and list words passed (block 0's count)
+  // L_r, D_{r-1}, R_{r-1} and X_{r-1} are no longer read (cleared first: its
+  // memory traffic overlaps the reads below)
+  words += clear_state(S.St[(r + 1) & 1], S.swords, S.tq[(r + 1) & 1], S.qc[kQT + ((r + 3) & 3)],
+                       S.tq_cap);
+  // D_r's list counts (used when phase A listed the round)
+  const uint32_t* qd = S.qc + kQD + 3 * (r & 1);
+  const uint32_t nds = qd[0], ndm = qd[1], ndh = qd[2];
   // phase A's totals (every workgroup reads the same slots: uniform choices)
   if (threadIdx.x < 64) {
     unsigned long long v[5] = {0ull, 0ull, 0ull, 0ull, 0ull};
@@ -1019,7 +1027,6 @@ __device__ __forceinline__ void round_c(const LoopState& S, int r, LoopLds& lds)
   const int32_t nl = r == 0 ? S.ctl->nl0 : S.trace[r - 1].y;
   const bool listed = nl <= S.list_c;
   const bool listing = listed && !heavy && lds.tot[2] <= S.list_slots;
-  uint64_t words = 0;   // state-array and list words passed (block 0's count)
   if (blockIdx.x == 0) {
     // round r's counts into the trace; round r+1's sums start at zero
     if (threadIdx.x == 0) {
@@ -1038,9 +1045,6 @@ __device__ __forceinline__ void round_c(const LoopState& S, int r, LoopLds& lds)
     for (int i = threadIdx.x; i < kAcc * kAccW; i += kBlock)
       S.acc[(size_t)((r + 1) & 1) * kAcc * kAccW + i] = 0ull;
   }
-  // L_r, D_{r-1}, R_{r-1} and X_{r-1} are no longer read
-  words += clear_state(S.St[(r + 1) & 1], S.swords, S.tq[(r + 1) & 1], S.qc[kQT + ((r + 3) & 3)],
-                       S.tq_cap);
   if (!nonempty) {
     // D_r's leaving members
     words += clear_state(st, S.swords, S.tq[r & 1], listed ? S.qc[kQT + (r & 3)] : kQInvalid,
@@ -1055,8 +1059,6 @@ __device__ __forceinline__ void round_c(const LoopState& S, int r, LoopLds& lds)
     words += S.swords;
   } else {
   // D_r's lists when phase A made them (and none overflowed)
-  const uint32_t* qd = S.qc + kQD + 3 * (r & 1);
-  const uint32_t nds = qd[0], ndm = qd[1], ndh = qd[2];
   const bool dl = listed && nds <= S.dq_s_cap && ndm <= S.dq_m_cap && ndh <= S.dq_h_cap;
   const Lister tl = listing ? Lister{S.tq[r & 1], S.qc + kQT + (r & 3), S.tq_cap}
                             : Lister{nullptr, nullptr, 0u};
