@@ -776,9 +776,10 @@ __device__ __forceinline__ uint32_t count_slots(const LoopState& S, const DState
         if (to != tn) {
           atomicAdd(&S.K[y[k]], tn ? 1 : -1);
           // R_r only gains members in this phase: a flag seen set stays set.
-          // A vertex that had no flag goes on the touched list.
+          // A vertex that had no flag goes on the touched list (one that had
+          // a flag when gathered has one now: no returning atomic).
           if (tn && !(fl[k] & kFR)) {
-            if (tl.q)
+            if (tl.q && fl[k] == 0u)
               fresh[k] = st_set_first(st, y[k], kFR);
             else
               st_set(st, y[k], kFR);
