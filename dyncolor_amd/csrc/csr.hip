@@ -270,32 +270,62 @@ __global__ __launch_bounds__(kBlock) void k_check(GraphView G, const int32_t* co
 // count vertices / chunks that can need the hub path (rows with cap >=
 // kMidLen) and rows that keep a colour histogram (cap >= kSmallLen).  Launched
 // with a small grid: one atomic per wave and counter.
-__global__ void k_hub_capacity(int32_t n, const uint32_t* cap, uint32_t* out) {
+// Capacity classes over all rows: {hub rows, their kChunk-slot chunks, rows
+// with a histogram}.  Four loads in flight per lane, one atomic per
+// workgroup and counter.
+__global__ __launch_bounds__(kBlock) void k_hub_capacity(int32_t n, const uint32_t* cap,
+                                                         uint32_t* out) {
+  __shared__ uint32_t part[kBlock / 64][3];
   uint32_t hubs = 0, chunks = 0, hrows = 0;
-  for (int32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
-    uint32_t c = cap[v];
-    if (c >= kMidLen) {
-      hubs += 1;
-      chunks += (c + kChunk - 1) / kChunk;
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t v0 = blockIdx.x * blockDim.x + threadIdx.x; v0 < (uint32_t)n; v0 += 4 * stride) {
+    uint32_t c[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) c[k] = v0 + k * stride < (uint32_t)n ? cap[v0 + k * stride] : 0u;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      if (c[k] >= kMidLen) {
+        hubs += 1;
+        chunks += (c[k] + kChunk - 1) / kChunk;
+      }
+      hrows += c[k] >= kSmallLen ? 1u : 0u;
     }
-    hrows += c >= kSmallLen ? 1u : 0u;
   }
-  wave_add_u32(hubs, &out[0]);
-  wave_add_u32(chunks, &out[1]);
-  wave_add_u32(hrows, &out[2]);
+  for (int off = 32; off > 0; off >>= 1) {
+    hubs += __shfl_xor(hubs, off);
+    chunks += __shfl_xor(chunks, off);
+    hrows += __shfl_xor(hrows, off);
+  }
+  if (lane_id() == 0) {
+    part[threadIdx.x >> 6][0] = hubs;
+    part[threadIdx.x >> 6][1] = chunks;
+    part[threadIdx.x >> 6][2] = hrows;
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    uint32_t t = 0;
+    for (int q = 0; q < kBlock / 64; ++q) t += part[q][threadIdx.x];
+    if (t) atomicAdd(&out[threadIdx.x], t);
+  }
 }
 constexpr int kCountGrid = 256;
 
-// (the staged batch's counters in g->ctr must survive this: it runs inside
-// the apply when a row overflows)
-static int ensure_hub_buffers(Graph* g) {
+// h = {hub rows, hub chunks, histogram rows} from the current capacities
+static int hub_counts(Graph* g, uint32_t h[3]) {
   uint32_t* tmp = g->aux;
   DC_HIP(hipMemsetAsync(tmp, 0, sizeof(uint32_t) * 3, g->stream));
   hipLaunchKernelGGL(k_hub_capacity, dim3(kCountGrid), dim3(kBlock), 0, g->stream, g->n, g->cap,
                      tmp);
-  uint32_t h[2];
-  DC_HIP(hipMemcpyAsync(h, tmp, sizeof(h), hipMemcpyDeviceToHost, g->stream));
+  DC_HIP(hipGetLastError());
+  DC_HIP(hipMemcpyAsync(h, tmp, sizeof(uint32_t) * 3, hipMemcpyDeviceToHost, g->stream));
   DC_HIP(hipStreamSynchronize(g->stream));
+  return DC_OK;
+}
+
+// frontier hub buffers for h (from hub_counts)
+// (the staged batch's counters in g->ctr must survive this: it runs inside
+// the apply when a row overflows)
+static int ensure_hub_buffers(Graph* g, const uint32_t h[3]) {
   uint32_t need_hubs = h[0] + 1, need_chunks = h[1] + 1;
   if (need_hubs > g->hub_cap) g->hub_cap = need_hubs + need_hubs / 4 + 16;
   if (need_chunks > g->hchunk_cap) {
@@ -442,14 +472,10 @@ static int hist_fill(Graph* g) {
   return DC_OK;
 }
 
-int hist_reindex(Graph* g) {
+// histogram rows, the hub chunk table and the histograms for h (from
+// hub_counts)
+static int hist_reindex(Graph* g, const uint32_t h[3]) {
   uint32_t* tmp = g->aux;
-  DC_HIP(hipMemsetAsync(tmp, 0, sizeof(uint32_t) * 4, g->stream));
-  hipLaunchKernelGGL(k_hub_capacity, dim3(kCountGrid), dim3(kBlock), 0, g->stream, g->n, g->cap,
-                     tmp);
-  uint32_t h[3];
-  DC_HIP(hipMemcpyAsync(h, tmp, sizeof(h), hipMemcpyDeviceToHost, g->stream));
-  DC_HIP(hipStreamSynchronize(g->stream));
   const uint32_t nh = h[2];
   if (nh > g->hist_cap) {
     if (g->hist) DC_HIP(hipFree(g->hist));
@@ -472,6 +498,14 @@ int hist_reindex(Graph* g) {
   return hist_fill(g);
 }
 
+int hist_reindex(Graph* g) {
+  uint32_t h[3];
+  int rc = hub_counts(g, h);
+  if (rc) return rc;
+  if ((rc = hist_reindex(g, h))) return rc;
+  return ensure_hub_buffers(g, h);
+}
+
 // the hub chunk table from the current capacities (chunks = an upper bound
 // of the entries, from k_hub_capacity)
 static int hub_chunk_table(Graph* g, uint32_t chunks) {
@@ -491,17 +525,6 @@ static int hub_chunk_table(Graph* g, uint32_t chunks) {
   DC_HIP(hipMemcpyAsync(&g->n_hch, tmp + 3, sizeof(uint32_t), hipMemcpyDeviceToHost, g->stream));
   DC_HIP(hipStreamSynchronize(g->stream));
   return DC_OK;
-}
-
-static int hub_chunk_table(Graph* g) {
-  uint32_t* tmp = g->aux;
-  DC_HIP(hipMemsetAsync(tmp, 0, sizeof(uint32_t) * 3, g->stream));
-  hipLaunchKernelGGL(k_hub_capacity, dim3(kCountGrid), dim3(kBlock), 0, g->stream, g->n, g->cap,
-                     tmp);
-  uint32_t h[3];
-  DC_HIP(hipMemcpyAsync(h, tmp, sizeof(h), hipMemcpyDeviceToHost, g->stream));
-  DC_HIP(hipStreamSynchronize(g->stream));
-  return hub_chunk_table(g, h[1]);
 }
 
 // tests: histogram entries that differ from a recount of the hub's row
@@ -623,8 +646,7 @@ int csr_build(Graph* g, int64_t m, const int32_t* du, const int32_t* dv) {
     hipLaunchKernelGGL(k_fill, dim3(grid_for(m)), dim3(kBlock), 0, g->stream, m, du, dv, g->view());
   DC_HIP(hipGetLastError());
   if ((rc = rebuild_ccol(g))) return rc;
-  if ((rc = hist_reindex(g))) return rc;
-  return ensure_hub_buffers(g);
+  return hist_reindex(g);
 }
 
 // ------------------------------------------------------------- apply -------
@@ -865,7 +887,6 @@ __global__ void k_relocate(const int4* ovf, uint32_t novf, GraphView G, uint32_t
   }
 }
 
-static int hub_chunk_table(Graph* g);
 static int regrow(Graph* g, uint32_t novf);
 
 // Make room for a batch's overflowing insertions: relocate the rows into the
@@ -894,12 +915,14 @@ static int make_room(Graph* g, uint32_t novf, bool* relaid) {
   // capacities grew: frontier hub buffers and the hub chunk table follow; a
   // row past kSmallLen needs a histogram (the existing ones already count
   // the batch)
+  uint32_t h[3];
+  if ((rc = hub_counts(g, h))) return rc;
   if (hf[1]) {
-    if ((rc = hist_reindex(g))) return rc;
-  } else if ((rc = hub_chunk_table(g))) {
+    if ((rc = hist_reindex(g, h))) return rc;
+  } else if ((rc = hub_chunk_table(g, h[1]))) {
     return rc;
   }
-  return ensure_hub_buffers(g);
+  return ensure_hub_buffers(g, h);
 }
 
 // Give every row that over-ran its capacity new room (used + slack), relay
@@ -936,8 +959,7 @@ static int regrow(Graph* g, uint32_t novf) {
                      g->view(), g->color[g->cur]);
   DC_HIP(hipGetLastError());
   // capacities changed: hub rows and their histograms are recomputed
-  if ((rc = hist_reindex(g))) return rc;
-  return ensure_hub_buffers(g);
+  return hist_reindex(g);
 }
 
 static int ensure_lists(Graph* g, int64_t k) {
