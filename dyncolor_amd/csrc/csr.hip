@@ -450,14 +450,37 @@ __global__ __launch_bounds__(kBlock) void k_hist_fill(GraphView G, const int32_t
 }
 
 // the static hub chunk table: one entry per kChunk slots of a hub's capacity
-__global__ void k_hub_chunks(const int32_t* hrowv, uint32_t n_hrows, const uint32_t* cap, int2* hch,
-                             uint32_t* counter) {
-  for (uint32_t h = blockIdx.x * blockDim.x + threadIdx.x; h < n_hrows; h += gridDim.x * blockDim.x) {
-    const int32_t v = hrowv[h];
-    const uint32_t nch = cap[v] >= kMidLen ? (cap[v] + kChunk - 1) / kChunk : 0u;
-    if (nch == 0) continue;
-    const uint32_t base = atomicAdd(counter, nch);
+// (a workgroup reserves its entries with one atomic: a prefix sum of its
+// threads' chunk counts)
+__global__ __launch_bounds__(kBlock) void k_hub_chunks(const int32_t* hrowv, uint32_t n_hrows,
+                                                       const uint32_t* cap, int2* hch,
+                                                       uint32_t* counter) {
+  __shared__ uint32_t wsum[kBlock / 64];
+  __shared__ uint32_t bbase;
+  const int lane = lane_id();
+  const int wib = threadIdx.x >> 6;
+  for (uint32_t b0 = blockIdx.x * blockDim.x; b0 < n_hrows; b0 += gridDim.x * blockDim.x) {
+    const uint32_t h = b0 + threadIdx.x;
+    const int32_t v = h < n_hrows ? hrowv[h] : 0;
+    const uint32_t cv = h < n_hrows ? cap[v] : 0u;
+    const uint32_t nch = cv >= kMidLen ? (cv + kChunk - 1) / kChunk : 0u;
+    uint32_t x = nch;   // inclusive scan over the wave
+    for (int off = 1; off < 64; off <<= 1) {
+      const uint32_t y = __shfl_up(x, off);
+      if (lane >= off) x += y;
+    }
+    if (lane == 63) wsum[wib] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t t = 0;
+      for (int q = 0; q < kBlock / 64; ++q) t += wsum[q];
+      bbase = t ? atomicAdd(counter, t) : 0u;
+    }
+    __syncthreads();
+    uint32_t base = bbase + x - nch;
+    for (int q = 0; q < wib; ++q) base += wsum[q];
     for (uint32_t c = 0; c < nch; ++c) hch[base + c] = make_int2(v, (int)c);
+    __syncthreads();
   }
 }
 
