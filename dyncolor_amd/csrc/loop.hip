@@ -79,8 +79,8 @@ constexpr int kWaveList = 64 * 16; // vertices of 128 state words (16 per lane)
 // smaller of kListC and 1/64 of the words that the scans pass: below that an
 // append's atomic costs less than the scan words it saves), and R too when
 // D's rows hold at most kListSlots slots (and the round is not a pull round).
-constexpr int32_t kListC = 1 << 14;
-constexpr unsigned long long kListSlots = 1ull << 20;
+constexpr int32_t kListC = 1 << 17;
+constexpr unsigned long long kListSlots = 1ull << 22;
 constexpr uint32_t kQInvalid = 0xffffffffu;  // touched count of a round that did not list
 constexpr int kQT = 0;      // qc[kQT + (r & 3)]: entries of round r's touched list
 constexpr int kQD = 4;      // qc[kQD + 3 * (r & 1) + k]: D_r's short rows, mid rows, hub chunks

@@ -67,7 +67,7 @@ typedef struct {
   int32_t list_limit;  /* independent-rule loop rounds with at most this many vertices
                           in L take their sets from worklists instead of scanning the
                           per-vertex state arrays; 0 (default): automatic (at most
-                          16384, and fewer on small graphs); -1: never; > 0 also lists
+                          131072, and fewer on small graphs); -1: never; > 0 also lists
                           R in every round that is not a pull round.  Results do not
                           depend on it.                                             */
 } dc_options;
