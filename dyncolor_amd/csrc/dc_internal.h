@@ -223,7 +223,7 @@ struct LoopResult {
 };
 int loop_hi(Graph* g, Frontier L0, int32_t nl0, bool persistent, LoopResult* res);
 void loop_free(Graph* g);
-int loop_check(Graph* g, int64_t* bad);   // candidate counts / bitmaps clean at rest
+int loop_check(Graph* g, int64_t* bad);   // candidate counts / state arrays clean at rest
 // fused.hip
 int recolor_fused(Graph* g, int rule, dc_stats* st);
 void fused_free(Graph* g);
