@@ -413,6 +413,10 @@ void free_state(Graph* g) {
   f(g->ctr); f(g->aux); f(g->tail); f(g->scan_tmp);
   f(g->scratch_batch.u); f(g->scratch_batch.v); f(g->scratch_batch.ins);
   if (g->hctr) (void)hipHostFree(g->hctr);
+  if (g->pre_host) (void)hipHostFree(g->pre_host);
+  f(g->pre_dev);
+  g->pre_dev = nullptr;
+  g->pre_host = nullptr;
   for (int i = 0; i < 4; ++i)
     if (g->ev[i]) (void)hipEventDestroy(g->ev[i]);
 }
@@ -1016,6 +1020,10 @@ int csr_apply(Graph* g, const Batch& b, int rule) {
   DC_HIP(hipMemcpyAsync(g->hctr, g->ctr, sizeof(Counters), hipMemcpyDeviceToHost, g->stream));
   DC_HIP(hipStreamSynchronize(g->stream));
   Counters c = *g->hctr;
+  g->staged_n[0] = c.n_del;
+  g->staged_n[1] = c.n_ins;
+  g->staged_n[2] = c.n_xdel;
+  g->staged_n[3] = c.n_xins;
   g->staged_regrows = 0;
   g->staged_moves = 0;
   if (c.overflow > 0) {

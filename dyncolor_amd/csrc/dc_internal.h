@@ -152,6 +152,13 @@ struct Graph {
 
   Counters* ctr = nullptr;    // device
   Counters* hctr = nullptr;   // pinned host mirror
+  // the staged batch's deletion, insertion, cross-deletion and cross-insertion
+  // line counts (host copy from the apply)
+  uint32_t staged_n[4] = {0u, 0u, 0u, 0u};
+  // pre-loop rounds' results, gathered on the device and read back once:
+  // kPreSlots x {frontier counts, Counters} (device and pinned host)
+  void* pre_dev = nullptr;
+  void* pre_host = nullptr;
   uint32_t* aux = nullptr;    // small device scratch (hub capacity count)
 
   // upload staging for host-array batches

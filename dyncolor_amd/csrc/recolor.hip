@@ -282,11 +282,27 @@ int recolor_plain(Graph* g, int rule, dc_stats* st, bool persistent_loop) {
   if (rc) return rc;
   DC_HIP(hipEventRecord(g->ev[0], g->stream));
 
-  // list sizes staged by the apply
+  // list sizes staged by the apply (kept on the host by it)
   RoundReadback rb;
-  DC_HIP(hipMemcpyAsync(g->hctr, g->ctr, sizeof(Counters), hipMemcpyDeviceToHost, g->stream));
-  DC_HIP(hipStreamSynchronize(g->stream));
-  const Counters staged = *g->hctr;
+  Counters staged{};
+  staged.n_del = g->staged_n[0];
+  staged.n_ins = g->staged_n[1];
+  staged.n_xdel = g->staged_n[2];
+  staged.n_xins = g->staged_n[3];
+  // pre-loop rounds' results: copied on the device after each round, read
+  // back together before the loop (one host wait instead of one per round)
+  constexpr int kPreSlots = 5;   // up to four target rounds, then the marked set's counts
+  if (!g->pre_dev) {
+    DC_HIP(hipMalloc(&g->pre_dev, sizeof(RoundReadback) * kPreSlots));
+    DC_HIP(hipHostMalloc(&g->pre_host, sizeof(RoundReadback) * kPreSlots, hipHostMallocDefault));
+  }
+  RoundReadback* pre_d = static_cast<RoundReadback*>(g->pre_dev);
+  RoundReadback* pre_h = static_cast<RoundReadback*>(g->pre_host);
+  struct PreRound {
+    int kind, mode;
+  };
+  PreRound pre[kPreSlots - 1];
+  int npre = 0;
   uint32_t* dctr_u32 = reinterpret_cast<uint32_t*>(g->ctr);
   const uint32_t* d_ndel = dctr_u32 + offsetof(Counters, n_del) / 4;
   const uint32_t* d_nins = dctr_u32 + offsetof(Counters, n_ins) / 4;
@@ -325,23 +341,15 @@ int recolor_plain(Graph* g, int rule, dc_stats* st, bool persistent_loop) {
     RecolorArgs A = make_rargs(g, g->T, Mk, mepoch);
     if ((r2 = launch_recolor(g, mode, A, nullptr))) return r2;
     ++launches;
-    if ((r2 = readback(g, g->T, &rb))) return r2;
-    uint64_t nt = fsize(rb.cnt);
-    g->trace.insert(g->trace.end(), {(int64_t)kind, (int64_t)nt, (int64_t)rb.c.changed, 0});
-    evaluated += nt;
-    changed_total += rb.c.changed;
-    slots_total += rb.c.slots + rb.c.rslots;
-    live_total += rb.c.live + rb.c.rlive;
-    account_recolor(nt, rb.c);
+    DC_HIP(hipMemcpyAsync(pre_d[npre].cnt, g->T.cnt, sizeof(uint32_t) * 4, hipMemcpyDeviceToDevice,
+                          g->stream));
+    DC_HIP(hipMemcpyAsync(&pre_d[npre].c, g->ctr, sizeof(Counters), hipMemcpyDeviceToDevice,
+                          g->stream));
+    pre[npre++] = PreRound{kind, mode};
     g->cur ^= 1;
     hipLaunchKernelGGL(k_catchup, dim3(grid_threads(g->n_owned)), blk, 0, g->stream, Mk, g->view(),
                        g->color[g->cur], g->color[g->cur ^ 1]);
     DC_HIP(hipGetLastError());
-    if (mode == M_INS) {
-      if (nt > 0) change = true;
-    } else if (rb.c.changed > 0) {
-      change = true;
-    }
     return DC_OK;
   };
 
@@ -401,9 +409,28 @@ int recolor_plain(Graph* g, int rule, dc_stats* st, bool persistent_loop) {
   // recolor loop: L = marked set, then L <- C every round
   Frontier L = Mk;
   DC_HIP(hipMemsetAsync(g->cbits[0], 0, sizeof(uint32_t) * 2 * g->cbits_words, g->stream));
-  DC_HIP(hipMemcpyAsync(rb.cnt, Mk.cnt, sizeof(uint32_t) * 4, hipMemcpyDeviceToHost, g->stream));
+  // the pre-loop rounds' results and the marked set's size in one read-back
+  DC_HIP(hipMemcpyAsync(pre_d[kPreSlots - 1].cnt, Mk.cnt, sizeof(uint32_t) * 4,
+                        hipMemcpyDeviceToDevice, g->stream));
+  DC_HIP(hipMemcpyAsync(pre_h, pre_d, sizeof(RoundReadback) * kPreSlots, hipMemcpyDeviceToHost,
+                        g->stream));
   DC_HIP(hipStreamSynchronize(g->stream));
-  uint64_t nL = fsize(rb.cnt);
+  for (int i = 0; i < npre; ++i) {
+    const RoundReadback& p = pre_h[i];
+    const uint64_t nt = fsize(p.cnt);
+    g->trace.insert(g->trace.end(), {(int64_t)pre[i].kind, (int64_t)nt, (int64_t)p.c.changed, 0});
+    evaluated += nt;
+    changed_total += p.c.changed;
+    slots_total += p.c.slots + p.c.rslots;
+    live_total += p.c.live + p.c.rlive;
+    account_recolor(nt, p.c);
+    if (pre[i].mode == M_INS) {
+      if (nt > 0) change = true;
+    } else if (p.c.changed > 0) {
+      change = true;
+    }
+  }
+  uint64_t nL = fsize(pre_h[kPreSlots - 1].cnt);
   if (rule == DC_RULE_INDEPENDENT && change) {
     // candidate counts kept incrementally (loop.hip); both colour buffers
     // are written there, so no catch-up follows
