@@ -70,6 +70,38 @@ __global__ void k_catchup(Frontier F, GraphView G, const int32_t* cur, int32_t* 
   catchup_body(F, G, cur, nxt);
 }
 
+// Start of a pre-loop round: the target frontier's and the changed list's
+// counts, the round's counters and (insertion rounds) the colour maximum
+// start at zero, in one launch.
+__global__ void k_round_reset(uint32_t* t_cnt, uint32_t* x_cnt, Counters* ctr, int32_t* cmax) {
+  const int i = threadIdx.x;
+  if (i < 4) {
+    t_cnt[i] = 0u;
+    x_cnt[i] = 0u;
+  }
+  if (i == 0) {
+    ctr->changed = 0u;
+    ctr->targets = 0u;
+    ctr->slots = 0ull;
+    ctr->live = 0ull;
+    ctr->rslots = 0ull;
+    ctr->rlive = 0ull;
+    if (cmax) *cmax = 0;
+  }
+}
+
+// End of a pre-loop round: the marked set's colours into the other buffer,
+// and the round's frontier counts and counters into its read-back slot.
+__global__ void k_catchup_save(Frontier F, GraphView G, const int32_t* cur, int32_t* nxt,
+                               const uint32_t* t_cnt, const Counters* ctr, uint32_t* s_cnt,
+                               Counters* s_c) {
+  catchup_body(F, G, cur, nxt);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    for (int k = 0; k < 4; ++k) s_cnt[k] = t_cnt[k];
+    *s_c = *ctr;
+  }
+}
+
 // ----------------------------------------------------------------------------
 // static coloring: pending[v] = number of predecessors (neighbours that come
 // first in the greedy order); vertices with none form the first ready set.
@@ -218,10 +250,11 @@ static void launch_pair(Graph* g, dim3 grid, const RecolorArgs& A) {
     hipLaunchKernelGGL(k_propagate, grid, dim3(kBlock), 0, g->stream, A.G, A.chx, A.nxt, A.cur);
 }
 
-static int launch_recolor(Graph* g, int mode, const RecolorArgs& A, float* ms) {
+static int launch_recolor(Graph* g, int mode, const RecolorArgs& A, float* ms,
+                          bool clear_chx = true) {
   (void)ms;
   dim3 grid(frontier_grid(g));
-  DC_HIP(hipMemsetAsync(A.chx.cnt, 0, sizeof(uint32_t) * 4, g->stream));
+  if (clear_chx) DC_HIP(hipMemsetAsync(A.chx.cnt, 0, sizeof(uint32_t) * 4, g->stream));
   int prc = prof_begin(g);
   if (prc) return prc;
   switch (mode) {
@@ -334,29 +367,27 @@ int recolor_plain(Graph* g, int rule, dc_stats* st, bool persistent_loop) {
 
   auto run_targets_round = [&](int kind, int mode, auto launch_targets) -> int {
     int r2;
-    if ((r2 = zero_frontier(g, g->T))) return r2;
-    if ((r2 = reset_round_ctr())) return r2;
+    RecolorArgs A = make_rargs(g, g->T, Mk, mepoch);
+    hipLaunchKernelGGL(k_round_reset, dim3(1), dim3(64), 0, g->stream, g->T.cnt, A.chx.cnt, g->ctr,
+                       mode == M_INS ? &g->ctr->cmax : nullptr);
     launch_targets(next_epoch(g));
     DC_HIP(hipGetLastError());
-    RecolorArgs A = make_rargs(g, g->T, Mk, mepoch);
-    if ((r2 = launch_recolor(g, mode, A, nullptr))) return r2;
+    if ((r2 = launch_recolor(g, mode, A, nullptr, false))) return r2;
     ++launches;
-    DC_HIP(hipMemcpyAsync(pre_d[npre].cnt, g->T.cnt, sizeof(uint32_t) * 4, hipMemcpyDeviceToDevice,
-                          g->stream));
-    DC_HIP(hipMemcpyAsync(&pre_d[npre].c, g->ctr, sizeof(Counters), hipMemcpyDeviceToDevice,
-                          g->stream));
-    pre[npre++] = PreRound{kind, mode};
+    pre[npre] = PreRound{kind, mode};
     g->cur ^= 1;
-    hipLaunchKernelGGL(k_catchup, dim3(grid_threads(g->n_owned)), blk, 0, g->stream, Mk, g->view(),
-                       g->color[g->cur], g->color[g->cur ^ 1]);
+    hipLaunchKernelGGL(k_catchup_save, dim3(grid_threads(g->n_owned)), blk, 0, g->stream, Mk,
+                       g->view(), g->color[g->cur], g->color[g->cur ^ 1], g->T.cnt, g->ctr,
+                       pre_d[npre].cnt, &pre_d[npre].c);
     DC_HIP(hipGetLastError());
+    ++npre;
     return DC_OK;
   };
 
   const int fg = frontier_grid(g);
   // highest current colour, for the insertion targets' histogram test
+  // (cmax was zeroed by the round's reset)
   auto launch_colour_max = [&]() {
-    (void)hipMemsetAsync(&g->ctr->cmax, 0, sizeof(int32_t), g->stream);
     hipLaunchKernelGGL(k_colour_max, dim3(grid_threads(g->n) < 256 ? grid_threads(g->n) : 256),
                        blk, 0, g->stream, g->color[g->cur], g->n, &g->ctr->cmax);
   };
