@@ -91,6 +91,11 @@ struct LoopCtl {
   int32_t nl0;          // |L_0|
   int32_t last_nc;      // |C| of the latest round
   int32_t last_changed; // colour changes of the latest round
+  // settle checks queued on the device (plain driver)
+  int32_t checks_left;  // checks still allowed
+  int32_t unsettled;    // a check found a vertex off its mex
+  int32_t settled;      // the loop stopped at settled colours (done_round = last round run)
+  int32_t pad;
 };
 
 struct LoopState {
@@ -1194,7 +1199,7 @@ __global__ __launch_bounds__(kBlock) void k_loop_persistent(LoopState S, int r0,
 // is; a stale one is recomputed (histogram or short-row scan, by 8-lane
 // groups) and cached.  A mex past the histogram counts as a mismatch: the
 // check only errs towards "not settled".  *unsettled is set, never cleared.
-__global__ __launch_bounds__(kBlock) void k_settle_check(LoopState S, int32_t* unsettled) {
+__device__ __forceinline__ void settle_check_body(const LoopState& S, int32_t* unsettled) {
   __shared__ int32_t stale_list[kBlock / 64][64];
   const GraphView& G = S.G;
   const int lane = lane_id();
@@ -1233,6 +1238,35 @@ __global__ __launch_bounds__(kBlock) void k_settle_check(LoopState S, int32_t* u
     }
   }
   if (__ballot(bad) != 0ull && lane == 0) *unsettled = 1;
+}
+
+__global__ __launch_bounds__(kBlock) void k_settle_check(LoopState S, int32_t* unsettled) {
+  settle_check_body(S, unsettled);
+}
+
+// Settle checks queued in the plain driver's stream after a round: the check
+// runs only if the loop is still going, that round changed no colour and
+// checks are left (every workgroup reads the same control words, which no
+// kernel changes between the check and its decision); k_settle_decide then
+// takes its result: settled colours end the loop after that round.
+__device__ __forceinline__ bool settle_check_due(const LoopCtl* c) {
+  return c->done_round == INT_MAX && c->last_changed == 0 && c->checks_left > 0;
+}
+
+__global__ __launch_bounds__(kBlock) void k_settle_check_if(LoopState S) {
+  if (!settle_check_due(S.ctl)) return;
+  settle_check_body(S, &S.ctl->unsettled);
+}
+
+__global__ void k_settle_decide(LoopState S, int32_t r) {
+  LoopCtl* c = S.ctl;
+  if (threadIdx.x != 0 || !settle_check_due(c)) return;
+  c->checks_left -= 1;
+  if (c->unsettled == 0) {
+    c->done_round = r;
+    c->settled = 1;
+  }
+  c->unsettled = 0;
 }
 
 // K and the loop's state arrays are zero between batches.
@@ -1365,6 +1399,18 @@ static int32_t block_end(int32_t r, bool persistent) {
   return persistent ? e : std::min(e, r + 32);
 }
 
+// Host blocks of the plain driver, whose settle checks are queued on the
+// device between rounds: 0-2, 3-8, 9-12, then blocks ending at 2^k - 3 (29,
+// 61, ...), at most 32 rounds long.
+static int32_t block_end_plain(int32_t r) {
+  if (r < 3) return 3;
+  if (r < 9) return 9;
+  if (r < 13) return 13;
+  int32_t e = 29;
+  while (e <= r) e = 2 * e + 3;
+  return std::min(e, r + 32);
+}
+
 // rounds after which a settle check may run (block ends of both drivers)
 static bool settle_point(int32_t r) {
   return (r >= 3 && r <= 8) || r == 12 || (r >= 28 && ((r + 4) & (r + 3)) == 0);
@@ -1421,6 +1467,7 @@ int loop_hi(Graph* g, Frontier L0, int32_t nl0, bool persistent, LoopResult* res
   *s->hctl = LoopCtl{};
   s->hctl->done_round = INT_MAX;
   s->hctl->nl0 = nl0;
+  s->hctl->checks_left = g->opt.exact_rounds ? 0 : 3;
   DC_HIP(hipMemcpyAsync(S.ctl, s->hctl, sizeof(LoopCtl), hipMemcpyHostToDevice, g->stream));
   DC_HIP(hipMemsetAsync(S.bstat, 0, sizeof(unsigned long long) * (4 * kMaxBlocks + 1), g->stream));
   DC_HIP(hipMemsetAsync(S.acc, 0, sizeof(unsigned long long) * 2 * kAcc * kAccW, g->stream));
@@ -1451,7 +1498,8 @@ int loop_hi(Graph* g, Frontier L0, int32_t nl0, bool persistent, LoopResult* res
   bool settled = false;
   int32_t rounds = -1, r = 0;
   while (r < max_rounds && rounds < 0) {
-    const int32_t nb = std::min(block_end(r, persistent), max_rounds) - r;
+    const int32_t nb =
+        std::min(persistent ? block_end(r, true) : block_end_plain(r), max_rounds) - r;
     if ((rc = inc_trace(g, s, r + nb))) return rc;
     if (g->profiling) {
       while (s->cev.size() < 2 * (size_t)nb) {
@@ -1473,6 +1521,11 @@ int loop_hi(Graph* g, Frontier L0, int32_t nl0, bool persistent, LoopResult* res
         if (g->profiling) DC_HIP(hipEventRecord(s->cev[2 * i], g->stream));
         hipLaunchKernelGGL(k_round_c, dim3(round_grid), blk, 0, g->stream, S, r + i);
         if (g->profiling) DC_HIP(hipEventRecord(s->cev[2 * i + 1], g->stream));
+        if (settle_point(r + i + 1) && !g->opt.exact_rounds) {
+          hipLaunchKernelGGL(k_settle_check_if, dim3(kMaxBlocks), blk, 0, g->stream, S);
+          hipLaunchKernelGGL(k_settle_decide, dim3(1), dim3(64), 0, g->stream, S, r + i);
+          launches += 2;
+        }
       }
       DC_HIP(hipGetLastError());
       launches += 2 * nb;
@@ -1497,7 +1550,12 @@ int loop_hi(Graph* g, Frontier L0, int32_t nl0, bool persistent, LoopResult* res
     r += nb;
     if (done != INT_MAX) {
       rounds = done + 1;
-    } else if (settle_left > 0 && settle_point(r) && s->hctl->last_changed == 0) {
+      if (s->hctl->settled) {
+        // a queued settle check ended the loop after round `done`
+        settled = true;
+        if ((rc = loop_reset(g, s))) return rc;
+      }
+    } else if (persistent && settle_left > 0 && settle_point(r) && s->hctl->last_changed == 0) {
       // rounds 0..r-1 ran in full; the last one changed nothing
       --settle_left;
       int32_t* flag = reinterpret_cast<int32_t*>(g->aux);
