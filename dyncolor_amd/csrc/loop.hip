@@ -1385,18 +1385,18 @@ static int persistent_grid(Graph* g, int* grid) {
   return DC_OK;
 }
 
-// Host blocks of rounds (the host reads the control word after each): rounds
-// 0-2, then one at a time up to 8 (a batch's changes usually end by round 4,
-// and the settle check wants the first round without a change), 8-11, then
-// blocks ending at 2^k - 4 (28, 60, 124, ...), the plain driver's at most 32
-// rounds long.
-static int32_t block_end(int32_t r, bool persistent) {
+// Host blocks of rounds of the persistent driver (the host reads the control
+// word after each and may run a settle check): rounds 0-2, then one at a time
+// up to 8 (a batch's changes usually end by round 4, and the settle check
+// wants the first round without a change), 8-11, then blocks ending at
+// 2^k - 4 (28, 60, 124, ...).
+static int32_t block_end_persistent(int32_t r) {
   if (r < 3) return 3;
   if (r < 8) return r + 1;
   if (r < 12) return 12;
   int32_t e = 28;
   while (e <= r) e = 2 * e + 4;
-  return persistent ? e : std::min(e, r + 32);
+  return e;
 }
 
 // Host blocks of the plain driver, whose settle checks are queued on the
@@ -1411,7 +1411,8 @@ static int32_t block_end_plain(int32_t r) {
   return std::min(e, r + 32);
 }
 
-// rounds after which a settle check may run (block ends of both drivers)
+// rounds after which a settle check may run (the persistent driver's block
+// ends; the plain driver queues its checks after the same rounds)
 static bool settle_point(int32_t r) {
   return (r >= 3 && r <= 8) || r == 12 || (r >= 28 && ((r + 4) & (r + 3)) == 0);
 }
@@ -1499,7 +1500,7 @@ int loop_hi(Graph* g, Frontier L0, int32_t nl0, bool persistent, LoopResult* res
   int32_t rounds = -1, r = 0;
   while (r < max_rounds && rounds < 0) {
     const int32_t nb =
-        std::min(persistent ? block_end(r, true) : block_end_plain(r), max_rounds) - r;
+        std::min(persistent ? block_end_persistent(r) : block_end_plain(r), max_rounds) - r;
     if ((rc = inc_trace(g, s, r + nb))) return rc;
     if (g->profiling) {
       while (s->cev.size() < 2 * (size_t)nb) {
