@@ -769,12 +769,15 @@ __global__ void k_apply(int64_t k, const int32_t* bu, const int32_t* bv, const u
 // (deleteEdgeFromAdj, HI/gpuFunctions_undir.cuh:118-142, scans both rows; the
 // twin links make one scan of min(deg) slots enough).  A ghost endpoint has
 // no row: the owned row is scanned.
-// One 16-lane group per line: most scanned rows are short, and a wave per
-// line would leave most lanes idle and keep one line's chain of dependent
-// loads in flight instead of four (on R-MAT 22 with 50 000-line deletion
-// batches: 64 lanes 0.88 ms per batch's staging, 32: 0.62, 16: 0.55, 8: 0.64,
-// 4: 0.96).
-constexpr int kTombLanes = 16;
+// One 8-lane group per line, 16 slot loads in flight per lane: most scanned
+// rows are short, and a wave per line would leave most lanes idle and keep
+// one line's chain of dependent loads in flight instead of eight.  Measured
+// (lanes x loads per lane): R-MAT 22, 50 000-line deletion batches, staging
+// per batch 64x8 0.88 ms, 16x8 0.46, 8x16 0.47, 8x8 0.54, 4x16 0.71; the
+// 512-graph arena, 8.7 M deletion lines per step, step time 16x8 150 ms,
+// 8x8 137, 8x16 137, 4x16 130.
+constexpr int kTombLanes = 8;
+constexpr int kTombUnroll = 16;
 __global__ void k_tombstone(const int2* del, uint32_t ndel, const int2* xdel, uint32_t nxdel,
                             GraphView G, Counters* ctr) {
   const int sub = lane_id() & (kTombLanes - 1);
@@ -803,16 +806,16 @@ __global__ void k_tombstone(const int2* del, uint32_t ndel, const int2* xdel, ui
     // (the same edge can come twice, in either orientation), which also
     // takes its colour out of the row's histogram.
     const int32_t hx = G.hidx[x], hy = cross ? -1 : G.hidx[y];
-    for (uint32_t j0 = sub; j0 < r.y; j0 += 8 * kTombLanes) {
-      // 8 loads in flight per lane; matches are rare
-      int32_t cv[8];
+    for (uint32_t j0 = sub; j0 < r.y; j0 += kTombUnroll * kTombLanes) {
+      // kTombUnroll loads in flight per lane; matches are rare
+      int32_t cv[kTombUnroll];
 #pragma unroll
-      for (int k = 0; k < 8; ++k) {
+      for (int k = 0; k < kTombUnroll; ++k) {
         const uint32_t jj = j0 + k * kTombLanes;
         cv[k] = jj < r.y ? G.col[(uint64_t)r.x + jj] : -2;
       }
 #pragma unroll
-      for (int k = 0; k < 8; ++k) {
+      for (int k = 0; k < kTombUnroll; ++k) {
         if (cv[k] != y) continue;
         const uint64_t s = (uint64_t)r.x + j0 + k * kTombLanes;
         if (atomicExch(&G.col[s], -1) == y) {
