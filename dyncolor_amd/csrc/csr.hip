@@ -769,15 +769,16 @@ __global__ void k_apply(int64_t k, const int32_t* bu, const int32_t* bv, const u
 // (deleteEdgeFromAdj, HI/gpuFunctions_undir.cuh:118-142, scans both rows; the
 // twin links make one scan of min(deg) slots enough).  A ghost endpoint has
 // no row: the owned row is scanned.
-// One 8-lane group per line, 16 slot loads in flight per lane: most scanned
-// rows are short, and a wave per line would leave most lanes idle and keep
-// one line's chain of dependent loads in flight instead of eight.  Measured
-// (lanes x loads per lane): R-MAT 22, 50 000-line deletion batches, staging
-// per batch 64x8 0.88 ms, 16x8 0.46, 8x16 0.47, 8x8 0.54, 4x16 0.71; the
-// 512-graph arena, 8.7 M deletion lines per step, step time 16x8 150 ms,
-// 8x8 137, 8x16 137, 4x16 130.
-constexpr int kTombLanes = 8;
-constexpr int kTombUnroll = 16;
+// One kTombLanes-lane group per line, kTombUnroll slot loads in flight per
+// lane: most scanned rows are short, and a wave per line would leave most
+// lanes idle and keep one line's chain of dependent loads in flight instead
+// of several.  Graphs with long rows take 16 lanes x 8 loads, the others 8 x
+// 16 (measured, lanes x loads: R-MAT 22 with 50 000-line deletion batches,
+// staging per batch 64x8 0.88 ms, 16x8 0.46, 8x16 0.47, 8x8 0.54, 4x16 0.71;
+// a 512-graph arena of mostly short rows, 8.7 M deletion lines per step,
+// step time 16x8 150 ms, 8x8 137, 8x16 137, 4x16 130; a quarter of R-MAT 24
+// with 500 000-line deletion batches: 16x8 1 ms per step faster than 8x16).
+template <int kTombLanes, int kTombUnroll>
 __global__ void k_tombstone(const int2* del, uint32_t ndel, const int2* xdel, uint32_t nxdel,
                             GraphView G, Counters* ctr) {
   const int sub = lane_id() & (kTombLanes - 1);
@@ -1038,9 +1039,14 @@ int csr_apply(Graph* g, const Batch& b, int rule) {
   }
   uint32_t items = c.n_del + c.n_xdel;
   if (items > 0) {
-    int64_t threads = (int64_t)items * kTombLanes;
-    hipLaunchKernelGGL(k_tombstone, dim3(grid_for(threads)), dim3(kBlock), 0, g->stream, g->del,
-                       c.n_del, g->xdel, c.n_xdel, g->view(), g->ctr);
+    // row slots (with slack) per owned vertex: 32 or more -> wide groups
+    const bool wide = g->rows_end >= 32ull * (uint64_t)std::max(g->n_owned, 1);
+    if (wide)
+      hipLaunchKernelGGL((k_tombstone<16, 8>), dim3(grid_for((int64_t)items * 16)), dim3(kBlock), 0,
+                         g->stream, g->del, c.n_del, g->xdel, c.n_xdel, g->view(), g->ctr);
+    else
+      hipLaunchKernelGGL((k_tombstone<8, 16>), dim3(grid_for((int64_t)items * 8)), dim3(kBlock), 0,
+                         g->stream, g->del, c.n_del, g->xdel, c.n_xdel, g->view(), g->ctr);
     DC_HIP(hipGetLastError());
   }
   g->staged = true;
