@@ -34,7 +34,7 @@ EXPORTS = [
     "dc_graph_info", "dc_colors_set", "dc_colors_get", "dc_static_color", "dc_apply_batch",
     "dc_batch_create", "dc_batch_destroy", "dc_apply_device_batch", "dc_recolor",
     "dc_process_batch", "dc_validate", "dc_trace_get", "dc_neighbors", "dc_prop_get",
-    "dc_compact", "dc_check_consistency", "dc_set_profiling", "dc_set_driver", "dc_synchronize", "dc_last_error", "dc_device_count",
+    "dc_compact", "dc_check_consistency", "dc_set_profiling", "dc_set_driver", "dc_set_parts", "dc_synchronize", "dc_last_error", "dc_device_count",
     "dc_gen_er", "dc_gen_rmat", "dc_gen_batches",
     "dc_server_create", "dc_server_destroy", "dc_server_run",
 ]
@@ -116,6 +116,7 @@ def lib():
         "dc_check_consistency": (C.c_int, [_vp, C.POINTER(C.c_int64)]),
         "dc_set_profiling": (C.c_int, [_vp, C.c_int32]),
         "dc_set_driver": (C.c_int, [_vp, C.c_int32, C.c_int32]),
+        "dc_set_parts": (C.c_int, [_vp, C.c_int32, _i32p]),
         "dc_synchronize": (C.c_int, [_vp]),
         "dc_last_error": (C.c_char_p, []),
         "dc_device_count": (C.c_int, [C.POINTER(C.c_int32)]),
@@ -306,6 +307,12 @@ class Graph:
 
     def set_driver(self, driver, blocks_per_cu=0):
         _check(lib().dc_set_driver(self._h, driver, blocks_per_cu))
+
+    def set_parts(self, offsets):
+        """dc_set_parts: the graph is a union of disjoint parts (offsets of
+        length nparts + 1; an empty array clears)."""
+        off = _arr(offsets, np.int32)
+        _check(lib().dc_set_parts(self._h, max(off.size - 1, 0), off))
 
     def set_profiling(self, on=True):
         _check(lib().dc_set_profiling(self._h, 1 if on else 0))

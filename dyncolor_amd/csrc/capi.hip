@@ -453,6 +453,17 @@ int dc_set_driver(dc_graph* h, int32_t driver, int32_t blocks_per_cu) {
   return DC_OK;
 }
 
+int dc_set_parts(dc_graph* h, int32_t nparts, const int32_t* offsets) {
+  if (!h || nparts < 0 || (nparts > 0 && !offsets)) return DC_ERR_INVALID;
+  Graph* g = G_(h);
+  if (g->staged) {
+    dc::set_error("dc_set_parts: a batch is staged");
+    return DC_ERR_STATE;
+  }
+  DC_HIP(hipSetDevice(g->device));
+  return dc::parts_set(g, nparts, offsets);
+}
+
 int dc_set_profiling(dc_graph* h, int32_t on) {
   if (!h) return DC_ERR_INVALID;
   G_(h)->profiling = on != 0;

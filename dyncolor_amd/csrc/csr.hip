@@ -414,6 +414,11 @@ void free_state(Graph* g) {
   f(g->scratch_batch.u); f(g->scratch_batch.v); f(g->scratch_batch.ins);
   if (g->hctr) (void)hipHostFree(g->hctr);
   if (g->pre_host) (void)hipHostFree(g->pre_host);
+  f(g->vpart);
+  f(g->pstate);
+  g->vpart = nullptr;
+  g->pstate = nullptr;
+  g->nparts = 0;
   f(g->pre_dev);
   g->pre_dev = nullptr;
   g->pre_host = nullptr;
@@ -676,6 +681,92 @@ int csr_build(Graph* g, int64_t m, const int32_t* du, const int32_t* dv) {
   return hist_reindex(g);
 }
 
+// ------------------------------------------------------------- parts -------
+// vertex -> part from the part offsets (binary search)
+__global__ void k_part_map(int32_t n_owned, const int32_t* off, int32_t nparts, int32_t* vpart) {
+  for (int32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n_owned; v += gridDim.x * blockDim.x) {
+    int32_t lo = 0, hi = nparts - 1;
+    while (lo < hi) {
+      const int32_t mid = (lo + hi + 1) >> 1;
+      if (off[mid] <= v) lo = mid;
+      else hi = mid - 1;
+    }
+    vpart[v] = lo;
+  }
+}
+
+// live slots whose two ends lie in different parts
+__global__ void k_part_cross(GraphView G, const int32_t* vpart, unsigned long long* bad) {
+  const int lane = lane_id();
+  const int64_t gwave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  unsigned long long b = 0;
+  for (int64_t v0 = gwave; v0 < G.n_owned; v0 += nwaves) {
+    const int32_t v = (int32_t)v0;
+    const uint2 r = G.row[v];
+    const int32_t pv = vpart[v];
+    for (uint32_t j = lane; j < r.y; j += 64) {
+      const int32_t w = G.col[(uint64_t)r.x + j];
+      if (w >= 0 && vpart[w] != pv) ++b;
+    }
+  }
+  wave_add_u64(b, bad);
+}
+
+// Declare the graph a union of disjoint parts (part p = owned vertices
+// offsets[p] .. offsets[p+1]-1, no edge between parts).  Checks the offsets
+// and that no live edge joins two parts.
+int parts_set(Graph* g, int32_t nparts, const int32_t* offsets) {
+  auto f = [](void* p) {
+    if (p) (void)hipFree(p);
+  };
+  f(g->vpart);
+  f(g->pstate);
+  g->vpart = nullptr;
+  g->pstate = nullptr;
+  g->nparts = 0;
+  if (nparts == 0) return DC_OK;
+  if (g->n_owned < g->n) {
+    set_error("dc_set_parts: not for a partition share (ghost vertices)");
+    return DC_ERR_INVALID;
+  }
+  if (offsets[0] != 0 || offsets[nparts] != g->n_owned) {
+    set_error("dc_set_parts: offsets must run from 0 to the vertex count");
+    return DC_ERR_INVALID;
+  }
+  for (int32_t p = 0; p < nparts; ++p)
+    if (offsets[p + 1] <= offsets[p]) {
+      set_error("dc_set_parts: every part needs at least one vertex, in increasing order");
+      return DC_ERR_INVALID;
+    }
+  int32_t* doff = nullptr;
+  DC_HIP(hipMalloc(&doff, sizeof(int32_t) * ((size_t)nparts + 1)));
+  DC_HIP(hipMalloc(&g->vpart, sizeof(int32_t) * ((size_t)g->n_owned + 1)));
+  DC_HIP(hipMalloc(&g->pstate, sizeof(uint32_t) * 4 * (size_t)nparts));
+  DC_HIP(hipMemcpyAsync(doff, offsets, sizeof(int32_t) * ((size_t)nparts + 1), hipMemcpyHostToDevice,
+                        g->stream));
+  hipLaunchKernelGGL(k_part_map, dim3(grid_for(g->n_owned)), dim3(kBlock), 0, g->stream, g->n_owned,
+                     doff, nparts, g->vpart);
+  unsigned long long* bad = reinterpret_cast<unsigned long long*>(g->aux);
+  DC_HIP(hipMemsetAsync(bad, 0, sizeof(unsigned long long), g->stream));
+  hipLaunchKernelGGL(k_part_cross, dim3(2048), dim3(kBlock), 0, g->stream, g->view(), g->vpart, bad);
+  DC_HIP(hipGetLastError());
+  unsigned long long h = 0;
+  DC_HIP(hipMemcpyAsync(&h, bad, sizeof(h), hipMemcpyDeviceToHost, g->stream));
+  DC_HIP(hipStreamSynchronize(g->stream));
+  DC_HIP(hipFree(doff));
+  if (h) {
+    f(g->vpart);
+    f(g->pstate);
+    g->vpart = nullptr;
+    g->pstate = nullptr;
+    set_error("dc_set_parts: an edge joins two parts");
+    return DC_ERR_INVALID;
+  }
+  g->nparts = nparts;
+  return DC_OK;
+}
+
 // ------------------------------------------------------------- apply -------
 // One thread per change line: append insertions to both rows (reserving a
 // slot with an atomic on the row's used count), move the LDF counter, and
@@ -712,6 +803,8 @@ __global__ void k_apply(int64_t k, const int32_t* bu, const int32_t* bv, const u
       G.mexc[a] = -1;
       if (!cross) G.mexc[b] = -1;
     }
+    // an insertion between two parts of an arena ends their independence
+    if (ok && is_ins && !cross && G.vpart && G.vpart[a] != G.vpart[b]) atomicOr(&ctr->xpart, 1u);
     // a new slot's colour enters its row's histogram (its slot may land
     // after a regrow, which recounts the histograms anyway)
     if (ok && is_ins) {
@@ -1024,6 +1117,9 @@ int csr_apply(Graph* g, const Batch& b, int rule) {
   DC_HIP(hipMemcpyAsync(g->hctr, g->ctr, sizeof(Counters), hipMemcpyDeviceToHost, g->stream));
   DC_HIP(hipStreamSynchronize(g->stream));
   Counters c = *g->hctr;
+  // an edge between parts: they are no longer independent, the loop stops
+  // treating them apart
+  if (c.xpart && g->nparts > 0) g->nparts = 0;
   g->staged_n[0] = c.n_del;
   g->staged_n[1] = c.n_ins;
   g->staged_n[2] = c.n_xdel;

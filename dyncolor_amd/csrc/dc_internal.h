@@ -52,7 +52,7 @@ struct Counters {
   int32_t maxc;
   int32_t window_full;    // a hub mex window was full -> slow path used
   int32_t cmax;           // highest colour (insertion round: are histograms complete?)
-  uint32_t pad1;
+  uint32_t xpart;         // insertion lines between two parts (dc_set_parts)
 };
 
 // Everything a kernel needs about one graph, passed by value.
@@ -74,6 +74,7 @@ struct GraphView {
   int32_t* mexc;      // vertex -> mex of all its live neighbours' colours, -1 = stale
   int32_t* hidx;      // vertex -> histogram row, -1 for rows with cap < kSmallLen
   uint32_t* hist;     // hub row -> live neighbours per colour 0..kHistW-1
+  const int32_t* vpart;  // owned vertex -> its part (dc_set_parts), or null
 };
 
 struct Batch {
@@ -155,6 +156,11 @@ struct Graph {
   // the staged batch's deletion, insertion, cross-deletion and cross-insertion
   // line counts (host copy from the apply)
   uint32_t staged_n[4] = {0u, 0u, 0u, 0u};
+  // disjoint parts of an arena (dc_set_parts): vertex -> part, and per part
+  // {changes by round parity [2], check failed, left the loop}
+  int32_t nparts = 0;
+  int32_t* vpart = nullptr;
+  uint32_t* pstate = nullptr;
   // pre-loop rounds' results, gathered on the device and read back once:
   // kPreSlots x {frontier counts, Counters} (device and pinned host)
   void* pre_dev = nullptr;
@@ -187,7 +193,8 @@ struct Graph {
   void* inc = nullptr;
 
   GraphView view() const {
-    return GraphView{n, n_owned, row, cap, col, prop, ccol, twin, mexc, hidx, hist};
+    return GraphView{n, n_owned, row, cap, col, prop, ccol, twin, mexc, hidx, hist,
+                     nparts > 0 ? vpart : nullptr};
   }
 };
 
@@ -207,6 +214,7 @@ int csr_apply(Graph* g, const Batch& b, int rule);
 int csr_compact(Graph* g);
 int rebuild_ccol(Graph* g);               // ccol[j] <- colour[col[j]] for every slot (+ hub histograms)
 int hist_reindex(Graph* g);               // hub rows from cap[], then rebuild the histograms
+int parts_set(Graph* g, int32_t nparts, const int32_t* offsets);   // nparts 0 clears
 int check_consistency(Graph* g, int64_t* bad);  // ccol/twin invariants (tests)
 int alloc_state(Graph* g);
 void free_state(Graph* g);

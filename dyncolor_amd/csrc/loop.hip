@@ -95,7 +95,7 @@ struct LoopCtl {
   int32_t checks_left;  // checks still allowed
   int32_t unsettled;    // a check found a vertex off its mex
   int32_t settled;      // the loop stopped at settled colours (done_round = last round run)
-  int32_t pad;
+  int32_t newly_left;   // parts left the loop at the latest part check
 };
 
 struct LoopState {
@@ -136,6 +136,14 @@ struct LoopState {
   int2* dq_h;
   uint32_t dq_s_cap, dq_m_cap, dq_h_cap;
   uint32_t* qc;               // [kQWords]
+  // disjoint parts (dc_set_parts): vertex -> part; per part its colour
+  // changes by round parity [2][nparts], a failed part check [nparts], and
+  // whether it left the loop [nparts] (2: at the latest check, 1: before)
+  const int32_t* vpart;
+  uint32_t* pchg;
+  uint32_t* punset;
+  uint32_t* pret;
+  int32_t nparts;
   int32_t list_c;             // rounds with |L| up to this list their sets
   unsigned long long list_slots;  // ... and R too when D's rows hold at most this many slots
   unsigned long long* wstat;  // 4-byte words of state arrays and lists passed by count passes
@@ -444,6 +452,9 @@ __global__ __launch_bounds__(kBlock) void k_loop_init(LoopState S) {
   // touched list
   if (blockIdx.x == 0 && threadIdx.x < kQWords)
     S.qc[threadIdx.x] = threadIdx.x == kQT + 3 ? kQInvalid : 0u;
+  // part state (changes of both parities, checks, left) starts clear
+  if (blockIdx.x == 0)
+    for (int i = threadIdx.x; i < 4 * S.nparts; i += blockDim.x) S.pchg[i] = 0u;
   const int64_t gwave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
   const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
   auto row = [&](int32_t v, uint2 r, uint32_t lo, uint32_t hi, uint32_t stride, bool head) {
@@ -484,13 +495,14 @@ __global__ __launch_bounds__(kBlock) void k_loop_init(LoopState S) {
 // prev) state changed.
 __device__ __forceinline__ bool commit_member(const LoopState& S, uint32_t* sn, int32_t w,
                                               int32_t cm, int32_t so, int32_t po, bool inl,
-                                              int32_t m, uint32_t& changed) {
+                                              int32_t m, uint32_t& changed, uint32_t* pch) {
   if (cm < 0) S.G.mexc[w] = m;
   S.col0[w] = m;
   S.col1[w] = m;
   S.prev[w] = so;
   const bool chv = m != so;
   changed += chv ? 1u : 0u;
+  if (chv && pch) atomicAdd(&pch[S.vpart[w]], 1u);
   const bool dv = !inl || po != so || chv;
   if (dv) {
     S.dold[w] = make_int2(so, (po + 1) | (inl ? (int32_t)0x80000000 : 0));
@@ -560,6 +572,8 @@ __device__ __forceinline__ void round_a(const LoopState& S, int r, LoopLds& lds)
   const int32_t nl = r == 0 ? S.ctl->nl0 : S.trace[r - 1].y;
   const bool listing = nl <= S.list_c;
   uint32_t* qd = S.qc + kQD + 3 * (r & 1);
+  // colour changes of this round by part (parts set)
+  uint32_t* pch = S.nparts ? S.pchg + (size_t)(r & 1) * S.nparts : nullptr;
   // a row put into D: its slots, and which count-pass path takes it
   auto d_row = [&](int32_t w) {
     const uint32_t len = G.row[w].y;
@@ -608,7 +622,7 @@ __device__ __forceinline__ void round_a(const LoopState& S, int r, LoopLds& lds)
         S.dold[w] = make_int2(so, (po + 1) | (int32_t)0x80000000);
         st_set(sn, w, kFD);
       }
-      if (member && cm >= 0) dv = commit_member(S, sn, w, cm, so, po, inl, cm, changed);
+      if (member && cm >= 0) dv = commit_member(S, sn, w, cm, so, po, inl, cm, changed, pch);
       if (dv) d_row(w);
       members += member ? 1u : 0u;
       const bool defer = member && cm < 0;
@@ -641,7 +655,7 @@ __device__ __forceinline__ void round_a(const LoopState& S, int r, LoopLds& lds)
       }
       const bool wst = valid && m < 0;
       if (valid && !wst && sub == 0 &&
-          commit_member(S, sn, w, -1, S.col0[w], S.prev[w], st_has(sp, w, kFC), m, changed))
+          commit_member(S, sn, w, -1, S.col0[w], S.prev[w], st_has(sp, w, kFC), m, changed, pch))
         d_row(w);
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
       __builtin_amdgcn_wave_barrier();
@@ -656,7 +670,7 @@ __device__ __forceinline__ void round_a(const LoopState& S, int r, LoopLds& lds)
       const int32_t w = lst[q];
       const int32_t m = wave_row_mex<M_LOOP_HI>(RA, w, win);
       if (lane == 0 &&
-          commit_member(S, sn, w, -1, S.col0[w], S.prev[w], st_has(sp, w, kFC), m, changed))
+          commit_member(S, sn, w, -1, S.col0[w], S.prev[w], st_has(sp, w, kFC), m, changed, pch))
         d_row(w);
     }
   };
@@ -1039,6 +1053,7 @@ __device__ __forceinline__ void round_c(const LoopState& S, int r, LoopLds& lds)
       S.trace[r].x = nl;
       S.trace[r].y = (int32_t)nmem;
       S.trace[r].z = (int32_t)lds.tot[1];
+      S.trace[r].w = 0;   // parts that leave after this round (part checks)
       S.ctl->last_nc = (int32_t)nmem;
       S.ctl->last_changed = (int32_t)lds.tot[1];
       if (!nonempty) S.ctl->done_round = r;
@@ -1050,6 +1065,8 @@ __device__ __forceinline__ void round_c(const LoopState& S, int r, LoopLds& lds)
     }
     for (int i = threadIdx.x; i < kAcc * kAccW; i += kBlock)
       S.acc[(size_t)((r + 1) & 1) * kAcc * kAccW + i] = 0ull;
+    for (int p = threadIdx.x; p < S.nparts; p += kBlock)
+      S.pchg[(size_t)((r + 1) & 1) * S.nparts + p] = 0u;
   }
   if (!nonempty) {
     // D_r's leaving members
@@ -1199,7 +1216,11 @@ __global__ __launch_bounds__(kBlock) void k_loop_persistent(LoopState S, int r0,
 // is; a stale one is recomputed (histogram or short-row scan, by 8-lane
 // groups) and cached.  A mex past the histogram counts as a mismatch: the
 // check only errs towards "not settled".  *unsettled is set, never cleared.
-__device__ __forceinline__ void settle_check_body(const LoopState& S, int32_t* unsettled) {
+// Every owned vertex not skipped: is its colour the mex of its neighbours'
+// colours?  report(v) for each that is not (a mex past the histogram counts
+// as not).  A stale cached mex is recomputed and cached.
+template <class Skip, class Report>
+__device__ __forceinline__ void settle_scan(const LoopState& S, Skip skip, Report report) {
   __shared__ int32_t stale_list[kBlock / 64][64];
   const GraphView& G = S.G;
   const int lane = lane_id();
@@ -1207,12 +1228,11 @@ __device__ __forceinline__ void settle_check_body(const LoopState& S, int32_t* u
   int32_t* lst = stale_list[threadIdx.x >> 6];
   const int64_t gwave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
   const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
-  bool bad = false;
   for (int64_t base = gwave * 64; base < G.n_owned; base += nwaves * 64) {
     const int64_t v = base + lane;
-    const bool valid = v < G.n_owned;
+    const bool valid = v < G.n_owned && !skip((int32_t)v);
     const int32_t m = valid ? G.mexc[v] : 0;
-    if (valid && m >= 0 && m != S.col0[v]) bad = true;
+    if (valid && m >= 0 && m != S.col0[v]) report((int32_t)v);
     const bool stale = valid && m < 0;
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -1232,12 +1252,17 @@ __device__ __forceinline__ void settle_check_body(const LoopState& S, int32_t* u
                     : group_small_mex(G, G.row[w], sub);
       }
       if (ok && sub == 0) {
-        if (mw < 0 || mw != S.col0[w]) bad = true;
+        if (mw < 0 || mw != S.col0[w]) report(w);
         if (mw >= 0) G.mexc[w] = mw;
       }
     }
   }
-  if (__ballot(bad) != 0ull && lane == 0) *unsettled = 1;
+}
+
+__device__ __forceinline__ void settle_check_body(const LoopState& S, int32_t* unsettled) {
+  bool bad = false;
+  settle_scan(S, [](int32_t) { return false; }, [&](int32_t) { bad = true; });
+  if (__ballot(bad) != 0ull && lane_id() == 0) *unsettled = 1;
 }
 
 __global__ __launch_bounds__(kBlock) void k_settle_check(LoopState S, int32_t* unsettled) {
@@ -1256,6 +1281,68 @@ __device__ __forceinline__ bool settle_check_due(const LoopCtl* c) {
 __global__ __launch_bounds__(kBlock) void k_settle_check_if(LoopState S) {
   if (!settle_check_due(S.ctl)) return;
   settle_check_body(S, &S.ctl->unsettled);
+}
+
+// Part checks (parts set), queued after the check rounds: a part that
+// changed no colour in round r and holds every vertex at its mex leaves the
+// loop: its later rounds in the reference recolour each member to the colour
+// it has, and no edge reaches another part.  k_part_check_if marks the parts
+// that fail, k_part_decide (one workgroup) marks the parts that leave (pret
+// 2) and counts them into round r's trace entry, k_part_clear takes their
+// vertices out of the loop (count 0, no state flags).
+__global__ __launch_bounds__(kBlock) void k_part_check_if(LoopState S, int32_t r) {
+  if (S.ctl->done_round != INT_MAX) return;
+  const uint32_t* pc = S.pchg + (size_t)(r & 1) * S.nparts;
+  settle_scan(S,
+              [&](int32_t v) {
+                const int32_t p = S.vpart[v];
+                return S.pret[p] != 0u || pc[p] != 0u;
+              },
+              [&](int32_t v) { S.punset[S.vpart[v]] = 1u; });
+}
+
+__global__ __launch_bounds__(1024) void k_part_decide(LoopState S, int32_t r) {
+  if (S.ctl->done_round != INT_MAX) return;
+  if (threadIdx.x == 0) S.ctl->newly_left = 0;
+  __syncthreads();
+  const uint32_t* pc = S.pchg + (size_t)(r & 1) * S.nparts;
+  uint32_t left = 0;
+  for (int32_t p = threadIdx.x; p < S.nparts; p += blockDim.x) {
+    uint32_t ret = S.pret[p];
+    if (ret == 2u) {
+      S.pret[p] = 1u;
+      ret = 1u;
+    }
+    if (ret == 0u && pc[p] == 0u && S.punset[p] == 0u) {
+      S.pret[p] = 2u;
+      ++left;
+    }
+    S.punset[p] = 0u;
+  }
+  if (left) {
+    atomicAdd(&S.trace[r].w, (int32_t)left);
+    S.ctl->newly_left = 1;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_part_clear(LoopState S) {
+  if (S.ctl->newly_left == 0) return;
+  for (uint32_t w = blockIdx.x * blockDim.x + threadIdx.x; w < S.swords; w += gridDim.x * blockDim.x) {
+    const uint32_t a = S.St[0][w], b = S.St[1][w];
+    uint32_t clear = 0u;
+    for (int i = 0; i < 8; ++i) {
+      const uint32_t f = ((a | b) >> (4 * i)) & 15u;
+      const int32_t v = (int32_t)(w * 8 + i);
+      if (f && v < S.G.n_owned && S.pret[S.vpart[v]] == 2u) {
+        S.K[v] = 0;
+        clear |= 15u << (4 * i);
+      }
+    }
+    if (clear) {
+      S.St[0][w] = a & ~clear;
+      S.St[1][w] = b & ~clear;
+    }
+  }
 }
 
 __global__ void k_settle_decide(LoopState S, int32_t r) {
@@ -1439,6 +1526,12 @@ int loop_hi(Graph* g, Frontier L0, int32_t nl0, bool persistent, LoopResult* res
   S.n_hch = g->n_hch;
   S.hrowv = g->hrowv;
   S.n_hrows = g->n_hrows;
+  // disjoint parts: checked on their own in the plain driver
+  S.nparts = (!persistent && !g->opt.exact_rounds) ? g->nparts : 0;
+  S.vpart = S.nparts ? g->vpart : nullptr;
+  S.pchg = S.nparts ? g->pstate : nullptr;
+  S.punset = S.nparts ? g->pstate + 2 * (size_t)S.nparts : nullptr;
+  S.pret = S.nparts ? g->pstate + 3 * (size_t)S.nparts : nullptr;
   // D's mid-row and hub-chunk lists hold at most the static tables' entries
   if (S.dq_m_cap < g->n_hrows) {
     if (S.dq_m) DC_HIP(hipFree(S.dq_m));
@@ -1526,6 +1619,14 @@ int loop_hi(Graph* g, Frontier L0, int32_t nl0, bool persistent, LoopResult* res
           hipLaunchKernelGGL(k_settle_check_if, dim3(kMaxBlocks), blk, 0, g->stream, S);
           hipLaunchKernelGGL(k_settle_decide, dim3(1), dim3(64), 0, g->stream, S, r + i);
           launches += 2;
+          if (S.nparts) {
+            hipLaunchKernelGGL(k_part_check_if, dim3(kMaxBlocks), blk, 0, g->stream, S, r + i);
+            hipLaunchKernelGGL(k_part_decide, dim3(1), dim3(1024), 0, g->stream, S, r + i);
+            hipLaunchKernelGGL(k_part_clear,
+                               dim3((uint32_t)std::min<uint64_t>(2048, (S.swords + kBlock - 1) / kBlock)),
+                               blk, 0, g->stream, S);
+            launches += 3;
+          }
         }
       }
       DC_HIP(hipGetLastError());
@@ -1592,6 +1693,9 @@ int loop_hi(Graph* g, Frontier L0, int32_t nl0, bool persistent, LoopResult* res
   for (int32_t i = 0; i < rounds; ++i) {
     g->trace.insert(g->trace.end(),
                     {(int64_t)10, (int64_t)tr[i].x, (int64_t)tr[i].y, (int64_t)tr[i].z});
+    // parts found settled after this round left the loop
+    if (tr[i].w > 0)
+      g->trace.insert(g->trace.end(), {(int64_t)12, (int64_t)i + 1, (int64_t)tr[i].w, (int64_t)0});
     res->frontier_sum += (uint64_t)tr[i].x;
     res->evaluated += (uint64_t)tr[i].y;
     res->changed += (uint64_t)tr[i].z;

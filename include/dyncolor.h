@@ -175,7 +175,9 @@ int dc_validate(dc_graph* g, int64_t* monochromatic, int32_t* max_color);
  * changed), 3 LDF change-edge round (endpoints, changed), 4 cross-deletion
  * round, 5 cross-insertion round, 10 loop round (|L|, |C|, changed), 11 the
  * loop stopped at settled colours (rounds run, 0, 0): the reference runs
- * more rounds, none of which changes a color. */
+ * more rounds, none of which changes a color, 12 parts (dc_set_parts) found
+ * settled after that many rounds left the loop (rounds run, parts, 0): the
+ * later loop rounds count only the other parts' vertices. */
 int dc_trace_get(dc_graph* g, int64_t* trace, int32_t cap, int32_t* n_phases);
 
 /* Live neighbours of v (multi-edges repeated), for tests and tools. */
@@ -204,6 +206,18 @@ int dc_set_profiling(dc_graph* g, int32_t on);
  * HI/Color_main.cu:181-202); the fused driver runs the same rounds inside one
  * cooperative launch. */
 int dc_set_driver(dc_graph* g, int32_t driver, int32_t blocks_per_cu);
+
+/* Declare the graph a union of disjoint parts (an arena of independent
+ * graphs): part p holds the vertices offsets[p] .. offsets[p+1]-1, with
+ * offsets[0] = 0, offsets[nparts] = the vertex count, and no edge between
+ * two parts (checked: DC_ERR_INVALID).  The independent-rule loop (plain
+ * driver, exact_rounds = 0) then also checks each part on its own: a part
+ * whose colours are settled leaves the loop while the others go on (trace
+ * kind 12); colours are the same either way.  A later batch inserting an edge
+ * between two parts ends this.  nparts = 0 clears it.  No counterpart in the
+ * reference, which colours one graph per process (HI/Color_main.cu:31-247);
+ * it serves the batched and served workloads (dc_server_create sets it). */
+int dc_set_parts(dc_graph* g, int32_t nparts, const int32_t* offsets);
 
 int dc_synchronize(dc_graph* g);
 const char* dc_last_error(void);
