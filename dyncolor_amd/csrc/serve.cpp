@@ -57,6 +57,16 @@ int dc_server_create(dc_graph* arena, int32_t ngraphs, const int32_t* offsets,
       dc::set_error("dc_server_create: offsets must be non-decreasing");
       return DC_ERR_INVALID;
     }
+  // the graphs are the arena's parts (empty graphs dropped): a graph whose
+  // colours settle leaves the loop while the others go on.  An arena with
+  // edges between its graphs keeps one loop (parts are only an optimisation).
+  {
+    std::vector<int32_t> parts{0};
+    for (int32_t i = 1; i <= ngraphs; ++i)
+      if (offsets[i] > parts.back()) parts.push_back(offsets[i]);
+    if (n_owned == n && dc_set_parts(arena, (int32_t)parts.size() - 1, parts.data()) != DC_OK)
+      (void)dc_set_parts(arena, 0, nullptr);
+  }
   dc_server* s = new (std::nothrow) dc_server();
   if (!s) return DC_ERR_NOMEM;
   s->arena = arena;
