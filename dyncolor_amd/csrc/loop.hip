@@ -318,19 +318,6 @@ __device__ __forceinline__ uint32_t block_sum(uint32_t v, uint32_t* scratch) {
   return t;
 }
 
-__device__ __forceinline__ unsigned long long block_sum64(unsigned long long v,
-                                                          unsigned long long* scratch) {
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-  const int wib = threadIdx.x >> 6;
-  __syncthreads();
-  if (lane_id() == 0) scratch[wib] = v;
-  __syncthreads();
-  unsigned long long t = 0;
-  for (int i = 0; i < (int)(blockDim.x >> 6); ++i) t += scratch[i];
-  __syncthreads();
-  return t;
-}
-
 // A per-lane value from lane src, word by word (a shuffle every lane takes
 // part in).
 template <class P>
