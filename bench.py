@@ -274,7 +274,6 @@ def run_c4(args, d, rule):
     arena = d.Graph(n, U, V)
     arena.static_color(rule)
     set_driver(arena, d, args)
-    arena.set_parts(off)   # the graphs are disjoint parts: settled ones leave the loop
     arena.set_profiling(True)
     ub = [arena.upload_batch(*d.union_batch([p[s] for p in per], off)) for s in range(nsteps)]
     for s in range(args.warmup):
