@@ -55,8 +55,15 @@
 //         The previous round's state array is cleared.
 // The plain driver launches one kernel per phase (the host enqueues rounds in
 // blocks and reads one control word per block; rounds past the last one
-// return at once); the persistent driver runs the rounds in one cooperative
-// launch with grid barriers between phases.
+// return at once; settle checks are queued between rounds); the persistent
+// driver runs the rounds in one cooperative launch with grid barriers between
+// phases.
+//
+// Settled colours end the loop early: after a round without a colour change
+// in which every owned vertex holds the mex of its neighbours' colours, every
+// later round recolours each member to the colour it has.  With disjoint
+// parts (an arena of graphs), a part that passes the same test on its own
+// leaves the loop while the others go on.
 //
 // Outside the loop K is zero everywhere and so are the state arrays.
 #include <hip/hip_runtime.h>
