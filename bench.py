@@ -131,11 +131,15 @@ def run_stream(args, d, rule, n, u, v, workload):
     tot = {"bytes": 0, "kms": 0.0, "launches": 0, "rounds": 0, "changed": 0, "evaluated": 0,
            "slots": 0, "regrows": 0, "settled": 0}
     t0 = time.perf_counter()
+    last_log = t0
     for i in range(args.warmup, args.warmup + args.steps):
         tb = time.perf_counter()
         g.apply_device_batch(dev_batches[i])
         st = g.recolor(rule)
         lat.append(time.perf_counter() - tb)
+        if tb - last_log > 20.0:   # slow workloads (the LDF rule at scale) report progress
+            log(f"timed batch {i - args.warmup + 1}/{args.steps}")
+            last_log = tb
         tot["bytes"] += st.recolor_bytes
         tot["kms"] += st.ms_recolor_kernel
         tot["launches"] += st.recolor_launches
@@ -166,7 +170,11 @@ def run_stream(args, d, rule, n, u, v, workload):
         set_driver(g, d, args)
         dev_batches = [g.upload_batch(*b) for b in batches]
         replay_rounds = 0
+        last_log = time.perf_counter()
         for i in range(args.warmup + args.steps):
+            if time.perf_counter() - last_log > 20.0:
+                log(f"replay batch {i + 1}/{args.warmup + args.steps}")
+                last_log = time.perf_counter()
             g.set_profiling(i >= args.warmup)
             g.apply_device_batch(dev_batches[i])
             st = g.recolor(rule)
