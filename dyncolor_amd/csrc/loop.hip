@@ -143,8 +143,8 @@ struct LoopState {
   int2* dq_h;
   uint32_t dq_s_cap, dq_m_cap, dq_h_cap;
   uint32_t* qc;               // [kQWords]
-  // disjoint parts (dc_set_parts): vertex -> part; per part its colour
-  // changes by round parity [2][nparts], a failed part check [nparts], and
+  // disjoint parts (dc_set_parts): vertex -> part; per part whether a colour
+  // changed, by round parity [2][nparts], a failed part check [nparts], and
   // whether it left the loop [nparts] (2: at the latest check, 1: before)
   const int32_t* vpart;
   uint32_t* pchg;
@@ -496,7 +496,12 @@ __device__ __forceinline__ bool commit_member(const LoopState& S, uint32_t* sn, 
   S.prev[w] = so;
   const bool chv = m != so;
   changed += chv ? 1u : 0u;
-  if (chv && pch) atomicAdd(&pch[S.vpart[w]], 1u);
+  // the part changed this round (a flag: a counter atomic per change would
+  // serialise on a large part's one address)
+  if (chv && pch) {
+    const int32_t p = S.vpart[w];
+    if (pch[p] == 0u) pch[p] = 1u;
+  }
   const bool dv = !inl || po != so || chv;
   if (dv) {
     S.dold[w] = make_int2(so, (po + 1) | (inl ? (int32_t)0x80000000 : 0));
