@@ -1038,12 +1038,17 @@ __device__ __forceinline__ void round_c(const LoopState& S, int r, LoopLds& lds)
   __syncthreads();
   const unsigned long long nmem = lds.tot[0];
   const bool nonempty = nmem != 0ull;
-  const bool heavy = lds.tot[2] > S.heavy_slots;
   const bool dhubs = lds.tot[3] != 0ull;
   const bool dmids = lds.tot[4] != 0ull;
+  const int32_t nl = r == 0 ? S.ctl->nl0 : S.trace[r - 1].y;
+  // pull recount when D's rows are many slots and C more than doubled (a push
+  // would spend an atomic on most new marking pairs), or when D's rows are
+  // half of all slots; otherwise push deltas (on R-MAT 22 the first round's
+  // D is a quarter of the slots with few term changes: push 0.8 ms faster)
+  const bool heavy = lds.tot[2] > S.heavy_slots &&
+                     (nmem > 2ull * (unsigned long long)nl || lds.tot[2] > 4ull * S.heavy_slots);
   // phase A listed round r's sets (the same test as there); R_r joins the
   // touched list in light rounds with few D slots
-  const int32_t nl = r == 0 ? S.ctl->nl0 : S.trace[r - 1].y;
   const bool listed = nl <= S.list_c;
   const bool listing = listed && !heavy && lds.tot[2] <= S.list_slots;
   if (blockIdx.x == 0) {
@@ -1546,8 +1551,8 @@ int loop_hi(Graph* g, Frontier L0, int32_t nl0, bool persistent, LoopResult* res
     DC_HIP(hipMalloc(&S.dq_h, sizeof(int2) * ((size_t)g->hch_cap + 1)));
     S.dq_h_cap = g->hch_cap + 1;
   }
-  // rounds whose D rows hold more than 1/8 of all slots recount by pull
-  // (measured on R-MAT 22 against 1/2 and 1/32)
+  // rounds whose D rows hold more than 1/8 of all slots (measured on R-MAT
+  // 22 against 1/2 and 1/32) recount by pull when C also more than doubled
   S.heavy_slots = std::max<unsigned long long>(g->rows_end / 8, 1ull << 16);
   S.list_c = (int32_t)std::min<uint64_t>(
       kListC, ((uint64_t)S.swords + g->n_hrows + g->n_hch) / 64);
