@@ -43,7 +43,8 @@ EXPORTS = [
 class Options(C.Structure):
     _fields_ = [("slack_frac", C.c_float), ("slack_min", C.c_int32), ("max_rounds", C.c_int32),
                 ("driver", C.c_int32), ("device", C.c_int32), ("spare_slots", C.c_int32),
-                ("exact_rounds", C.c_int32), ("list_limit", C.c_int32)]
+                ("exact_rounds", C.c_int32), ("list_limit", C.c_int32),
+                ("pull_mode", C.c_int32), ("reserved0", C.c_int32), ("heavy_slots", C.c_int64)]
 
 
 class Stats(C.Structure):
@@ -55,10 +56,13 @@ class Stats(C.Structure):
                 ("ms_recolor", C.c_double), ("ms_recolor_kernel", C.c_double),
                 ("recolor_launches", C.c_int32), ("regrows", C.c_int32),
                 ("ms_count_pass", C.c_double), ("count_pass_bytes", C.c_int64),
-                ("count_pass_launches", C.c_int32), ("loop_settled", C.c_int32)]
+                ("count_pass_launches", C.c_int32), ("loop_settled", C.c_int32),
+                ("pull_rounds", C.c_int32), ("heavy_push_rounds", C.c_int32),
+                ("heavy_push_listed", C.c_int32), ("reserved0", C.c_int32),
+                ("lc_vertices", C.c_int64), ("lc_slots", C.c_int64), ("del_row_slots", C.c_int64)]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+        return {k: getattr(self, k) for k, _ in self._fields_ if not k.startswith("reserved")}
 
 
 class ServeStats(C.Structure):
@@ -299,6 +303,7 @@ class Graph:
 
     def compact(self):
         _check(lib().dc_compact(self._h))
+
 
     def check_consistency(self) -> int:
         bad = C.c_int64(0)

@@ -878,7 +878,7 @@ __global__ void k_tombstone(const int2* del, uint32_t ndel, const int2* xdel, ui
   const int64_t grp = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / kTombLanes;
   const int64_t ngrp = ((int64_t)gridDim.x * blockDim.x) / kTombLanes;
   const int64_t items = (int64_t)ndel + nxdel;
-  unsigned long long tomb = 0;
+  unsigned long long tomb = 0, dslots = 0;
   for (int64_t it = grp; it < items; it += ngrp) {
     const bool cross = it >= (int64_t)ndel;
     const int2 e = cross ? xdel[it - ndel] : del[it];
@@ -896,6 +896,7 @@ __global__ void k_tombstone(const int2* del, uint32_t ndel, const int2* xdel, ui
         ry = u;
       }
     }
+    if (sub == 0) dslots += (unsigned long long)r.y + ry.y;
     // Each slot is retired by the one thread that swaps its live value out
     // (the same edge can come twice, in either orientation), which also
     // takes its colour out of the row's histogram.
@@ -929,6 +930,7 @@ __global__ void k_tombstone(const int2* del, uint32_t ndel, const int2* xdel, ui
     }
   }
   wave_add_u64(tomb, &ctr->tomb);
+  wave_add_u64(dslots, &ctr->delslots);
 }
 
 // regrow: copy every row's slots [0, min(used, old cap)) to the new layout

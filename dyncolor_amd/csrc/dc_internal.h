@@ -53,6 +53,7 @@ struct Counters {
   int32_t window_full;    // a hub mex window was full -> slow path used
   int32_t cmax;           // highest colour (insertion round: are histograms complete?)
   uint32_t xpart;         // insertion lines between two parts (dc_set_parts)
+  unsigned long long delslots;  // deletion lines: used slots of both endpoints' rows
 };
 
 // Everything a kernel needs about one graph, passed by value.
@@ -235,6 +236,11 @@ struct LoopResult {
   int64_t count_launches = 0;
   uint64_t count_bytes = 0;     // algorithmic bytes of those launches
   bool settled = false;         // stopped at settled colours (see loop_hi)
+  int32_t pull_rounds = 0;      // rounds recounted by pull
+  int32_t heavy_push_rounds = 0;  // pushed rounds whose D rows exceeded heavy_slots
+  int32_t heavy_push_listed = 0;  // ... that also listed R
+  uint64_t lc_vertices = 0;     // sum over rounds of |L_r u C_r| (profiled batches)
+  uint64_t lc_slots = 0;        // sum over rounds of the row slots of L_r u C_r
 };
 int loop_hi(Graph* g, Frontier L0, int32_t nl0, bool persistent, LoopResult* res);
 void loop_free(Graph* g);

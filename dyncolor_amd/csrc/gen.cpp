@@ -11,10 +11,7 @@
 #include <vector>
 
 #include "../../include/dyncolor.h"
-
-namespace dc {
-void set_error(const std::string& msg);
-}
+#include "dc_guard.h"
 
 namespace {
 
@@ -177,160 +174,166 @@ struct EdgeIndex {
 extern "C" {
 
 int dc_gen_er(int32_t n, int64_t m, uint64_t seed, int32_t* u, int32_t* v) {
-  if (n < 2 || m < 0 || (m > 0 && (!u || !v)) || (double)m > (double)n * (n - 1) / 2.0) {
-    dc::set_error("dc_gen_er: invalid argument");
-    return DC_ERR_INVALID;
-  }
-  // oversample candidate pairs in rounds; keep first occurrences in draw order
-  std::vector<uint64_t> chosen;  // (draw index << 0) keys in draw order
-  chosen.reserve((size_t)m);
-  std::vector<uint64_t> seen;    // sorted keys already chosen
-  uint64_t drawn = 0;
-  while ((int64_t)chosen.size() < m) {
-    int64_t need = m - (int64_t)chosen.size();
-    int64_t cnt = need + need / 8 + 1024;
-    std::vector<uint64_t> cand(cnt);
-    uint64_t base = drawn;
-    parallel_for(cnt, [&](int64_t lo, int64_t hi) {
-      for (int64_t i = lo; i < hi; ++i) {
-        Rng r(splitmix64(seed ^ 0x5851F42D4C957F2Dull) + (base + i) * 0x9E3779B97F4A7C15ull * 2);
-        uint32_t a = (uint32_t)r.below((uint64_t)n), b = (uint32_t)r.below((uint64_t)n);
-        cand[i] = a == b ? ~0ull : key_of(a, b);
+  return dc::guarded("dc_gen_er", [&]() -> int {
+    if (n < 2 || m < 0 || (m > 0 && (!u || !v)) || (double)m > (double)n * (n - 1) / 2.0) {
+      dc::set_error("dc_gen_er: invalid argument");
+      return DC_ERR_INVALID;
+    }
+    // oversample candidate pairs in rounds; keep first occurrences in draw order
+    std::vector<uint64_t> chosen;  // (draw index << 0) keys in draw order
+    chosen.reserve((size_t)m);
+    std::vector<uint64_t> seen;    // sorted keys already chosen
+    uint64_t drawn = 0;
+    while ((int64_t)chosen.size() < m) {
+      int64_t need = m - (int64_t)chosen.size();
+      int64_t cnt = need + need / 8 + 1024;
+      std::vector<uint64_t> cand(cnt);
+      uint64_t base = drawn;
+      parallel_for(cnt, [&](int64_t lo, int64_t hi) {
+        for (int64_t i = lo; i < hi; ++i) {
+          Rng r(splitmix64(seed ^ 0x5851F42D4C957F2Dull) + (base + i) * 0x9E3779B97F4A7C15ull * 2);
+          uint32_t a = (uint32_t)r.below((uint64_t)n), b = (uint32_t)r.below((uint64_t)n);
+          cand[i] = a == b ? ~0ull : key_of(a, b);
+        }
+      });
+      drawn += cnt;
+      // (key, draw order) sorted by key to find first occurrences
+      std::vector<std::pair<uint64_t, int64_t>> kp(cnt);
+      for (int64_t i = 0; i < cnt; ++i) kp[i] = {cand[i], i};
+      std::sort(kp.begin(), kp.end());
+      std::vector<int64_t> firsts;
+      for (int64_t i = 0; i < cnt; ++i) {
+        if (kp[i].first == ~0ull) continue;
+        if (i > 0 && kp[i].first == kp[i - 1].first) continue;
+        if (std::binary_search(seen.begin(), seen.end(), kp[i].first)) continue;
+        firsts.push_back(kp[i].second);
       }
-    });
-    drawn += cnt;
-    // (key, draw order) sorted by key to find first occurrences
-    std::vector<std::pair<uint64_t, int64_t>> kp(cnt);
-    for (int64_t i = 0; i < cnt; ++i) kp[i] = {cand[i], i};
-    std::sort(kp.begin(), kp.end());
-    std::vector<int64_t> firsts;
-    for (int64_t i = 0; i < cnt; ++i) {
-      if (kp[i].first == ~0ull) continue;
-      if (i > 0 && kp[i].first == kp[i - 1].first) continue;
-      if (std::binary_search(seen.begin(), seen.end(), kp[i].first)) continue;
-      firsts.push_back(kp[i].second);
+      std::sort(firsts.begin(), firsts.end());
+      for (int64_t idx : firsts) {
+        if ((int64_t)chosen.size() >= m) break;
+        chosen.push_back(cand[idx]);
+      }
+      seen = chosen;
+      std::sort(seen.begin(), seen.end());
     }
-    std::sort(firsts.begin(), firsts.end());
-    for (int64_t idx : firsts) {
-      if ((int64_t)chosen.size() >= m) break;
-      chosen.push_back(cand[idx]);
+    for (int64_t i = 0; i < m; ++i) {
+      u[i] = (int32_t)(chosen[i] >> 32);
+      v[i] = (int32_t)(chosen[i] & 0xffffffffu);
     }
-    seen = chosen;
-    std::sort(seen.begin(), seen.end());
-  }
-  for (int64_t i = 0; i < m; ++i) {
-    u[i] = (int32_t)(chosen[i] >> 32);
-    v[i] = (int32_t)(chosen[i] & 0xffffffffu);
-  }
-  return DC_OK;
+    return DC_OK;
+  });
 }
 
 int dc_gen_rmat(int32_t scale, int32_t edge_factor, double a, double b, double c, uint64_t seed,
                 int32_t* u, int32_t* v, int64_t* m_out) {
-  if (scale < 1 || scale > 30 || edge_factor < 1 || !u || !v || !m_out || a < 0 || b < 0 ||
-      c < 0 || a + b + c > 1.0) {
-    dc::set_error("dc_gen_rmat: invalid argument");
-    return DC_ERR_INVALID;
-  }
-  const int64_t samples = (int64_t)edge_factor << scale;
-  std::vector<uint64_t> keys(samples);
-  const double ab = a + b, abc = a + b + c;
-  parallel_for(samples, [&](int64_t lo, int64_t hi) {
-    for (int64_t i = lo; i < hi; ++i) {
-      Rng r(splitmix64(seed) + (uint64_t)i * 0xD1B54A32D192ED03ull);
-      uint32_t x = 0, y = 0;
-      for (int l = 0; l < scale; ++l) {
-        double p = r.unit();
-        uint32_t bx = p >= ab ? 1u : 0u;
-        uint32_t by = ((p >= a && p < ab) || p >= abc) ? 1u : 0u;
-        x = (x << 1) | bx;
-        y = (y << 1) | by;
-      }
-      keys[i] = x == y ? ~0ull : key_of(x, y);
+  return dc::guarded("dc_gen_rmat", [&]() -> int {
+    if (scale < 1 || scale > 30 || edge_factor < 1 || !u || !v || !m_out || a < 0 || b < 0 ||
+        c < 0 || a + b + c > 1.0) {
+      dc::set_error("dc_gen_rmat: invalid argument");
+      return DC_ERR_INVALID;
     }
+    const int64_t samples = (int64_t)edge_factor << scale;
+    std::vector<uint64_t> keys(samples);
+    const double ab = a + b, abc = a + b + c;
+    parallel_for(samples, [&](int64_t lo, int64_t hi) {
+      for (int64_t i = lo; i < hi; ++i) {
+        Rng r(splitmix64(seed) + (uint64_t)i * 0xD1B54A32D192ED03ull);
+        uint32_t x = 0, y = 0;
+        for (int l = 0; l < scale; ++l) {
+          double p = r.unit();
+          uint32_t bx = p >= ab ? 1u : 0u;
+          uint32_t by = ((p >= a && p < ab) || p >= abc) ? 1u : 0u;
+          x = (x << 1) | bx;
+          y = (y << 1) | by;
+        }
+        keys[i] = x == y ? ~0ull : key_of(x, y);
+      }
+    });
+    psort(keys);
+    int64_t m = 0;
+    for (int64_t i = 0; i < samples; ++i) {
+      if (keys[i] == ~0ull) break;
+      if (i > 0 && keys[i] == keys[i - 1]) continue;
+      u[m] = (int32_t)(keys[i] >> 32);
+      v[m] = (int32_t)(keys[i] & 0xffffffffu);
+      ++m;
+    }
+    *m_out = m;
+    return DC_OK;
   });
-  psort(keys);
-  int64_t m = 0;
-  for (int64_t i = 0; i < samples; ++i) {
-    if (keys[i] == ~0ull) break;
-    if (i > 0 && keys[i] == keys[i - 1]) continue;
-    u[m] = (int32_t)(keys[i] >> 32);
-    v[m] = (int32_t)(keys[i] & 0xffffffffu);
-    ++m;
-  }
-  *m_out = m;
-  return DC_OK;
 }
 
 int dc_gen_batches(int32_t n, int64_t m, const int32_t* u, const int32_t* v, int32_t nb, int64_t bs,
                    double ins_frac, uint64_t seed, int32_t* bu, int32_t* bv, uint8_t* bins) {
-  if (n < 2 || m < 0 || nb < 0 || bs < 0 || ins_frac < 0 || ins_frac > 1 ||
-      (m > 0 && (!u || !v)) || (nb > 0 && bs > 0 && (!bu || !bv || !bins))) {
-    dc::set_error("dc_gen_batches: invalid argument");
-    return DC_ERR_INVALID;
-  }
-  const int64_t n_ins = (int64_t)std::llround((double)bs * ins_frac);
-  const int64_t n_del = bs - n_ins;
-  std::vector<uint64_t> E((size_t)m);
-  for (int64_t i = 0; i < m; ++i) E[i] = key_of((uint32_t)u[i], (uint32_t)v[i]);
-  EdgeIndex idx((uint64_t)m + (uint64_t)nb * (uint64_t)n_ins + 16);
-  parallel_for(m, [&](int64_t lo, int64_t hi) {
-    for (int64_t i = lo; i < hi; ++i) idx.insert_concurrent(E[i], (uint32_t)i);
-  });
-  if ((int64_t)E.size() != m) return DC_ERR_INVALID;
-  Rng r(splitmix64(seed ^ 0xA0761D6478BD642Full));
-  std::vector<uint64_t> ins_keys, del_keys;
-  std::vector<uint64_t> batch_new;
-  for (int32_t bi = 0; bi < nb; ++bi) {
-    if (n_del > (int64_t)E.size()) {
-      dc::set_error("dc_gen_batches: not enough edges to delete");
+  return dc::guarded("dc_gen_batches", [&]() -> int {
+    if (n < 2 || m < 0 || nb < 0 || bs < 0 || ins_frac < 0 || ins_frac > 1 ||
+        (m > 0 && (!u || !v)) || (nb > 0 && bs > 0 && (!bu || !bv || !bins))) {
+      dc::set_error("dc_gen_batches: invalid argument");
       return DC_ERR_INVALID;
     }
-    // insertions: uniform pairs that are neither edges nor already chosen
-    ins_keys.clear();
-    EdgeIndex fresh((uint64_t)n_ins + 16);
-    while ((int64_t)ins_keys.size() < n_ins) {
-      uint32_t x = (uint32_t)r.below((uint64_t)n), y = (uint32_t)r.below((uint64_t)n);
-      if (x == y) continue;
-      uint64_t k = key_of(x, y);
-      if (idx.find(k) >= 0 || fresh.find(k) >= 0) continue;
-      fresh.insert(k, 0);
-      ins_keys.push_back(k);
-    }
-    // deletions: distinct edges of the pre-batch graph, uniformly
-    del_keys.clear();
-    for (int64_t d = 0; d < n_del; ++d) {
-      uint64_t p = r.below((uint64_t)E.size());
-      uint64_t k = E[p];
-      int64_t slot = idx.find(k);
-      // swap-remove from E, keep the index consistent
-      uint64_t last = E.back();
-      E[p] = last;
-      E.pop_back();
-      if (last != k) {
-        int64_t ls = idx.find(last);
-        idx.pos[ls] = (uint32_t)p;
+    const int64_t n_ins = (int64_t)std::llround((double)bs * ins_frac);
+    const int64_t n_del = bs - n_ins;
+    std::vector<uint64_t> E((size_t)m);
+    for (int64_t i = 0; i < m; ++i) E[i] = key_of((uint32_t)u[i], (uint32_t)v[i]);
+    EdgeIndex idx((uint64_t)m + (uint64_t)nb * (uint64_t)n_ins + 16);
+    parallel_for(m, [&](int64_t lo, int64_t hi) {
+      for (int64_t i = lo; i < hi; ++i) idx.insert_concurrent(E[i], (uint32_t)i);
+    });
+    if ((int64_t)E.size() != m) return DC_ERR_INVALID;
+    Rng r(splitmix64(seed ^ 0xA0761D6478BD642Full));
+    std::vector<uint64_t> ins_keys, del_keys;
+    std::vector<uint64_t> batch_new;
+    for (int32_t bi = 0; bi < nb; ++bi) {
+      if (n_del > (int64_t)E.size()) {
+        dc::set_error("dc_gen_batches: not enough edges to delete");
+        return DC_ERR_INVALID;
       }
-      idx.erase_at((uint64_t)slot);
-      del_keys.push_back(k);
+      // insertions: uniform pairs that are neither edges nor already chosen
+      ins_keys.clear();
+      EdgeIndex fresh((uint64_t)n_ins + 16);
+      while ((int64_t)ins_keys.size() < n_ins) {
+        uint32_t x = (uint32_t)r.below((uint64_t)n), y = (uint32_t)r.below((uint64_t)n);
+        if (x == y) continue;
+        uint64_t k = key_of(x, y);
+        if (idx.find(k) >= 0 || fresh.find(k) >= 0) continue;
+        fresh.insert(k, 0);
+        ins_keys.push_back(k);
+      }
+      // deletions: distinct edges of the pre-batch graph, uniformly
+      del_keys.clear();
+      for (int64_t d = 0; d < n_del; ++d) {
+        uint64_t p = r.below((uint64_t)E.size());
+        uint64_t k = E[p];
+        int64_t slot = idx.find(k);
+        // swap-remove from E, keep the index consistent
+        uint64_t last = E.back();
+        E[p] = last;
+        E.pop_back();
+        if (last != k) {
+          int64_t ls = idx.find(last);
+          idx.pos[ls] = (uint32_t)p;
+        }
+        idx.erase_at((uint64_t)slot);
+        del_keys.push_back(k);
+      }
+      for (uint64_t k : ins_keys) {
+        idx.insert(k, (uint32_t)E.size());
+        E.push_back(k);
+      }
+      int64_t o = (int64_t)bi * bs;
+      for (int64_t i = 0; i < n_del; ++i, ++o) {
+        bu[o] = (int32_t)(del_keys[i] >> 32);
+        bv[o] = (int32_t)(del_keys[i] & 0xffffffffu);
+        bins[o] = 0;
+      }
+      for (int64_t i = 0; i < n_ins; ++i, ++o) {
+        bu[o] = (int32_t)(ins_keys[i] >> 32);
+        bv[o] = (int32_t)(ins_keys[i] & 0xffffffffu);
+        bins[o] = 1;
+      }
     }
-    for (uint64_t k : ins_keys) {
-      idx.insert(k, (uint32_t)E.size());
-      E.push_back(k);
-    }
-    int64_t o = (int64_t)bi * bs;
-    for (int64_t i = 0; i < n_del; ++i, ++o) {
-      bu[o] = (int32_t)(del_keys[i] >> 32);
-      bv[o] = (int32_t)(del_keys[i] & 0xffffffffu);
-      bins[o] = 0;
-    }
-    for (int64_t i = 0; i < n_ins; ++i, ++o) {
-      bu[o] = (int32_t)(ins_keys[i] >> 32);
-      bv[o] = (int32_t)(ins_keys[i] & 0xffffffffu);
-      bins[o] = 1;
-    }
-  }
-  return DC_OK;
+    return DC_OK;
+  });
 }
 
 }  // extern "C"

@@ -103,7 +103,22 @@ struct LoopCtl {
   int32_t unsettled;    // a check found a vertex off its mex
   int32_t settled;      // the loop stopped at settled colours (done_round = last round run)
   int32_t newly_left;   // parts left the loop at the latest part check
+  // members of C_r whose part left after round nl_sub_round (= r): round
+  // r + 1's L is C_r without them
+  int32_t nl_sub;
+  int32_t nl_sub_round;
+  // rounds by count-pass path: pull recounts, pushes whose D rows exceeded
+  // heavy_slots, and those of them that listed R_r
+  int32_t pull_rounds;
+  int32_t heavy_push_rounds;
+  int32_t heavy_push_listed;
 };
+
+// |L_r| (uniform: every workgroup reads the same words)
+__device__ __forceinline__ int32_t loop_nl(const LoopCtl* c, const int4* trace, int r) {
+  if (r == 0) return c->nl0;
+  return trace[r - 1].y - (c->nl_sub_round == r - 1 ? c->nl_sub : 0);
+}
 
 struct LoopState {
   GraphView G;
@@ -122,6 +137,10 @@ struct LoopState {
   const int32_t* hrowv;  // rows with a colour histogram (capacity >= kSmallLen)
   uint32_t n_hrows;
   unsigned long long heavy_slots;  // D_r rows beyond this many slots: pull recount
+  int32_t pull_mode;   // 0: by heavy_slots; 1: every round pulls; -1: no round pulls
+  // per-batch work terms of the reference's algorithm (null: not counted):
+  // {sum over rounds of |L_r u C_r|, of the row slots of L_r u C_r}
+  unsigned long long* terms;
   // phase A's totals by round parity, spread over kAcc slots so that no
   // address takes more than gridDim / kAcc atomics: {members, changes, slots
   // of the rows put into D, hub rows put into D, histogram rows below hub
@@ -189,6 +208,7 @@ struct IncState {
   LoopCtl* hctl = nullptr;
   unsigned long long* hstat = nullptr;
   int32_t* tqbuf = nullptr;     // tq[0], tq[1]
+  unsigned long long* terms = nullptr;   // LoopState::terms (device, 4 words)
 };
 
 // Membership flags of one vertex in a state array (one gather tells a
@@ -568,8 +588,9 @@ __device__ __forceinline__ void round_a(const LoopState& S, int r, LoopLds& lds)
   uint32_t members = 0, changed = 0, dhubs = 0, dmids = 0;
   unsigned long long slots = 0, dsum = 0;
   // rounds with a small L list their sets (uniform: the previous round's |C|)
-  const int32_t nl = r == 0 ? S.ctl->nl0 : S.trace[r - 1].y;
+  const int32_t nl = loop_nl(S.ctl, S.trace, r);
   const bool listing = nl <= S.list_c;
+  unsigned long long lc_n = 0, lc_deg = 0;   // terms: |L_r u C_r| and its row slots
   uint32_t* qd = S.qc + kQD + 3 * (r & 1);
   // colour changes of this round by part (parts set)
   uint32_t* pch = S.nparts ? S.pchg + (size_t)(r & 1) * S.nparts : nullptr;
@@ -613,6 +634,10 @@ __device__ __forceinline__ void round_a(const LoopState& S, int r, LoopLds& lds)
       }
       const bool member = valid && k > 0;
       const bool leaving = valid && !member && inl;
+      if (S.terms && (member || inl)) {
+        lc_n += 1;
+        lc_deg += G.row[w].y;
+      }
       // w gets its first flag of the round (C or D) in this phase
       if (listing)
         wave_append_capped(member || leaving, w, S.tq[r & 1], S.qc + kQT + (r & 3), S.tq_cap);
@@ -717,6 +742,10 @@ __device__ __forceinline__ void round_a(const LoopState& S, int r, LoopLds& lds)
     if (bmid) atomicAdd(&a[4], (unsigned long long)bmid);
   }
   if (lane == 0 && slots) atomicAdd(&S.bstat[blockIdx.x * 4 + 0], slots);
+  if (S.terms) {
+    wave_add_u64(lc_n, &S.terms[0]);
+    wave_add_u64(lc_deg, &S.terms[1]);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1040,13 +1069,17 @@ __device__ __forceinline__ void round_c(const LoopState& S, int r, LoopLds& lds)
   const bool nonempty = nmem != 0ull;
   const bool dhubs = lds.tot[3] != 0ull;
   const bool dmids = lds.tot[4] != 0ull;
-  const int32_t nl = r == 0 ? S.ctl->nl0 : S.trace[r - 1].y;
+  const int32_t nl = loop_nl(S.ctl, S.trace, r);
   // pull recount when D's rows are many slots and C more than doubled (a push
   // would spend an atomic on most new marking pairs), or when D's rows are
   // half of all slots; otherwise push deltas (on R-MAT 22 the first round's
-  // D is a quarter of the slots with few term changes: push 0.8 ms faster)
-  const bool heavy = lds.tot[2] > S.heavy_slots &&
-                     (nmem > 2ull * (unsigned long long)nl || lds.tot[2] > 4ull * S.heavy_slots);
+  // D is a quarter of the slots with few term changes: push 0.8 ms faster).
+  // pull_mode forces either path (tests).
+  const bool big = lds.tot[2] > S.heavy_slots;
+  const bool heavy =
+      S.pull_mode > 0 ||
+      (S.pull_mode == 0 && big &&
+       (nmem > 2ull * (unsigned long long)nl || lds.tot[2] > 4ull * S.heavy_slots));
   // phase A listed round r's sets (the same test as there); R_r joins the
   // touched list in light rounds with few D slots
   const bool listed = nl <= S.list_c;
@@ -1061,6 +1094,11 @@ __device__ __forceinline__ void round_c(const LoopState& S, int r, LoopLds& lds)
       S.ctl->last_nc = (int32_t)nmem;
       S.ctl->last_changed = (int32_t)lds.tot[1];
       if (!nonempty) S.ctl->done_round = r;
+      if (nonempty && heavy) S.ctl->pull_rounds += 1;
+      if (nonempty && !heavy && big) {
+        S.ctl->heavy_push_rounds += 1;
+        if (listing) S.ctl->heavy_push_listed += 1;
+      }
       // round r+1's list counts start at zero; a round that does not list
       // R_r has no usable touched list (no block reads that count here)
       S.qc[kQT + ((r + 1) & 3)] = 0u;
@@ -1307,7 +1345,11 @@ __global__ __launch_bounds__(kBlock) void k_part_check_if(LoopState S, int32_t r
 
 __global__ __launch_bounds__(1024) void k_part_decide(LoopState S, int32_t r) {
   if (S.ctl->done_round != INT_MAX) return;
-  if (threadIdx.x == 0) S.ctl->newly_left = 0;
+  if (threadIdx.x == 0) {
+    S.ctl->newly_left = 0;
+    S.ctl->nl_sub = 0;
+    S.ctl->nl_sub_round = r;
+  }
   __syncthreads();
   const uint32_t* pc = S.pchg + (size_t)(r & 1) * S.nparts;
   uint32_t left = 0;
@@ -1329,10 +1371,14 @@ __global__ __launch_bounds__(1024) void k_part_decide(LoopState S, int32_t r) {
   }
 }
 
-__global__ __launch_bounds__(kBlock) void k_part_clear(LoopState S) {
+__global__ __launch_bounds__(kBlock) void k_part_clear(LoopState S, int32_t r) {
   if (S.ctl->newly_left == 0) return;
+  // members of C_r taken out: round r + 1's L does not hold them
+  const uint32_t* cr = S.St[r & 1];
+  uint32_t gone = 0;
   for (uint32_t w = blockIdx.x * blockDim.x + threadIdx.x; w < S.swords; w += gridDim.x * blockDim.x) {
     const uint32_t a = S.St[0][w], b = S.St[1][w];
+    const uint32_t c = cr[w];
     uint32_t clear = 0u;
     for (int i = 0; i < 8; ++i) {
       const uint32_t f = ((a | b) >> (4 * i)) & 15u;
@@ -1340,6 +1386,7 @@ __global__ __launch_bounds__(kBlock) void k_part_clear(LoopState S) {
       if (f && v < S.G.n_owned && S.pret[S.vpart[v]] == 2u) {
         S.K[v] = 0;
         clear |= 15u << (4 * i);
+        gone += (c >> (4 * i)) & kFC;
       }
     }
     if (clear) {
@@ -1347,6 +1394,7 @@ __global__ __launch_bounds__(kBlock) void k_part_clear(LoopState S) {
       S.St[1][w] = b & ~clear;
     }
   }
+  wave_add_u32(gone, reinterpret_cast<uint32_t*>(&S.ctl->nl_sub));
 }
 
 __global__ void k_settle_decide(LoopState S, int32_t r) {
@@ -1400,6 +1448,7 @@ static int inc_alloc(Graph* g, IncState* s) {
   S.dq_h = nullptr;
   S.dq_m_cap = S.dq_h_cap = 0;
   DC_HIP(hipMalloc(&S.qc, sizeof(uint32_t) * kQWords));
+  DC_HIP(hipMalloc(&s->terms, sizeof(unsigned long long) * 4));
   DC_HIP(hipHostMalloc(&s->hctl, sizeof(LoopCtl), hipHostMallocDefault));
   s->hstat = new unsigned long long[4 * kMaxBlocks + 1];
   DC_HIP(hipMemsetAsync(S.K, 0, sizeof(int32_t) * n, g->stream));
@@ -1432,7 +1481,7 @@ void loop_free(Graph* g) {
     if (p) (void)hipFree(p);
   };
   f(s->S.K); f(s->S.dold); f(s->bits); f(s->S.acc); f(s->S.bstat); f(s->S.trace);
-  f(s->S.ctl); f(s->tqbuf); f(s->S.dq_s); f(s->S.dq_m); f(s->S.dq_h); f(s->S.qc);
+  f(s->S.ctl); f(s->tqbuf); f(s->S.dq_s); f(s->S.dq_m); f(s->S.dq_h); f(s->S.qc); f(s->terms);
   if (s->hctl) (void)hipHostFree(s->hctl);
   for (hipEvent_t e : s->cev) (void)hipEventDestroy(e);
   delete[] s->hstat;
@@ -1553,7 +1602,13 @@ int loop_hi(Graph* g, Frontier L0, int32_t nl0, bool persistent, LoopResult* res
   }
   // rounds whose D rows hold more than 1/8 of all slots (measured on R-MAT
   // 22 against 1/2 and 1/32) recount by pull when C also more than doubled
-  S.heavy_slots = std::max<unsigned long long>(g->rows_end / 8, 1ull << 16);
+  S.heavy_slots = g->opt.heavy_slots > 0
+                      ? (unsigned long long)g->opt.heavy_slots
+                      : std::max<unsigned long long>(g->rows_end / 8, 1ull << 16);
+  S.pull_mode = g->opt.pull_mode > 0 ? 1 : g->opt.pull_mode < 0 ? -1 : 0;
+  // the reference's work terms are counted in profiled batches only (one
+  // more row-bounds gather per candidate)
+  S.terms = g->profiling ? s->terms : nullptr;
   S.list_c = (int32_t)std::min<uint64_t>(
       kListC, ((uint64_t)S.swords + g->n_hrows + g->n_hch) / 64);
   S.list_slots = kListSlots;
@@ -1566,9 +1621,11 @@ int loop_hi(Graph* g, Frontier L0, int32_t nl0, bool persistent, LoopResult* res
   s->hctl->done_round = INT_MAX;
   s->hctl->nl0 = nl0;
   s->hctl->checks_left = g->opt.exact_rounds ? 0 : 3;
+  s->hctl->nl_sub_round = -1;
   DC_HIP(hipMemcpyAsync(S.ctl, s->hctl, sizeof(LoopCtl), hipMemcpyHostToDevice, g->stream));
   DC_HIP(hipMemsetAsync(S.bstat, 0, sizeof(unsigned long long) * (4 * kMaxBlocks + 1), g->stream));
   DC_HIP(hipMemsetAsync(S.acc, 0, sizeof(unsigned long long) * 2 * kAcc * kAccW, g->stream));
+  if (S.terms) DC_HIP(hipMemsetAsync(S.terms, 0, sizeof(unsigned long long) * 4, g->stream));
   const dim3 blk(kBlock);
   hipLaunchKernelGGL(k_loop_init, dim3(kMaxBlocks), blk, 0, g->stream, S);
   DC_HIP(hipGetLastError());
@@ -1628,7 +1685,7 @@ int loop_hi(Graph* g, Frontier L0, int32_t nl0, bool persistent, LoopResult* res
             hipLaunchKernelGGL(k_part_decide, dim3(1), dim3(1024), 0, g->stream, S, r + i);
             hipLaunchKernelGGL(k_part_clear,
                                dim3((uint32_t)std::min<uint64_t>(2048, (S.swords + kBlock - 1) / kBlock)),
-                               blk, 0, g->stream, S);
+                               blk, 0, g->stream, S, r + i);
             launches += 3;
           }
         }
@@ -1691,7 +1748,15 @@ int loop_hi(Graph* g, Frontier L0, int32_t nl0, bool persistent, LoopResult* res
                         g->stream));
   DC_HIP(hipMemcpyAsync(s->hstat, S.bstat, sizeof(unsigned long long) * (4 * kMaxBlocks + 1),
                         hipMemcpyDeviceToHost, g->stream));
+  unsigned long long terms[4] = {0ull, 0ull, 0ull, 0ull};
+  if (S.terms)
+    DC_HIP(hipMemcpyAsync(terms, S.terms, sizeof(terms), hipMemcpyDeviceToHost, g->stream));
   DC_HIP(hipStreamSynchronize(g->stream));
+  res->pull_rounds = s->hctl->pull_rounds;
+  res->heavy_push_rounds = s->hctl->heavy_push_rounds;
+  res->heavy_push_listed = s->hctl->heavy_push_listed;
+  res->lc_vertices = terms[0];
+  res->lc_slots = terms[1];
   res->rounds = rounds;
   res->settled = settled;
   for (int32_t i = 0; i < rounds; ++i) {

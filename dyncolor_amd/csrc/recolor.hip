@@ -462,6 +462,9 @@ int recolor_plain(Graph* g, int rule, dc_stats* st, bool persistent_loop) {
     }
   }
   uint64_t nL = fsize(pre_h[kPreSlots - 1].cnt);
+  // the tombstone pass's row-slot sum stays in the counters through the
+  // pre-loop rounds (no round resets it)
+  const int64_t del_row_slots = npre > 0 ? (int64_t)pre_h[npre - 1].c.delslots : 0;
   if (rule == DC_RULE_INDEPENDENT && change) {
     // candidate counts kept incrementally (loop.hip); both colour buffers
     // are written there, so no catch-up follows
@@ -486,6 +489,12 @@ int recolor_plain(Graph* g, int rule, dc_stats* st, bool persistent_loop) {
       st->phases = (int32_t)(g->trace.size() / 4);
       st->loop_rounds = lr.rounds;
       st->loop_settled = lr.settled ? 1 : 0;
+      st->pull_rounds = lr.pull_rounds;
+      st->heavy_push_rounds = lr.heavy_push_rounds;
+      st->heavy_push_listed = lr.heavy_push_listed;
+      st->lc_vertices = (int64_t)lr.lc_vertices;
+      st->lc_slots = (int64_t)lr.lc_slots;
+      st->del_row_slots = g->profiling ? del_row_slots : 0;
       st->evaluated = (int64_t)evaluated;
       st->changed = (int64_t)changed_total;
       st->frontier_sum = (int64_t)frontier_sum;
@@ -567,6 +576,7 @@ int recolor_plain(Graph* g, int rule, dc_stats* st, bool persistent_loop) {
     st->ms_recolor = ms;
     st->ms_recolor_kernel = kernel_ms;
     st->recolor_launches = launches;
+    st->del_row_slots = g->profiling ? del_row_slots : 0;
   }
   return DC_OK;
 }

@@ -70,6 +70,16 @@ typedef struct {
                           131072, and fewer on small graphs); -1: never; > 0 also lists
                           R in every round that is not a pull round.  Results do not
                           depend on it.                                             */
+  int32_t pull_mode;   /* independent-rule loop, how a round's candidate counts are
+                          brought up to date: 0 (default) pushes count changes from
+                          the rows that changed state and recounts every count by a
+                          pull over all rows in rounds whose changed rows hold more
+                          than heavy_slots slots and whose candidate set more than
+                          doubled; 1: every round pulls; -1: no round pulls.
+                          Results do not depend on it.                               */
+  int32_t reserved0;
+  int64_t heavy_slots; /* the pull threshold in row slots; 0 (default): 1/8 of the
+                          laid-out row slots, at least 65536                         */
 } dc_options;
 
 /* Per-batch counters.  Byte counts follow the algorithmic model: each visited
@@ -96,6 +106,18 @@ typedef struct {
   int64_t count_pass_bytes; /* algorithmic bytes of those launches            */
   int32_t count_pass_launches;
   int32_t loop_settled;     /* 1 if the loop stopped at settled colours       */
+  int32_t pull_rounds;      /* loop rounds whose counts were recounted by pull */
+  int32_t heavy_push_rounds;  /* loop rounds that pushed count changes from more
+                               than heavy_slots changed row slots             */
+  int32_t heavy_push_listed;  /* ... and took their next candidates from lists */
+  int32_t reserved0;
+  /* Work terms of the reference's algorithm for this batch (profiling on;
+   * zero otherwise): the reference rescans every row of L_r and C_r each loop
+   * round (HI/gpuFunctions_undir.cuh:229-284) and both endpoint rows per
+   * deletion (HI/gpuFunctions_undir.cuh:118-142). */
+  int64_t lc_vertices;      /* sum over loop rounds of |L_r u C_r|            */
+  int64_t lc_slots;         /* sum over loop rounds of the row slots of L_r u C_r */
+  int64_t del_row_slots;    /* sum over deletion lines of both rows' slots    */
 } dc_stats;
 
 /* ---- graph lifetime ------------------------------------------------------ */

@@ -156,7 +156,7 @@ static int64_t distinct_colors(orc_graph* g, const int32_t* color, int32_t t) {
   }
   int64_t k = 0;
   FOR_LIVE(g, t, w, { g->nbuf[k++] = color[w]; });
-  qsort(g->nbuf, (size_t)k, sizeof(int32_t), cmp_i32);
+  if (k > 1) qsort(g->nbuf, (size_t)k, sizeof(int32_t), cmp_i32);
   int64_t distinct = 0;
   for (int64_t i = 0; i < k; ++i)
     if (i == 0 || g->nbuf[i] != g->nbuf[i - 1]) ++distinct;

@@ -113,3 +113,27 @@ def test_header_documents_reference_citations():
         block = src[: src.index(sym + "(")]
         last = block[block.rfind("/*"):]
         assert re.search(r"\.(cuh|cu|h):\d+", last), sym
+
+
+def test_struct_layouts_match_header(tmp_path):
+    # the ctypes mirrors of dc_options, dc_stats and dc_serve_stats against
+    # the C header: size and every field offset
+    structs = {"dc_options": d.Options, "dc_stats": d.Stats, "dc_serve_stats": d.ServeStats}
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "dyncolor.h"', "int main(void) {"]
+    for cname, py in structs.items():
+        lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')
+        for f, _ in py._fields_:
+            lines.append(f'  printf("{cname} {f} %zu\\n", offsetof({cname}, {f}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = str(tmp_path / "layout")
+    import subprocess
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"), "-o", exe, str(src)],
+                   check=True)
+    out = subprocess.run([exe], check=True, capture_output=True, text=True).stdout.split("\n")
+    for line in filter(None, out):
+        cname, field, val = line.split()
+        py = structs[cname]
+        want = C.sizeof(py) if field == "size" else getattr(py, field).offset
+        assert int(val) == want, (cname, field, int(val), want)

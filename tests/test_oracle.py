@@ -241,3 +241,23 @@ def test_independent_rule_converges_on_er_c1(oracle_mod):
         ph = g.recolor(rule, max_loop=5000)
         assert g.validate()[0] == 0
         assert len(ph) < 500
+
+
+def test_partition_sample_fixture(oracle_mod):
+    # hand-derived partition case (tests/golden/partition_sample.json): both
+    # cross kinds, a cross deletion skipped for a ghost with the larger
+    # colour, a cross insertion recolouring the owned endpoint, three loop
+    # rounds with frozen ghost colours in every mex
+    import json
+    import os
+    p = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "partition_sample.json")))
+    e = np.array(p["edges"], np.int32)
+    ch = np.array(p["changes"], np.int32)
+    n = p["n_owned"] + p["n_ghost"]
+    og = oracle_mod.OracleGraph(n, e[:, 0], e[:, 1], n_owned=p["n_owned"])
+    og.set_colors(p["colors"])
+    og.apply_batch(ch[:, 0], ch[:, 1], ch[:, 2].astype(np.uint8))
+    tr = og.recolor(0)
+    assert og.color.tolist() == p["expected"]["colors"]
+    assert tr.tolist() == p["expected"]["trace"]
+    assert og.validate() == (p["expected"]["monochromatic"], p["expected"]["max_color"])
