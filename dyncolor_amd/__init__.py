@@ -34,7 +34,7 @@ EXPORTS = [
     "dc_graph_info", "dc_colors_set", "dc_colors_get", "dc_static_color", "dc_apply_batch",
     "dc_batch_create", "dc_batch_destroy", "dc_apply_device_batch", "dc_recolor",
     "dc_process_batch", "dc_validate", "dc_trace_get", "dc_neighbors", "dc_prop_get",
-    "dc_compact", "dc_check_consistency", "dc_set_profiling", "dc_set_driver", "dc_set_parts", "dc_synchronize", "dc_last_error", "dc_device_count",
+    "dc_compact", "dc_compact_noalloc", "dc_check_consistency", "dc_set_profiling", "dc_set_driver", "dc_set_parts", "dc_synchronize", "dc_last_error", "dc_device_count",
     "dc_gen_er", "dc_gen_rmat", "dc_gen_batches",
     "dc_server_create", "dc_server_destroy", "dc_server_run",
 ]
@@ -117,6 +117,7 @@ def lib():
         "dc_neighbors": (C.c_int, [_vp, C.c_int32, _i32p, C.c_int64, C.POINTER(C.c_int64)]),
         "dc_prop_get": (C.c_int, [_vp, _i32p]),
         "dc_compact": (C.c_int, [_vp]),
+        "dc_compact_noalloc": (C.c_int, [_vp]),
         "dc_check_consistency": (C.c_int, [_vp, C.POINTER(C.c_int64)]),
         "dc_set_profiling": (C.c_int, [_vp, C.c_int32]),
         "dc_set_driver": (C.c_int, [_vp, C.c_int32, C.c_int32]),
@@ -304,6 +305,18 @@ class Graph:
     def compact(self):
         _check(lib().dc_compact(self._h))
 
+    def compact_noalloc(self):
+        """dc_compact_noalloc: the same compaction with its slot map in the
+        library's static device array (no device allocation)."""
+        _check(lib().dc_compact_noalloc(self._h))
+
+    def info(self):
+        n = C.c_int32(0)
+        no = C.c_int32(0)
+        live = C.c_int64(0)
+        cap = C.c_int64(0)
+        _check(lib().dc_graph_info(self._h, C.byref(n), C.byref(no), C.byref(live), C.byref(cap)))
+        return {"n": n.value, "n_owned": no.value, "capacity": cap.value}
 
     def check_consistency(self) -> int:
         bad = C.c_int64(0)

@@ -482,6 +482,19 @@ int dc_compact(dc_graph* h) {
   });
 }
 
+int dc_compact_noalloc(dc_graph* h) {
+  return dc::guarded("dc_compact_noalloc", [&]() -> int {
+    if (!h) return DC_ERR_INVALID;
+    Graph* g = G_(h);
+    if (g->staged) {
+      dc::set_error("dc_compact_noalloc: a batch is staged");
+      return DC_ERR_STATE;
+    }
+    DC_HIP(hipSetDevice(g->device));
+    return dc::csr_compact_noalloc(g);
+  });
+}
+
 int dc_set_driver(dc_graph* h, int32_t driver, int32_t blocks_per_cu) {
   return dc::guarded("dc_set_driver", [&]() -> int {
     if (!h || (driver != DC_DRIVER_PLAIN && driver != DC_DRIVER_FUSED) || blocks_per_cu < 0)

@@ -12,6 +12,8 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
+#include <string>
 
 #include "dc_device.h"
 #include "dc_internal.h"
@@ -1199,17 +1201,65 @@ __global__ void k_remap_twins(GraphView G, const int32_t* newpos) {
   }
 }
 
-int csr_compact(Graph* g) {
-  int32_t* newpos = nullptr;
-  DC_HIP(hipMalloc(&newpos, sizeof(int32_t) * (size_t)(g->total_cap + 1)));
+static int compact_with(Graph* g, int32_t* newpos) {
   const dim3 grid(grid_for((int64_t)g->n_owned * 64));
   hipLaunchKernelGGL(k_compact_rows, grid, dim3(kBlock), 0, g->stream, g->view(), newpos);
   hipLaunchKernelGGL(k_remap_twins, grid, dim3(kBlock), 0, g->stream, g->view(), newpos);
   hipError_t e = hipGetLastError();
-  (void)hipStreamSynchronize(g->stream);
-  (void)hipFree(newpos);
+  hipError_t s = hipStreamSynchronize(g->stream);
   if (e != hipSuccess) return hip_fail(e, "compact");
+  if (s != hipSuccess) return hip_fail(s, "compact");
   return DC_OK;
+}
+
+int csr_compact(Graph* g) {
+  int32_t* newpos = nullptr;
+  DC_HIP(hipMalloc(&newpos, sizeof(int32_t) * (size_t)(g->total_cap + 1)));
+  const int rc = compact_with(g, newpos);
+  (void)hipFree(newpos);
+  return rc;
+}
+
+// ---- compaction without allocation -------------------------------------------
+// The slot map of the compaction (old slot -> new position) in a file-scope
+// device array instead of a per-call allocation, for callers that cannot
+// allocate device memory.  Each GPU holds its own copy for the life of the
+// process; every call on that GPU shares it, so calls that overlap on one GPU
+// race.  Nothing in it needs to start at zero: k_compact_rows writes the entry
+// of every used slot of every owned row before k_remap_twins reads any (a
+// twin always points at a used slot of an owned row).
+constexpr uint64_t kScratchSlots = 1ull << 23;   // 32 MiB of int32
+static __device__ __align__(256) int32_t scratch[kScratchSlots];
+constexpr int kMaxDevices = 64;
+static std::mutex scratch_mu;
+static int32_t* scratch_dev[kMaxDevices] = {};   // the array's address on each device
+
+uint64_t compact_noalloc_max_slots() { return kScratchSlots - 1; }
+
+int csr_compact_noalloc(Graph* g) {
+  if (g->total_cap + 1 > kScratchSlots) {
+    set_error("dc_compact_noalloc: the graph has " + std::to_string(g->total_cap) +
+              " row slots; the static slot map holds at most " +
+              std::to_string(kScratchSlots - 1));
+    return DC_ERR_INVALID;
+  }
+  if (g->device < 0 || g->device >= kMaxDevices) {
+    set_error("dc_compact_noalloc: device ordinal out of range");
+    return DC_ERR_INVALID;
+  }
+  int32_t* ptr = nullptr;
+  {
+    // once per device, with that device current (the caller set it): the
+    // host-side name of the array is not a device address
+    std::lock_guard<std::mutex> lk(scratch_mu);
+    if (!scratch_dev[g->device]) {
+      void* p = nullptr;
+      DC_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+      scratch_dev[g->device] = static_cast<int32_t*>(p);
+    }
+    ptr = scratch_dev[g->device];
+  }
+  return compact_with(g, ptr);
 }
 
 }  // namespace dc

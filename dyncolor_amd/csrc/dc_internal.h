@@ -213,6 +213,8 @@ int hip_fail(hipError_t e, const char* where);
 int csr_build(Graph* g, int64_t m, const int32_t* du, const int32_t* dv);
 int csr_apply(Graph* g, const Batch& b, int rule);
 int csr_compact(Graph* g);
+int csr_compact_noalloc(Graph* g);   // dc_compact_noalloc: static slot map
+uint64_t compact_noalloc_max_slots();
 int rebuild_ccol(Graph* g);               // ccol[j] <- colour[col[j]] for every slot (+ hub histograms)
 int hist_reindex(Graph* g);               // hub rows from cap[], then rebuild the histograms
 int parts_set(Graph* g, int32_t nparts, const int32_t* offsets);   // nparts 0 clears

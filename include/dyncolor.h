@@ -212,6 +212,18 @@ int dc_prop_get(dc_graph* g, int32_t* prop);
 /* Compact tombstoned row slots (keeps spare capacity). */
 int dc_compact(dc_graph* g);
 
+/* dc_compact without any device allocation: the slot map of the compaction
+ * (4 bytes per row slot, which dc_compact allocates on every call) lives in a
+ * static device array of 2^23 entries held by each GPU the library runs on, so
+ * graphs of up to 2^23 - 1 row slots (dc_graph_info's capacity, spare tail
+ * included) are accepted and larger ones rejected with DC_ERR_INVALID.  Calls
+ * that overlap on one GPU share the array and race; the caller serialises
+ * them.  No counterpart in the reference, which rebuilds its CSR from text on
+ * every run (HB/supportingFunctions.cu:27-45): it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops that
+ * forbid allocation). */
+int dc_compact_noalloc(dc_graph* g);
+
 /* Count row slots whose cached neighbour colour or reverse-slot link is
  * stale (0 when the device layout is consistent); for tests.  No counterpart
  * in the reference, whose rows hold only {neighbour, flag}
