@@ -37,6 +37,8 @@ EXPORTS = [
     "dc_compact", "dc_compact_noalloc", "dc_check_consistency", "dc_set_profiling", "dc_set_driver", "dc_set_parts", "dc_synchronize", "dc_last_error", "dc_device_count",
     "dc_gen_er", "dc_gen_rmat", "dc_gen_batches",
     "dc_server_create", "dc_server_destroy", "dc_server_run",
+    "dc_partition_plan_create", "dc_partition_plan_info", "dc_partition_plan_arrays",
+    "dc_partition_plan_border", "dc_partition_create_global", "dc_partition_plan_destroy",
 ]
 
 
@@ -72,6 +74,18 @@ class ServeStats(C.Structure):
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_ if not k.startswith("reserved")}
+
+
+class PartitionInfo(C.Structure):
+    _fields_ = [("n_global", C.c_int32), ("rank", C.c_int32), ("n_owned", C.c_int32),
+                ("n_ghost", C.c_int32), ("n_border", C.c_int32), ("n_parts", C.c_int32),
+                ("m_local", C.c_int64), ("m_internal", C.c_int64), ("m_cross", C.c_int64),
+                ("m_dropped", C.c_int64), ("k_local", C.c_int64), ("k_del", C.c_int64),
+                ("k_ins", C.c_int64), ("k_xdel", C.c_int64), ("k_xins", C.c_int64),
+                ("k_dropped", C.c_int64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class DynColorError(RuntimeError):
@@ -131,6 +145,13 @@ def lib():
         "dc_gen_batches": (C.c_int, [C.c_int32, C.c_int64, _vp, _vp, C.c_int32, C.c_int64,
                                      C.c_double, C.c_uint64, _i32p, _i32p, _u8p]),
         "dc_server_create": (C.c_int, [_vp, C.c_int32, _i32p, C.c_int64, C.c_int32, C.POINTER(_vp)]),
+        "dc_partition_plan_create": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, _vp, _vp, C.c_int64,
+                                               _vp, _vp, C.c_int64, _vp, _vp, _vp, C.POINTER(_vp)]),
+        "dc_partition_plan_info": (C.c_int, [_vp, C.POINTER(PartitionInfo)]),
+        "dc_partition_plan_arrays": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+        "dc_partition_plan_border": (C.c_int, [_vp, _i64p, C.c_int32]),
+        "dc_partition_create_global": (C.c_int, [_vp, _i32p, C.POINTER(Options), C.POINTER(_vp)]),
+        "dc_partition_plan_destroy": (C.c_int, [_vp]),
         "dc_server_destroy": (C.c_int, [_vp]),
         "dc_server_run": (C.c_int, [_vp, C.c_int64, _i32p, _i64p, _vp, _vp, _vp,
                                     np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS"),
@@ -239,6 +260,14 @@ class Graph:
     def __exit__(self, *a):
         self.close()
 
+    @classmethod
+    def _wrap(cls, h, n, n_owned):
+        g = cls.__new__(cls)
+        g._h = h
+        g.n = n
+        g.n_owned = n_owned
+        return g
+
     @property
     def handle(self):
         return self._h
@@ -337,6 +366,62 @@ class Graph:
 
     def synchronize(self):
         _check(lib().dc_synchronize(self._h))
+
+
+class PartitionPlan:
+    """One rank's share of a partitioned graph from global IDs
+    (dc_partition_plan_*; host only until create_graph)."""
+
+    def __init__(self, n_global, rank, owned_ids, part_of, u, v, cu, cv, cins):
+        owned = _arr(owned_ids, np.int32)
+        part = None if part_of is None else _arr(part_of, np.int32)
+        u, v = _arr(u, np.int32), _arr(v, np.int32)
+        cu, cv, cins = _arr(cu, np.int32), _arr(cv, np.int32), _arr(cins, np.uint8)
+        h = C.c_void_p()
+        _check(lib().dc_partition_plan_create(n_global, rank, owned.size, _ptr(owned), _ptr(part),
+                                              u.size, _ptr(u), _ptr(v), cu.size, _ptr(cu), _ptr(cv),
+                                              _ptr(cins), C.byref(h)))
+        self._h = h
+        inf = PartitionInfo()
+        _check(lib().dc_partition_plan_info(h, C.byref(inf)))
+        self.info = inf.as_dict()
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().dc_partition_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def arrays(self):
+        """(local_to_global, u, v, bu, bv, bins) in local IDs."""
+        i = self.info
+        l2g = np.zeros(max(i["n_owned"] + i["n_ghost"], 1), np.int32)
+        u = np.zeros(max(i["m_local"], 1), np.int32)
+        v = np.zeros_like(u)
+        bu = np.zeros(max(i["k_local"], 1), np.int32)
+        bv = np.zeros_like(bu)
+        bi = np.zeros(max(i["k_local"], 1), np.uint8)
+        _check(lib().dc_partition_plan_arrays(self._h, _ptr(l2g), _ptr(u), _ptr(v), _ptr(bu),
+                                              _ptr(bv), _ptr(bi)))
+        n, m, k = i["n_owned"] + i["n_ghost"], i["m_local"], i["k_local"]
+        return l2g[:n], u[:m], v[:m], bu[:k], bv[:k], bi[:k]
+
+    def border(self):
+        out = np.zeros(max(self.info["n_parts"], 1), np.int64)
+        _check(lib().dc_partition_plan_border(self._h, out, self.info["n_parts"]))
+        return out[: self.info["n_parts"]]
+
+    def create_graph(self, colors_global, opts: Options | None = None) -> "Graph":
+        h = C.c_void_p()
+        o = C.byref(opts) if opts is not None else None
+        _check(lib().dc_partition_create_global(self._h, _arr(colors_global, np.int32), o,
+                                                C.byref(h)))
+        return Graph._wrap(h, self.info["n_owned"] + self.info["n_ghost"], self.info["n_owned"])
 
 
 class DeviceBatch:

@@ -141,6 +141,55 @@ int dc_partition_create(int32_t n_owned, int32_t n_ghost, int64_t m,
                         const int32_t* ghost_colors, const dc_options* opt,
                         dc_graph** out);
 
+/* ---- partition ingest in global IDs ---------------------------------------
+ * The multi-GPU variant's inputs, for one rank: owned_ids = the rank's
+ * partition file (one global ID per line; line i becomes local ID i,
+ * read_Partition_Vertices, MG/Supporting.h:29-46), part_of = the part of
+ * every global vertex (partitionAll.txt, read_PartitionID_AllVertices,
+ * MG/Supporting.h:49-63; may be NULL), the whole graph's edges and the whole
+ * change file in global IDs.  The plan keeps what the rank holds, as
+ * read_graphEdges and readin_changes do (MG/all_structure_undir.cuh:229-312,
+ * 78-192): edges with an owned endpoint (internal ones, then cross ones as
+ * (owned, other)), the other parts' vertices they name as ghosts with local
+ * IDs n_owned, n_owned + 1, ... in order of first appearance (graph lines,
+ * then change lines), and the change lines with an owned endpoint in file
+ * order (cross lines as (owned, ghost)); the rest is dropped.  Host only: no
+ * device is touched until dc_partition_create_global. */
+typedef struct dc_partition_plan dc_partition_plan;
+
+typedef struct {
+  int32_t n_global, rank;
+  int32_t n_owned, n_ghost;
+  int32_t n_border;     /* owned vertices with a ghost neighbour (edges or inserted lines) */
+  int32_t n_parts;      /* 1 + the highest part in part_of (0 without it)           */
+  int64_t m_local;      /* edges kept = m_internal + m_cross                         */
+  int64_t m_internal, m_cross, m_dropped;
+  int64_t k_local;      /* change lines kept = k_del + k_ins + k_xdel + k_xins       */
+  int64_t k_del, k_ins, k_xdel, k_xins, k_dropped;
+} dc_partition_info;
+
+int dc_partition_plan_create(int32_t n_global, int32_t rank, int32_t n_owned,
+                             const int32_t* owned_ids, const int32_t* part_of, int64_t m,
+                             const int32_t* u, const int32_t* v, int64_t k, const int32_t* cu,
+                             const int32_t* cv, const uint8_t* cins, dc_partition_plan** out);
+int dc_partition_plan_info(const dc_partition_plan* p, dc_partition_info* info);
+/* Copy out (any pointer may be NULL): local_to_global[n_owned + n_ghost],
+ * local edges u, v [m_local], local change lines bu, bv, bins [k_local]. */
+int dc_partition_plan_arrays(const dc_partition_plan* p, int32_t* local_to_global, int32_t* u,
+                             int32_t* v, int32_t* bu, int32_t* bv, uint8_t* bins);
+/* Border vertices per part: per_part[q] = owned vertices with a neighbour in
+ * part q (the vertices whose colours the reference's MPI exchange sends to
+ * rank q, bvPerPart of MG/kernel.cu:127,351-354).  Needs part_of. */
+int dc_partition_plan_border(const dc_partition_plan* p, int64_t* per_part, int32_t nparts);
+/* The rank's device graph from the plan: dc_partition_create with the plan's
+ * local edges, owned and ghost colours looked up in colors_global (one per
+ * global vertex; a negative entry for a vertex the rank holds is an error, as
+ * read_Input_Color, MG/all_structure_undir.cuh:199-222, would leave it
+ * unset).  Apply the plan's change lines with dc_apply_batch. */
+int dc_partition_create_global(const dc_partition_plan* p, const int32_t* colors_global,
+                               const dc_options* opt, dc_graph** out);
+int dc_partition_plan_destroy(dc_partition_plan* p);
+
 int dc_graph_destroy(dc_graph* g);
 
 int dc_graph_info(const dc_graph* g, int32_t* n, int32_t* n_owned,
