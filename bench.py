@@ -1,25 +1,31 @@
 """Throughput benchmark of the dynamic-recoloring hot path on one MI355X.
 
-Default workload (c2, the metric's single-GPU configuration): R-MAT scale 22
-(edge factor 16, a/b/c = .57/.19/.19, self loops and duplicates removed, IDs
-not permuted), greedy static coloring, then consecutive update batches of 100k
-lines (50% insertions of uniform non-edges, 50% deletions of existing edges),
-each applied to the graph left by the previous one.  A step = apply one
-device-resident batch + run every recolor round until the coloring is final.
-The first --warmup batches are untimed; the next --steps batches are timed.
+Default run: the metric's single-GPU configuration (R-MAT scale 22, edge
+factor 16, a/b/c = .57/.19/.19, self loops and duplicates removed, IDs not
+permuted; greedy static colouring, then consecutive update batches of 100k
+lines, 50% insertions of uniform non-edges, 50% deletions of existing edges,
+each applied to the graph left by the previous one).  A step = apply one
+device-resident batch + run every recolour round until the colouring is
+final.  The first --warmup batches are untimed; the next --steps are timed.
+The same JSON line carries two compact sub-records: "batched" (512 graphs
+recoloured per step in one arena against one plain call per graph) and
+"served" (8 resident R-MAT graphs under a Poisson request stream, continuous
+batching against one plain call per request); vs_baseline is the batched
+speedup over the plain per-batch path (the metric's "vs plain per-batch").
 
-Other workloads (--workload): c1 ER 10k/50k one batch; c3 one partition of
-an R-MAT scale-24 graph; c4 512 graphs recolored per step in one launch vs
-one plain call per graph; c5 served request stream over 8 resident graphs.
+Other workloads alone (--workload): c1 ER 10k/50k, c3 one partition of an
+R-MAT scale-24 graph (ingested from global IDs), c4 batched, c5 served.
 
-Prints one JSON line.
+Prints one JSON line on stdout; progress goes to stderr.
 """
 from __future__ import annotations
 
 import argparse
+import gc
 import json
 import math
 import os
+import platform
 import sys
 import time
 
@@ -40,7 +46,7 @@ def pct(x, q):
     return float(np.percentile(np.asarray(x), q)) if len(x) else None
 
 
-def base_record(args, value, elapsed_steps, steps, warmup, config):
+def base_record(value, elapsed_steps, steps, warmup, config):
     return {
         "metric": METRIC,
         "value": value,
@@ -58,21 +64,11 @@ def base_record(args, value, elapsed_steps, steps, warmup, config):
     }
 
 
-def roofline(kernel, algo_bytes, kernel_ms, launches):
-    kern_s = kernel_ms / 1e3
-    achieved = algo_bytes / kern_s / 1e9 if kern_s > 0 else 0.0
-    return {"bound": "hbm", "kernel": kernel, "achieved": achieved, "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-            "launches": launches, "avg_launch_us": kernel_ms * 1e3 / max(launches, 1),
-            "algorithmic_bytes": int(algo_bytes)}
-
-
 def traffic_for(kernel):
     """HBM bytes per launch of the kernel from the committed rocprofv3 --pmc
     summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel factor), or
     None when no summary covers it."""
-    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
-                        "pmc_traffic.json")
+    path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
     try:
         with open(path) as f:
             t = json.load(f)
@@ -82,18 +78,52 @@ def traffic_for(kernel):
     return None if ent is None else ent.get("bytes_per_launch")
 
 
+def roofline(kernel, algo_bytes, kernel_ms, launches):
+    """Roofline of one kernel: achieved = its algorithmic bytes per launch
+    over its mean launch duration (hipEvents on its stream); traffic = the
+    measured HBM bytes per launch, both per launch."""
+    per_launch = algo_bytes / max(launches, 1)
+    avg_s = kernel_ms / 1e3 / max(launches, 1)
+    achieved = per_launch / avg_s / 1e9 if avg_s > 0 else 0.0
+    traffic = traffic_for(kernel)
+    out = {"bound": "hbm", "kernel": kernel, "achieved": achieved, "peak": HBM_PEAK_GBS,
+           "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+           "basis": "achieved and frac: the kernel's algorithmic bytes per launch (row slots "
+                    "streamed, rows visited, state and list words) / mean launch time; "
+                    "traffic: rocprofv3 FETCH_SIZE + WRITE_SIZE bytes per launch",
+           "algorithmic_bytes_per_launch": per_launch, "launches": launches,
+           "avg_launch_us": avg_s * 1e6, "algorithmic_bytes_total": int(algo_bytes)}
+    if traffic:
+        out["traffic_over_algorithmic"] = traffic / per_launch if per_launch else None
+        out["achieved_by_traffic"] = traffic / avg_s / 1e9 if avg_s > 0 else None
+    return out
+
+
 def set_driver(g, d, args):
     g.set_driver(d.DRIVER_FUSED if args.driver == "fused" else d.DRIVER_PLAIN, args.blocks_per_cu)
+
+
+def cpu_info():
+    model = platform.processor() or ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return os.cpu_count(), model
 
 
 # ----------------------------------------------------------------- c1 / c2 --
 def run_stream(args, d, rule, n, u, v, workload):
     """One graph, a stream of device-resident batches (c1, c2)."""
-    # per-launch hipEvents on the count pass (they cost ~7 % when recorded in
-    # the timed batches): by default in an untimed replay of the warm-up and
-    # timed batches on a fresh copy of the initial graph (deterministic: the
-    # same rounds and launches), with --events-after in 2 batches after the
-    # timed ones
+    # per-launch hipEvents on the count pass cost ~7 % when recorded in the
+    # timed batches: by default they are taken in an untimed replay of the
+    # warm-up and timed batches on a fresh copy of the initial graph
+    # (deterministic: the same rounds and launches); with --events-after in 2
+    # batches after the timed ones
     nprof = 2 if args.events_after else 0
     nb = args.warmup + args.steps + nprof
     t = time.perf_counter()
@@ -110,14 +140,25 @@ def run_stream(args, d, rule, n, u, v, workload):
     g.set_profiling(False)
     set_driver(g, d, args)
     prof = {"ms": 0.0, "bytes": 0, "launches": 0, "kms": 0.0, "rbytes": 0, "rl": 0}
+    terms = {"updates": 0, "del_row_slots": 0, "lc_vertices": 0, "lc_slots": 0, "loop_rounds": 0,
+             "sum_C": 0, "batches": 0, "kernel_ms": 0.0}
 
-    def add_prof(st):
+    def add_prof(st, gr):
         prof["ms"] += st.ms_count_pass
         prof["bytes"] += st.count_pass_bytes
         prof["launches"] += st.count_pass_launches
         prof["kms"] += st.ms_recolor_kernel
         prof["rbytes"] += st.recolor_bytes
         prof["rl"] += st.recolor_launches
+        tr = gr.trace()
+        terms["updates"] += st.updates
+        terms["del_row_slots"] += st.del_row_slots
+        terms["lc_vertices"] += st.lc_vertices
+        terms["lc_slots"] += st.lc_slots
+        terms["loop_rounds"] += int((tr[:, 0] == 10).sum()) if tr.size else 0
+        terms["sum_C"] += int(tr[tr[:, 0] == 10][:, 2].sum()) if tr.size else 0
+        terms["batches"] += 1
+        terms["kernel_ms"] += st.ms_recolor_kernel
 
     colors_after_first = None
     for i in range(args.warmup):
@@ -128,8 +169,7 @@ def run_stream(args, d, rule, n, u, v, workload):
     g.synchronize()
 
     lat = []
-    tot = {"bytes": 0, "kms": 0.0, "launches": 0, "rounds": 0, "changed": 0, "evaluated": 0,
-           "slots": 0, "regrows": 0, "settled": 0}
+    tot = {"rounds": 0, "changed": 0, "evaluated": 0, "slots": 0, "regrows": 0, "settled": 0}
     t0 = time.perf_counter()
     last_log = t0
     for i in range(args.warmup, args.warmup + args.steps):
@@ -140,9 +180,6 @@ def run_stream(args, d, rule, n, u, v, workload):
         if tb - last_log > 20.0:   # slow workloads (the LDF rule at scale) report progress
             log(f"timed batch {i - args.warmup + 1}/{args.steps}")
             last_log = tb
-        tot["bytes"] += st.recolor_bytes
-        tot["kms"] += st.ms_recolor_kernel
-        tot["launches"] += st.recolor_launches
         tot["rounds"] += st.loop_rounds
         tot["changed"] += st.changed
         tot["evaluated"] += st.evaluated
@@ -159,12 +196,13 @@ def run_stream(args, d, rule, n, u, v, workload):
         g.set_profiling(True)
         for i in range(args.warmup + args.steps, nb):
             g.apply_device_batch(dev_batches[i])
-            add_prof(g.recolor(rule))
+            add_prof(g.recolor(rule), g)
         g.set_profiling(False)
         g.synchronize()
     else:
         colors_timed = g.colors()
         del dev_batches, g
+        gc.collect()
         g = d.Graph(n, u, v)
         g.static_color(rule)
         set_driver(g, d, args)
@@ -179,17 +217,19 @@ def run_stream(args, d, rule, n, u, v, workload):
             g.apply_device_batch(dev_batches[i])
             st = g.recolor(rule)
             if i >= args.warmup:
-                add_prof(st)
+                add_prof(st, g)
                 replay_rounds += st.loop_rounds
         g.set_profiling(False)
         g.synchronize()
         if replay_rounds != tot["rounds"] or not (g.colors() == colors_timed).all():
             log(f"replay differs: {replay_rounds} rounds against {tot['rounds']}")
             replay_rounds = None
+    del dev_batches, g
+    gc.collect()
     log(f"timed {args.steps} batches in {elapsed:.3f}s, rounds {tot['rounds']}, colors {maxc + 1}")
     value = args.steps * args.batch / elapsed if mono == 0 else 0.0
     lat_ms = np.array(lat) * 1e3
-    out = base_record(args, value, elapsed, args.steps, args.warmup, {
+    out = base_record(value, elapsed, args.steps, args.warmup, {
         "workload": workload, "edges": int(u.size), "vertices": int(n), "batch": args.batch,
         "rule": args.rule, "driver": args.driver, "colors": int(maxc + 1),
         "monochromatic": int(mono)})
@@ -203,42 +243,81 @@ def run_stream(args, d, rule, n, u, v, workload):
     out["roofline"]["measured_on"] = (
         f"{nprof} batches after the timed ones" if args.events_after else
         f"replay of the warm-up and timed batches (same {replay_rounds} rounds)")
-    out["roofline"]["traffic"] = traffic_for(out["roofline"]["kernel"])
+    out["reference_model"] = reference_model(terms, n)
     out["work"] = {"loop_rounds": tot["rounds"], "evaluated": tot["evaluated"],
                    "changed": tot["changed"], "row_slots_scanned": tot["slots"],
                    "batches_with_row_growth": tot["regrows"],
                    "batches_settled": tot["settled"]}
     out["cpu_baseline"] = None
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(n, u, v, batches[0], rule, colors_after_first)
+        out["cpu_baseline"] = cpu_baseline(args, n, u, v, batches[0], rule, colors_after_first)
     return out
 
 
-def cpu_baseline(n, u, v, batch, rule, gpu_colors_after_first):
-    """The oracle (single thread) on the first batch of the same stream."""
+def reference_model(t, n):
+    """Bytes the reference's algorithm moves for the same batches (per batch):
+    12 per update record + 8 per adjacency write or tombstone + 4 per row slot
+    of both endpoints per deletion (its linear deletion search) + per loop round
+    16 + 8 per row slot for every vertex of L_r u C_r (it rescans each of their
+    rows) + a bitmap over the vertices and 4 per member of C_r (compaction),
+    counted over the rounds this build ran (rounds after settled colours are
+    not counted: the figure is a lower bound).  achieved = those bytes over the
+    summed recolour-kernel time of the same batches."""
+    b = max(t["batches"], 1)
+    B = (20 * t["updates"] + 4 * t["del_row_slots"] + 16 * t["lc_vertices"] + 8 * t["lc_slots"]
+         + t["loop_rounds"] * (n / 8.0) + 4 * t["sum_C"])
+    s = t["kernel_ms"] / 1e3
+    ach = B / s / 1e9 if s > 0 else None
+    return {"terms_per_batch": {k: t[k] / b for k in ("updates", "del_row_slots", "lc_vertices",
+                                                     "lc_slots", "loop_rounds", "sum_C")},
+            "bytes_per_batch": B / b, "recolor_kernel_ms_per_batch": t["kernel_ms"] / b,
+            "achieved_GBs": ach, "frac_of_peak": ach / HBM_PEAK_GBS if ach else None,
+            "formula": "20*U + 4*del_row_slots + 16*|L u C| + 8*slots(L u C) + rounds*n/8 + 4*sum|C|"}
+
+
+def cpu_baseline(args, n, u, v, batch, rule, gpu_colors_after_first):
+    """The oracle replaying the first batch of the same stream on this host:
+    single-threaded, and with each round's work split over OpenMP threads
+    (independent rule); both checked bit-equal to the GPU's colours."""
     from oracle import oracle as O
     try:
         O.lib()
     except Exception as e:  # oracle not built
         log(f"cpu baseline skipped: {e}")
         return None
-    t = time.perf_counter()
-    og = O.OracleGraph(n, u, v)
-    og.static_color(rule)
-    log(f"oracle graph + static coloring: {time.perf_counter() - t:.1f}s")
+    nproc, model = cpu_info()
+    threads = max(1, min(args.cpu_threads, nproc or 1))
     bu, bv, bi = batch
-    t = time.perf_counter()
-    og.apply_batch(bu, bv, bi)
-    og.recolor(rule)
-    dt = time.perf_counter() - t
-    same = None
-    if gpu_colors_after_first is not None:
-        same = bool((og.color == gpu_colors_after_first).all())
-    log(f"oracle batch: {dt:.1f}s, colors equal to GPU: {same}")
-    return {"value": bu.size / dt, "unit": "updates/s", "cores": 1, "kind": "port",
-            "sample": f"first batch ({bu.size} lines) of the same stream from the same initial "
-                      f"graph, single-thread oracle replay (apply + all rounds)",
-            "seconds": dt, "bit_equal_to_gpu": same}
+    res = {}
+    for th in ([1, threads] if rule == 0 and threads > 1 else [1]):
+        t = time.perf_counter()
+        og = O.OracleGraph(n, u, v)
+        og.static_color(rule)
+        log(f"oracle graph + static coloring: {time.perf_counter() - t:.1f}s")
+        t = time.perf_counter()
+        og.apply_batch(bu, bv, bi)
+        if th == 1:
+            og.recolor(rule)
+        else:
+            og.recolor_par(rule, th)
+        dt = time.perf_counter() - t
+        same = None
+        if gpu_colors_after_first is not None:
+            same = bool((og.color == gpu_colors_after_first).all())
+        log(f"oracle batch, {th} thread(s): {dt:.1f}s, colors equal to GPU: {same}")
+        res[th] = (dt, same)
+        del og
+    th = max(res)
+    dt, same = res[th]
+    out = {"value": bu.size / dt, "unit": "updates/s", "cores": th, "kind": "port",
+           "sample": f"first batch ({bu.size} lines) of the same stream from the same initial "
+                     f"graph, oracle replay (apply + all rounds) on {th} thread(s)",
+           "seconds": dt, "bit_equal_to_gpu": same, "nproc": nproc, "cpu_model": model,
+           "threads": th}
+    if th != 1:
+        out["single_thread"] = {"value": bu.size / res[1][0], "seconds": res[1][0],
+                                "bit_equal_to_gpu": res[1][1]}
+    return out
 
 
 # ---------------------------------------------------------------------- c4 --
@@ -263,10 +342,10 @@ def c4_graphs(d, count, seed):
     return graphs, kinds, rng
 
 
-def run_c4(args, d, rule):
+def run_c4(args, d, rule, steps, warmup):
     t = time.perf_counter()
     graphs, kinds, rng = c4_graphs(d, args.graphs, args.seed)
-    nsteps = args.warmup + args.steps
+    nsteps = warmup + steps
     per = []
     for i, (n, u, v) in enumerate(graphs):
         frac = rng.uniform(0.01, 0.10)
@@ -277,37 +356,36 @@ def run_c4(args, d, rule):
     log(f"c4: {len(graphs)} graphs, {sum(g[0] for g in graphs)} vertices, {tot_edges} edges, "
         f"{step_lines[-1]} lines per step ({time.perf_counter() - t:.1f}s)")
 
-    # batched: one arena, one launch per step
+    # batched: one arena, one launch sequence per step
     n, U, V, off = d.union_edges(graphs)
     arena = d.Graph(n, U, V)
     arena.static_color(rule)
     set_driver(arena, d, args)
-    arena.set_profiling(True)
     ub = [arena.upload_batch(*d.union_batch([p[s] for p in per], off)) for s in range(nsteps)]
-    for s in range(args.warmup):
+    for s in range(warmup):
         arena.apply_device_batch(ub[s])
         arena.recolor(rule)
     arena.synchronize()
-    lat, kms, byts, rounds = [], 0.0, 0, 0
+    lat, rounds = [], 0
     t0 = time.perf_counter()
-    for s in range(args.warmup, nsteps):
+    for s in range(warmup, nsteps):
         tb = time.perf_counter()
         arena.apply_device_batch(ub[s])
         st = arena.recolor(rule)
         lat.append(time.perf_counter() - tb)
-        kms += st.ms_recolor_kernel
-        byts += st.recolor_bytes
         rounds += st.loop_rounds
     arena.synchronize()
     t_batched = time.perf_counter() - t0
     mono, _ = arena.validate()
-    lines = sum(step_lines[args.warmup:])
+    got = arena.colors()
+    del ub, arena
+    gc.collect()
+    lines = sum(step_lines[warmup:])
     batched = lines / t_batched if mono == 0 else 0.0
-    log(f"c4 batched: {t_batched:.3f}s for {args.steps} steps, rounds {rounds}, mono {mono}")
+    log(f"c4 batched: {t_batched:.3f}s for {steps} steps, rounds {rounds}, mono {mono}")
 
     # plain per-batch baseline: one graph handle and one plain call per graph
     t_plain = 0.0
-    plain_colors = []
     handles = []
     for i, (gn, gu, gv) in enumerate(graphs):
         g = d.Graph(gn, gu, gv)
@@ -318,32 +396,28 @@ def run_c4(args, d, rule):
         for g, bl in handles:
             g.apply_device_batch(bl[s])
             g.recolor(rule)
-        if s >= args.warmup:
+        if s >= warmup:
             t_plain += time.perf_counter() - tb
-    for g, _ in handles:
-        plain_colors.append(g.colors())
+    same = all((got[off[i]:off[i + 1]] == h[0].colors()).all() for i, h in enumerate(handles))
+    del handles
+    gc.collect()
     plain = lines / t_plain
-    got = arena.colors()
-    same = all((got[off[i]:off[i + 1]] == plain_colors[i]).all() for i in range(len(graphs)))
     log(f"c4 plain: {t_plain:.3f}s, speedup {batched / plain:.1f}x, colors equal: {same}")
-    out = base_record(args, batched, t_batched, args.steps, args.warmup, {
+    lat_ms = np.array(lat) * 1e3
+    return {
         "workload": f"c4_{len(graphs)}graphs_lognormal20k_er_rmat_1to10pct",
+        "value": batched, "unit": "updates/s", "steps": steps, "warmup": warmup,
+        "ms_per_step": t_batched / steps * 1e3, "p50_ms": pct(lat_ms, 50), "p99_ms": pct(lat_ms, 99),
         "graphs": len(graphs), "vertices": int(n), "edges": int(tot_edges),
-        "lines_per_step": int(step_lines[-1]), "rule": args.rule, "driver": args.driver,
-        "monochromatic": int(mono)})
-    out["latency_ms"] = {"p50": pct(np.array(lat) * 1e3, 50), "p99": pct(np.array(lat) * 1e3, 99)}
-    out["plain_per_batch"] = {"value": plain, "seconds": t_plain, "calls_per_step": len(graphs)}
-    out["speedup_vs_plain"] = batched / plain
-    out["bit_equal_to_plain"] = bool(same)
-    out["roofline"] = roofline("k_fused" if args.driver == "fused" else "k_recolor<M_LOOP>",
-                               byts, kms, args.steps)
-    return out
+        "lines_per_step": int(step_lines[-1]), "loop_rounds": rounds, "monochromatic": int(mono),
+        "plain_value": plain, "plain_seconds": t_plain, "plain_calls_per_step": len(graphs),
+        "speedup_vs_plain": batched / plain, "bit_equal_to_plain": bool(same)}
 
 
 # ---------------------------------------------------------------------- c5 --
-def c5_requests(d, graphs, rate, duration, rng):
-    nreq = max(1, int(rng.poisson(rate * duration)))
-    arrival = np.sort(rng.uniform(0, duration, nreq))
+def c5_requests(graphs, rate, duration, max_req, rng):
+    nreq = max(1, min(int(rng.poisson(rate * duration)), max_req))
+    arrival = np.sort(rng.uniform(0, nreq / rate, nreq))
     gid = rng.integers(0, len(graphs), nreq).astype(np.int32)
     sizes = np.clip(np.exp(rng.normal(math.log(5000), 1.2, nreq)), 10, 500_000).astype(np.int64)
     roff = np.zeros(nreq + 1, np.int64)
@@ -365,6 +439,18 @@ def c5_requests(d, graphs, rate, duration, rng):
     return gid, roff, u, v, ins, arrival
 
 
+def lane_split(graphs, nlanes):
+    """Greedy: largest graph first into the lane with the fewest edges."""
+    order = sorted(range(len(graphs)), key=lambda g: -graphs[g][1].size)
+    load = [0] * nlanes
+    lane_of = [0] * len(graphs)
+    for g in order:
+        l = int(np.argmin(load))
+        lane_of[g] = l
+        load[l] += graphs[g][1].size
+    return lane_of
+
+
 def run_c5(args, d, rule):
     scales = [18, 19, 20, 21, 22, 23, 18, 20]
     t = time.perf_counter()
@@ -372,28 +458,42 @@ def run_c5(args, d, rule):
     for i, s in enumerate(scales):
         u, v = d.gen_rmat(s, 16, seed=args.seed + 31 * i)
         graphs.append((1 << s, u, v))
-    n, U, V, off = d.union_edges(graphs)
-    arena = d.Graph(n, U, V)
-    arena.static_color(rule)
-    set_driver(arena, d, args)
-    log(f"c5: 8 resident graphs, {n} vertices, {U.size} edges ({time.perf_counter() - t:.1f}s)")
+    nl = max(1, min(args.lanes, len(graphs)))
+    lane_of = lane_split(graphs, nl)
+    arenas, first, size = [], [0] * len(graphs), [0] * len(graphs)
+    for l in range(nl):
+        mine = [g for g in range(len(graphs)) if lane_of[g] == l]
+        n, U, V, off = d.union_edges([graphs[g] for g in mine])
+        a = d.Graph(n, U, V)
+        a.static_color(rule)
+        set_driver(a, d, args)
+        arenas.append(a)
+        for k, g in enumerate(mine):
+            first[g] = int(off[k])
+            size[g] = int(off[k + 1] - off[k])
+    tot_edges = sum(g[1].size for g in graphs)
+    log(f"c5: 8 resident graphs in {nl} lanes, {tot_edges} edges ({time.perf_counter() - t:.1f}s)")
     rng = np.random.default_rng(args.seed)
-    srv = d.Server(arena, off, budget_lines=args.budget_lines, rule=rule)
+    budget = args.budget_mb << 20
+    srv = d.Server(arenas, lane_of, first, size, budget_bytes=budget, rule=rule)
     sweep = []
     for rate in args.rates:
-        gid, roff, u, v, ins, arrival = c5_requests(d, graphs, rate, args.duration, rng)
+        gid, roff, u, v, ins, arrival = c5_requests(graphs, rate, args.duration, args.max_requests, rng)
         lat, step, st = srv.run(gid, roff, u, v, ins, arrival)
         ok = lat >= 0
-        lines = int(roff[-1])
         rec = {"rate_req_s": rate, "requests": int(gid.size), "served": int(st.served),
-               "rejected": int(st.rejected), "lines": lines, "steps": int(st.steps),
-               "elapsed_s": st.elapsed_s, "busy_s": st.busy_s,
+               "rejected": int(st.rejected), "lines": int(st.lines), "steps": int(st.steps),
+               "elapsed_s": st.elapsed_s, "offered_s": float(arrival[-1]),
                "updates_per_s": st.lines / st.elapsed_s,
                "p50_ms": pct(lat[ok] * 1e3, 50), "p99_ms": pct(lat[ok] * 1e3, 99),
-               "max_step_lines": int(st.max_step_lines)}
+               "max_step_lines": int(st.max_step_lines), "admit_waits": int(st.admit_waits),
+               "max_inflight_MB": st.max_inflight_bytes / 2**20}
         log(f"c5 rate {rate}/s: {rec}")
         sweep.append(rec)
-    mono, _ = arena.validate()
+    mono = sum(a.validate()[0] for a in arenas)
+    srv.close()
+    del srv, arenas
+    gc.collect()
     # plain per-batch capacity: a sample of requests, one plain call each on
     # its own graph handle, no coalescing
     rng2 = np.random.default_rng(args.seed + 1)
@@ -402,7 +502,7 @@ def run_c5(args, d, rule):
         g = d.Graph(gn, gu, gv)
         g.static_color(rule)
         handles.append(g)
-    gid, roff, u, v, ins, _ = c5_requests(d, graphs, 40, 1.0, rng2)
+    gid, roff, u, v, ins, _ = c5_requests(graphs, 40, 1.0, 200, rng2)
     t0 = time.perf_counter()
     plat = []
     for r in range(gid.size):
@@ -411,31 +511,33 @@ def run_c5(args, d, rule):
         handles[gid[r]].process_batch(u[lo:hi], v[lo:hi], ins[lo:hi], rule)
         plat.append(time.perf_counter() - tb)
     t_plain = time.perf_counter() - t0
+    del handles
+    gc.collect()
     plain = float(roff[-1]) / t_plain
     best = max(sweep, key=lambda r: r["updates_per_s"])
-    out = base_record(args, best["updates_per_s"], best["busy_s"], best["steps"], 0, {
+    return {
         "workload": "c5_8rmat_s18-23_poisson_lognormal5k_60ins",
-        "vertices": int(n), "edges": int(U.size), "budget_lines": args.budget_lines,
-        "rule": args.rule, "driver": args.driver, "monochromatic": int(mono)})
-    out["latency_ms"] = {"p50": best["p50_ms"], "p99": best["p99_ms"]}
-    out["sweep"] = sweep
-    out["plain_per_batch"] = {"value": plain, "requests": int(gid.size), "seconds": t_plain,
-                              "p50_ms": pct(np.array(plat) * 1e3, 50),
-                              "p99_ms": pct(np.array(plat) * 1e3, 99)}
-    out["speedup_vs_plain"] = best["updates_per_s"] / plain
-    return out
+        "value": best["updates_per_s"], "unit": "updates/s", "capacity_rate_req_s": best["rate_req_s"],
+        "p50_ms": best["p50_ms"], "p99_ms": best["p99_ms"], "lanes": nl,
+        "budget_MB": args.budget_mb, "edges": int(tot_edges), "monochromatic": int(mono),
+        "sweep": sweep,
+        "plain_value": plain, "plain_requests": int(gid.size), "plain_seconds": t_plain,
+        "plain_p50_ms": pct(np.array(plat) * 1e3, 50), "plain_p99_ms": pct(np.array(plat) * 1e3, 99),
+        "speedup_vs_plain": best["updates_per_s"] / plain}
 
 
 # ---------------------------------------------------------------------- c3 --
 def run_c3(args, d, rule):
-    """Partition 0 of 4 of an R-MAT scale-24 graph (owned + frozen ghosts)."""
+    """Partition 0 of 4 of an R-MAT scale-24 graph: the rank's share built by
+    the global-ID ingest (dc_partition_plan), ghost colours frozen at the
+    whole graph's static colouring; the first batch is checked against the
+    oracle."""
     scale = args.c3_scale
     t = time.perf_counter()
     u, v = d.gen_rmat(scale, 16, seed=args.seed)
     N = 1 << scale
     n_own = N // 4
     log(f"c3: rmat {scale}: {u.size} edges ({time.perf_counter() - t:.1f}s)")
-    # ghost colors: the whole graph's static coloring
     full = d.Graph(N, u, v)
     full.static_color(rule)
     colors = full.colors()
@@ -445,62 +547,63 @@ def run_c3(args, d, rule):
     rng = np.random.default_rng(args.seed)
     nsteps = args.warmup + args.steps
     bs = args.batch
-    # batches: half insertions (owned endpoint + uniform vertex), half deletions
-    # of existing edges touching the partition
-    blines = []
+    # batches: half insertions (owned endpoint + uniform vertex), half
+    # deletions of existing edges touching the partition
+    cu, cv, ci = [], [], []
     for s in range(nsteps):
         k_ins = bs // 2
         a = rng.integers(0, n_own, k_ins)
         b = (a + rng.integers(1, N, k_ins)) % N
         e = rng.integers(0, pu.size, bs - k_ins)
-        bu = np.concatenate([a, pu[e]]).astype(np.int64)
-        bv = np.concatenate([b, pv[e]]).astype(np.int64)
-        bi = np.concatenate([np.ones(k_ins, np.uint8), np.zeros(bs - k_ins, np.uint8)])
-        blines.append((bu, bv, bi))
-    # ghost IDs: every non-owned vertex named by the edges or the batches
-    others = np.concatenate([pu[pu >= n_own], pv[pv >= n_own]] +
-                            [x[x >= n_own] for b in blines for x in (b[0], b[1])])
-    ghosts = np.unique(others)
-    lid = np.full(N, -1, np.int64)
-    lid[:n_own] = np.arange(n_own)
-    lid[ghosts] = n_own + np.arange(ghosts.size)
-    g = d.Graph(n_own, lid[pu].astype(np.int32), lid[pv].astype(np.int32),
-                n_ghost=int(ghosts.size), ghost_colors=colors[ghosts])
-    init = np.zeros(n_own + ghosts.size, np.int32)
-    init[:n_own] = colors[:n_own]
-    init[n_own:] = colors[ghosts]
-    g.set_colors(init)
+        cu.append(np.concatenate([a, pu[e]]).astype(np.int32))
+        cv.append(np.concatenate([b, pv[e]]).astype(np.int32))
+        ci.append(np.concatenate([np.ones(k_ins, np.uint8), np.zeros(bs - k_ins, np.uint8)]))
+    part = (np.arange(N, dtype=np.int64) // n_own).astype(np.int32)
+    t = time.perf_counter()
+    plan = d.PartitionPlan(N, 0, np.arange(n_own, dtype=np.int32), part, u, v,
+                           np.concatenate(cu), np.concatenate(cv), np.concatenate(ci))
+    l2g, lu, lv, bu, bv, bi = plan.arrays()
+    info = plan.info
+    g = plan.create_graph(colors)
+    log(f"c3 partition (global-ID ingest {time.perf_counter() - t:.1f}s): {info['n_owned']} owned, "
+        f"{info['n_ghost']} ghosts, {info['m_local']} edges")
     set_driver(g, d, args)
-    g.set_profiling(True)
-    db = [g.upload_batch(lid[b[0]].astype(np.int32), lid[b[1]].astype(np.int32), b[2])
-          for b in blines]
-    log(f"c3 partition: {n_own} owned, {ghosts.size} ghosts, {pu.size} edges "
-        f"({time.perf_counter() - t:.1f}s)")
+    db = [g.upload_batch(bu[s * bs:(s + 1) * bs], bv[s * bs:(s + 1) * bs], bi[s * bs:(s + 1) * bs])
+          for s in range(nsteps)]
+    first_colors = None
     for s in range(args.warmup):
         g.apply_device_batch(db[s])
         g.recolor(rule)
+        if s == 0:
+            first_colors = g.colors()
     g.synchronize()
-    lat, kms, byts, launches = [], 0.0, 0, 0
+    lat = []
     t0 = time.perf_counter()
     for s in range(args.warmup, nsteps):
         tb = time.perf_counter()
         g.apply_device_batch(db[s])
-        st = g.recolor(rule)
+        g.recolor(rule)
         lat.append(time.perf_counter() - tb)
-        kms += st.ms_recolor_kernel
-        byts += st.recolor_bytes
-        launches += st.recolor_launches
     g.synchronize()
     el = time.perf_counter() - t0
     mono, maxc = g.validate()
     value = args.steps * bs / el if mono == 0 else 0.0
-    out = base_record(args, value, el, args.steps, args.warmup, {
-        "workload": f"c3_rmat{scale}_partition0of4_{bs}mixed", "owned": n_own,
-        "ghosts": int(ghosts.size), "edges": int(pu.size), "batch": bs, "rule": args.rule,
-        "driver": args.driver, "monochromatic": int(mono)})
+    out = base_record(value, el, args.steps, args.warmup, {
+        "workload": f"c3_rmat{scale}_partition0of4_{bs}mixed", "owned": int(info["n_owned"]),
+        "ghosts": int(info["n_ghost"]), "edges": int(info["m_local"]), "batch": bs,
+        "rule": args.rule, "driver": args.driver, "monochromatic": int(mono)})
     out["latency_ms"] = {"p50": pct(np.array(lat) * 1e3, 50), "p99": pct(np.array(lat) * 1e3, 99)}
-    out["roofline"] = roofline("k_recolor<M_LOOP>" if args.driver == "plain" else "k_fused",
-                               byts, kms, launches)
+    out["border_vertices_per_part"] = plan.border().tolist()
+    if first_colors is not None and not args.no_cpu_baseline:
+        from oracle import oracle as O
+        t = time.perf_counter()
+        og = O.OracleGraph(l2g.size, lu, lv, n_owned=info["n_owned"])
+        og.set_colors(colors[l2g])
+        og.apply_batch(bu[:bs], bv[:bs], bi[:bs])
+        og.recolor(rule)
+        same = bool((og.color == first_colors).all())
+        log(f"c3 oracle first batch: {time.perf_counter() - t:.1f}s, colors equal: {same}")
+        out["bit_equal_to_oracle_first_batch"] = same
     return out
 
 
@@ -518,14 +621,20 @@ def main():
     ap.add_argument("--blocks-per-cu", type=int, default=0)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--graphs", type=int, default=512)
-    ap.add_argument("--rates", type=float, nargs="+", default=[20, 80, 320])
-    ap.add_argument("--duration", type=float, default=4.0)
-    ap.add_argument("--budget-lines", type=int, default=4_000_000)
+    ap.add_argument("--c4-steps", type=int, default=3, help="timed c4 steps of the default run")
+    ap.add_argument("--rates", type=float, nargs="+", default=[80, 320, 1280, 5120])
+    ap.add_argument("--duration", type=float, default=1.5)
+    ap.add_argument("--max-requests", type=int, default=6000)
+    ap.add_argument("--lanes", type=int, default=3)
+    ap.add_argument("--budget-mb", type=int, default=512)
     ap.add_argument("--c3-scale", type=int, default=24)
+    ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batched", action="store_true", help="default run: skip the c4 sub-record")
+    ap.add_argument("--no-served", action="store_true", help="default run: skip the c5 sub-record")
     ap.add_argument("--events-after", action="store_true",
                     help="time the count pass per launch in extra batches after the timed "
-                         "ones instead of in the timed batches")
+                         "ones instead of in an untimed replay")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="copy 1 GiB five times (device to device) and exit: the reference "
                          "kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE readings")
@@ -564,14 +673,33 @@ def main():
         log(f"rmat scale {args.scale}: {u.size} edges ({time.perf_counter() - t:.1f}s)")
         out = run_stream(args, d, rule, 1 << args.scale, u, v,
                          f"rmat{args.scale}_ef{args.edge_factor}_{args.steps}x{args.batch}_mixed50")
+        del u, v
+        gc.collect()
+        if not args.no_batched:
+            b = run_c4(args, d, rule, args.c4_steps, 1)
+            out["batched"] = b
+            out["vs_baseline"] = b["speedup_vs_plain"]
+            out["vs_baseline_basis"] = ("batched (512 graphs per step in one arena) updates/s over "
+                                        "the plain per-batch path (one call per graph), same inputs")
+        if not args.no_served:
+            out["served"] = run_c5(args, d, rule)
     elif args.workload == "c3":
         if args.batch == 100000:
             args.batch = 1_000_000
         out = run_c3(args, d, rule)
     elif args.workload == "c4":
-        out = run_c4(args, d, rule)
+        b = run_c4(args, d, rule, args.steps, args.warmup)
+        out = base_record(b["value"], b["ms_per_step"] / 1e3 * args.steps, args.steps, args.warmup,
+                          {"workload": b["workload"], "graphs": b["graphs"], "rule": args.rule,
+                           "driver": args.driver})
+        out["batched"] = b
+        out["vs_baseline"] = b["speedup_vs_plain"]
     else:
-        out = run_c5(args, d, rule)
+        s = run_c5(args, d, rule)
+        out = base_record(s["value"], 0.0, 0, 0, {"workload": s["workload"], "rule": args.rule,
+                                                  "driver": args.driver})
+        out["served"] = s
+        out["vs_baseline"] = s["speedup_vs_plain"]
     print(json.dumps(out), flush=True)
 
 

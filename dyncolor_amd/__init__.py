@@ -36,7 +36,7 @@ EXPORTS = [
     "dc_process_batch", "dc_validate", "dc_trace_get", "dc_neighbors", "dc_prop_get",
     "dc_compact", "dc_compact_noalloc", "dc_check_consistency", "dc_set_profiling", "dc_set_driver", "dc_set_parts", "dc_synchronize", "dc_last_error", "dc_device_count",
     "dc_gen_er", "dc_gen_rmat", "dc_gen_batches",
-    "dc_server_create", "dc_server_destroy", "dc_server_run",
+    "dc_server_create", "dc_server_destroy", "dc_server_run", "dc_server_request_bytes",
     "dc_partition_plan_create", "dc_partition_plan_info", "dc_partition_plan_arrays",
     "dc_partition_plan_border", "dc_partition_create_global", "dc_partition_plan_destroy",
 ]
@@ -68,9 +68,11 @@ class Stats(C.Structure):
 
 
 class ServeStats(C.Structure):
-    _fields_ = [("steps", C.c_int32), ("reserved0", C.c_int32), ("served", C.c_int64),
+    _fields_ = [("steps", C.c_int32), ("lanes", C.c_int32), ("served", C.c_int64),
                 ("rejected", C.c_int64), ("lines", C.c_int64), ("max_step_lines", C.c_int64),
-                ("elapsed_s", C.c_double), ("busy_s", C.c_double), ("reserved", C.c_int64 * 4)]
+                ("elapsed_s", C.c_double), ("busy_s", C.c_double), ("budget_bytes", C.c_int64),
+                ("max_inflight_bytes", C.c_int64), ("admit_waits", C.c_int64),
+                ("reserved", C.c_int64 * 1)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_ if not k.startswith("reserved")}
@@ -144,7 +146,9 @@ def lib():
                                   C.c_uint64, _i32p, _i32p, C.POINTER(C.c_int64)]),
         "dc_gen_batches": (C.c_int, [C.c_int32, C.c_int64, _vp, _vp, C.c_int32, C.c_int64,
                                      C.c_double, C.c_uint64, _i32p, _i32p, _u8p]),
-        "dc_server_create": (C.c_int, [_vp, C.c_int32, _i32p, C.c_int64, C.c_int32, C.POINTER(_vp)]),
+        "dc_server_create": (C.c_int, [C.c_int32, C.POINTER(_vp), C.c_int32, _i32p, _i32p, _i32p,
+                                       C.c_int64, C.c_int32, C.POINTER(_vp)]),
+        "dc_server_request_bytes": (C.c_int64, [C.c_int64, C.c_int64]),
         "dc_partition_plan_create": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, _vp, _vp, C.c_int64,
                                                _vp, _vp, C.c_int64, _vp, _vp, _vp, C.POINTER(_vp)]),
         "dc_partition_plan_info": (C.c_int, [_vp, C.POINTER(PartitionInfo)]),
@@ -470,15 +474,27 @@ def union_batch(batches, offsets):
 
 
 class Server:
-    """Continuous-batching loop over the graphs resident in one arena Graph."""
+    """Continuous batching over resident graphs (dc_server_*): lanes = arena
+    Graphs, graph g at [first[g], first[g] + size[g]) of lanes[lane_of[g]]."""
 
-    def __init__(self, arena: Graph, offsets, budget_lines, rule=RULE_INDEPENDENT):
+    def __init__(self, lanes, lane_of, first, size, budget_bytes=0, rule=RULE_INDEPENDENT):
+        self.lanes = list(lanes)
+        arr = (C.c_void_p * len(self.lanes))(*[g.handle for g in self.lanes])
+        self._lane_of = _arr(lane_of, np.int32)
+        self._first = _arr(first, np.int32)
+        self._size = _arr(size, np.int32)
         h = C.c_void_p()
-        self._off = _arr(offsets, np.int32)
-        _check(lib().dc_server_create(arena.handle, self._off.size - 1, self._off,
-                                      int(budget_lines), rule, C.byref(h)))
+        _check(lib().dc_server_create(len(self.lanes), arr, self._lane_of.size, self._lane_of,
+                                      self._first, self._size, int(budget_bytes), rule,
+                                      C.byref(h)))
         self._h = h
-        self.arena = arena
+
+    @classmethod
+    def one_lane(cls, arena, offsets, budget_bytes=0, rule=RULE_INDEPENDENT):
+        """All graphs in one arena with the offsets of union_edges."""
+        off = _arr(offsets, np.int64)
+        n = off.size - 1
+        return cls([arena], np.zeros(n, np.int32), off[:-1], np.diff(off), budget_bytes, rule)
 
     def close(self):
         if getattr(self, "_h", None):
@@ -505,3 +521,7 @@ class Server:
         _check(lib().dc_server_run(self._h, nreq, gid, roff, _ptr(u), _ptr(v), _ptr(ins),
                                    arrival, lat, step, C.byref(st)))
         return lat[:nreq], step[:nreq], st
+
+
+def request_bytes(lines, insertions):
+    return int(lib().dc_server_request_bytes(int(lines), int(insertions)))

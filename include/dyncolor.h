@@ -308,33 +308,50 @@ int dc_device_count(int32_t* count);
 
 /* ---- serving: continuous batching over resident graphs --------------------
  * No counterpart in the reference (it recolors one batch per process run,
- * HI/Color_main.cu:31-247).  The resident graphs are one arena dc_graph
- * holding their disjoint union: graph g owns vertex IDs [offsets[g],
- * offsets[g+1]).  Requests are replayed against the wall clock: request i
- * targets graph gid[i], carries lines [roff[i], roff[i+1]) of (u, v,
- * is_insert) in that graph's local IDs and arrives arrival[i] seconds after
- * the start (non-decreasing).  Arrived requests are admitted in arrival order
- * while the lines in flight fit budget_lines (a larger request is rejected,
- * latency -1); each step coalesces every graph's admitted requests into one
- * batch per graph and recolors all of them in one arena launch.
- * latency[i] = completion - arrival (seconds); step_of_req may be NULL. */
+ * HI/Color_main.cu:31-247).  The resident graphs live in nlanes lanes: lane l
+ * is the arena dc_graph arenas[l] (the disjoint union of its graphs, each
+ * lane with its own stream and host worker thread); graph g owns arena IDs
+ * [first[g], first[g] + size[g]) of lane lane_of[g].  The server uses each
+ * arena from its worker only while dc_server_run runs.  Requests are replayed
+ * against the wall clock: request i targets graph gid[i], carries lines
+ * [roff[i], roff[i+1]) of (u, v, is_insert) in that graph's local IDs and
+ * arrives arrival[i] seconds after the start (non-decreasing).  Requests are
+ * admitted in arrival order while the device bytes in flight
+ * (dc_server_request_bytes of each admitted, unfinished request) fit
+ * budget_bytes, also while steps run (a request that does not fit waits for
+ * a step to finish; one larger than the whole budget is rejected, latency
+ * -1).  Each lane repeatedly takes all its admitted requests, coalesces them
+ * per graph into one batch and recolors its arena in one step; lanes run
+ * concurrently on the device.  latency[i] = completion - arrival (seconds);
+ * step_of_req[i] = the step (numbered in completion order over all lanes)
+ * that served request i, or NULL.  budget_bytes = 0: a quarter of the free
+ * device memory at creation. */
 typedef struct dc_server dc_server;
 
 typedef struct {
   int32_t steps;
-  int32_t reserved0;
+  int32_t lanes;
   int64_t served, rejected, lines, max_step_lines;
-  double elapsed_s, busy_s;
-  int64_t reserved[4];
+  double elapsed_s;
+  double busy_s;              /* summed over lanes: wall time inside steps */
+  int64_t budget_bytes;
+  int64_t max_inflight_bytes; /* most device bytes admitted at once         */
+  int64_t admit_waits;        /* requests that waited for bytes to free up  */
+  int64_t reserved[1];
 } dc_serve_stats;
 
-int dc_server_create(dc_graph* arena, int32_t ngraphs, const int32_t* offsets,
-                     int64_t budget_lines, int32_t rule, dc_server** out);
+int dc_server_create(int32_t nlanes, dc_graph* const* arenas, int32_t ngraphs,
+                     const int32_t* lane_of, const int32_t* first, const int32_t* size,
+                     int64_t budget_bytes, int32_t rule, dc_server** out);
 int dc_server_destroy(dc_server* s);
 int dc_server_run(dc_server* s, int64_t nreq, const int32_t* gid, const int64_t* roff,
                   const int32_t* u, const int32_t* v, const uint8_t* is_insert,
                   const double* arrival, double* latency, int32_t* step_of_req,
                   dc_serve_stats* st);
+/* The device bytes a request of `lines` lines, `insertions` of them
+ * insertions, counts against the budget (staged lines, change-list entries,
+ * row slots its insertions may add); -1 for bad sizes. */
+int64_t dc_server_request_bytes(int64_t lines, int64_t insertions);
 
 /* ---- synthetic inputs (host, deterministic, splitmix64-seeded) ------------ */
 /* Erdos-Renyi G(n, m): m distinct undirected edges, no self loops. */

@@ -44,6 +44,7 @@ struct orc_graph {
   int32_t* list_a; int32_t* list_b;
   int32_t* newc;
   int32_t* nbuf; int64_t nbuf_cap;
+  int32_t* pstamp; int32_t pgen;   /* parallel replay: atomic set membership */
 };
 
 static int is_frozen(const orc_graph* g, int32_t v) { return v >= g->n_owned; }
@@ -228,7 +229,7 @@ void orc_graph_destroy(orc_graph* g) {
   free(g->off); free(g->adj); free(g->prop);
   free(g->del); free(g->ins); free(g->xdel); free(g->xins);
   free(g->vstamp); free(g->cstamp); free(g->list_a); free(g->list_b);
-  free(g->newc); free(g->nbuf);
+  free(g->newc); free(g->nbuf); free(g->pstamp);
   free(g);
 }
 
@@ -594,4 +595,217 @@ int64_t orc_validate(const orc_graph* g, const int32_t* color, int32_t* max_colo
   }
   if (max_color) *max_color = mc;
   return bad;
+}
+
+/* ---- parallel replay (OpenMP) ----------------------------------------------
+ * The same independent-rule rounds as orc_recolor, each round's work split
+ * over threads: a round reads only the snapshot and writes only its own
+ * result lists, so the results, the trace counts and the colours equal the
+ * single-thread replay's (sets are order-free).  Timed as the all-cores CPU
+ * baseline; tests compare it with orc_recolor. */
+#ifdef _OPENMP
+#include <omp.h>
+#else
+static int omp_get_thread_num(void) { return 0; }
+#endif
+
+typedef struct {
+  int32_t* stamp; int32_t gen; int64_t cap;   /* mex marks, indexed by colour <= row length */
+  int32_t* buf; int64_t bcap;                 /* distinct-colour sort buffer */
+  int32_t* out; int64_t nout, ocap;           /* this thread's appended vertices */
+} par_scratch;
+
+static int64_t row_len(const orc_graph* g, int32_t v) {
+  return (g->off[v + 1] - g->off[v]) + g->ext_len[v];
+}
+
+static int out_push(par_scratch* s, int32_t v) {
+  if (s->nout == s->ocap) {
+    int64_t nc = s->ocap ? 2 * s->ocap : 1024;
+    int32_t* p = (int32_t*)realloc(s->out, sizeof(int32_t) * (size_t)nc);
+    if (!p) return ORC_ENOMEM;
+    s->out = p;
+    s->ocap = nc;
+  }
+  s->out[s->nout++] = v;
+  return ORC_OK;
+}
+
+/* true for exactly one caller per generation */
+static int par_claim(orc_graph* g, int32_t v) {
+  return __atomic_exchange_n(&g->pstamp[v], g->pgen, __ATOMIC_RELAXED) != g->pgen;
+}
+
+static void par_next_gen(orc_graph* g) {
+  if (++g->pgen == 0x7fffffff) {
+    memset(g->pstamp, 0, sizeof(int32_t) * (size_t)g->n);
+    g->pgen = 1;
+  }
+}
+
+static int32_t par_mex(const orc_graph* g, const int32_t* color, int32_t t, par_scratch* s) {
+  if (++s->gen == 0x7fffffff) {
+    memset(s->stamp, 0, sizeof(int32_t) * (size_t)s->cap);
+    s->gen = 1;
+  }
+  const int64_t lim = s->cap - 1;   /* the mex never exceeds the row length */
+  FOR_LIVE(g, t, w, {
+    int32_t c = color[w];
+    if (c >= 0 && c <= lim) s->stamp[c] = s->gen;
+  });
+  int32_t r = 0;
+  while (r <= lim && s->stamp[r] == s->gen) ++r;
+  return r;
+}
+
+static int64_t par_distinct(const orc_graph* g, const int32_t* color, int32_t t, par_scratch* s) {
+  int64_t k = 0;
+  FOR_LIVE(g, t, w, { s->buf[k++] = color[w]; });
+  if (k > 1) qsort(s->buf, (size_t)k, sizeof(int32_t), cmp_i32);
+  int64_t distinct = 0;
+  for (int64_t i = 0; i < k; ++i)
+    if (i == 0 || s->buf[i] != s->buf[i - 1]) ++distinct;
+  return distinct;
+}
+
+/* gather every thread's out list into list (count into *cnt), clearing them */
+static void par_gather(par_scratch* sc, int nt, int32_t* list, int64_t* cnt) {
+  int64_t n = 0;
+  for (int i = 0; i < nt; ++i) {
+    memcpy(list + n, sc[i].out, sizeof(int32_t) * (size_t)sc[i].nout);
+    n += sc[i].nout;
+    sc[i].nout = 0;
+  }
+  *cnt = n;
+}
+
+int orc_recolor_par(orc_graph* g, int rule, int32_t* color, int32_t* prev, int32_t max_loop,
+                    int64_t* trace, int32_t trace_cap, int32_t* n_phases, int32_t threads) {
+  trace_t tr = {trace, trace_cap, 0};
+  const int partition = g->n_owned < g->n;
+  if (!g->staged || rule != ORC_RULE_INDEPENDENT || threads < 1) return ORC_EINVAL;
+  g->staged = 0;
+  if (!g->pstamp) {
+    g->pstamp = (int32_t*)calloc((size_t)g->n + 1, sizeof(int32_t));
+    if (!g->pstamp) return ORC_ENOMEM;
+    g->pgen = 0;
+  }
+  int64_t maxlen = 1;
+  for (int32_t v = 0; v < g->n; ++v) {
+    const int64_t l = row_len(g, v);
+    if (l > maxlen) maxlen = l;
+  }
+  const int nt = threads;
+  par_scratch* sc = (par_scratch*)calloc((size_t)nt, sizeof(par_scratch));
+  int32_t* marked = (int32_t*)malloc(sizeof(int32_t) * ((size_t)g->n + 1));
+  int32_t* newc = g->newc;
+  int32_t* L = g->list_b;
+  int32_t* C = g->list_a;
+  int rc = sc && marked ? ORC_OK : ORC_ENOMEM;
+  for (int i = 0; rc == ORC_OK && i < nt; ++i) {
+    sc[i].cap = maxlen + 2;
+    sc[i].stamp = (int32_t*)calloc((size_t)sc[i].cap, sizeof(int32_t));
+    sc[i].bcap = maxlen + 1;
+    sc[i].buf = (int32_t*)malloc(sizeof(int32_t) * (size_t)sc[i].bcap);
+    if (!sc[i].stamp || !sc[i].buf) rc = ORC_ENOMEM;
+  }
+  int64_t nmarked = 0, nL = 0;
+  int change = 0;
+  if (rc) goto done;
+
+  for (int32_t i = 0; i < g->n; ++i) prev[i] = -1;
+  /* pre-loop rounds: targets of the lines (a set), their mex on the snapshot,
+   * then the commit (deletion rounds: only changed targets are written and
+   * marked; insertion rounds: every target) */
+  for (int kind = 0; kind < 4; ++kind) {
+    const orc_edge* e = kind == 0 ? g->del : kind == 1 ? g->xdel : kind == 2 ? g->ins : g->xins;
+    const int64_t ne = kind == 0 ? g->ndel : kind == 1 ? g->nxdel : kind == 2 ? g->nins : g->nxins;
+    const int ins = kind >= 2;
+    const int use_filter = kind == 2 && !partition;
+    if (ins ? (kind == 2 ? !count_filtered(e, ne, use_filter) : ne == 0) : ne == 0) continue;
+    par_next_gen(g);
+#pragma omp parallel for num_threads(nt) schedule(dynamic, 256)
+    for (int64_t i = 0; i < ne; ++i) {
+      par_scratch* s = &sc[omp_get_thread_num()];
+      int32_t a = e[i].a, b = e[i].b, t;
+      if (!ins) {
+        t = color[a] > color[b] ? a : b;
+      } else {
+        if (use_filter && !e[i].eq0) continue;
+        if (color[a] != color[b]) continue;
+        int64_t ka = is_frozen(g, a) ? INT64_MAX : par_distinct(g, color, a, s);
+        int64_t kb = is_frozen(g, b) ? INT64_MAX : par_distinct(g, color, b, s);
+        t = ka < kb ? a : b;
+      }
+      if (is_frozen(g, t) || !par_claim(g, t)) continue;
+      if (out_push(s, t)) continue;
+    }
+    int64_t nt_ = 0;
+    par_gather(sc, nt, C, &nt_);
+    int64_t changed = 0;
+#pragma omp parallel for num_threads(nt) schedule(dynamic, 64)
+    for (int64_t i = 0; i < nt_; ++i) newc[i] = par_mex(g, color, C[i], &sc[omp_get_thread_num()]);
+    for (int64_t i = 0; i < nt_; ++i) {
+      const int32_t t = C[i];
+      const int ch = newc[i] != color[t];
+      changed += ch;
+      if (ins || ch) {
+        prev[t] = color[t];
+        color[t] = newc[i];
+        marked[nmarked++] = t;
+      }
+    }
+    trace_add(&tr, kind == 0 ? ORC_PH_DEL : kind == 1 ? ORC_PH_XDEL : kind == 2 ? ORC_PH_INS : ORC_PH_XINS,
+              nt_, changed, 0);
+    if (ins ? nt_ > 0 : changed > 0) change = 1;
+  }
+  /* the marked set (a vertex can be marked by more than one round) */
+  par_next_gen(g);
+  for (int64_t i = 0; i < nmarked; ++i)
+    if (par_claim(g, marked[i])) L[nL++] = marked[i];
+
+  int32_t iter = 0;
+  while (change) {
+    if (iter >= max_loop) {
+      rc = ORC_ECAP;
+      goto done;
+    }
+    par_next_gen(g);
+#pragma omp parallel for num_threads(nt) schedule(dynamic, 64)
+    for (int64_t i = 0; i < nL; ++i) {
+      par_scratch* s = &sc[omp_get_thread_num()];
+      const int32_t v = L[i];
+      FOR_LIVE(g, v, w, {
+        if (is_frozen(g, w)) continue;
+        if (((color[v] == color[w] && v < w) || prev[v] < color[w]) && par_claim(g, w))
+          (void)out_push(s, w);
+      });
+    }
+    int64_t nC = 0, changed = 0;
+    par_gather(sc, nt, C, &nC);
+#pragma omp parallel for num_threads(nt) schedule(dynamic, 64)
+    for (int64_t i = 0; i < nC; ++i) newc[i] = par_mex(g, color, C[i], &sc[omp_get_thread_num()]);
+#pragma omp parallel for num_threads(nt) reduction(+ : changed)
+    for (int64_t i = 0; i < nC; ++i) {
+      const int32_t w = C[i];
+      changed += newc[i] != color[w];
+      prev[w] = color[w];
+      color[w] = newc[i];
+    }
+    trace_add(&tr, ORC_PH_LOOP, nL, nC, changed);
+    change = nC > 0;
+    int32_t* t = L; L = C; C = t;
+    nL = nC;
+    ++iter;
+  }
+done:
+  if (n_phases) *n_phases = tr.n;
+  for (int i = 0; sc && i < nt; ++i) {
+    free(sc[i].stamp);
+    free(sc[i].buf);
+    free(sc[i].out);
+  }
+  free(sc);
+  free(marked);
+  return rc;
 }

@@ -79,6 +79,13 @@ int orc_recolor(orc_graph* g, int rule, int32_t* color, int32_t* prev,
                 int32_t max_loop, int64_t* trace, int32_t trace_cap,
                 int32_t* n_phases);
 
+/* orc_recolor for the independent rule with each round's work split over
+ * `threads` OpenMP threads (the all-cores CPU baseline): the same colours,
+ * prev values and trace as orc_recolor. */
+int orc_recolor_par(orc_graph* g, int rule, int32_t* color, int32_t* prev,
+                    int32_t max_loop, int64_t* trace, int32_t trace_cap,
+                    int32_t* n_phases, int32_t threads);
+
 /* Textbook first-fit greedy: independent -> vertex-ID order, ldf -> priority
  * order (prop desc, ID asc).  Ghost colors (frozen) are taken from `color`. */
 void orc_static_greedy(orc_graph* g, int rule, int32_t* color);

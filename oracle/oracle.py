@@ -45,6 +45,9 @@ def lib():
         L.orc_recolor.restype = C.c_int
         L.orc_recolor.argtypes = [C.c_void_p, C.c_int, _i32p, _i32p, C.c_int32,
                                   _i64p, C.c_int32, C.POINTER(C.c_int32)]
+        L.orc_recolor_par.restype = C.c_int
+        L.orc_recolor_par.argtypes = [C.c_void_p, C.c_int, _i32p, _i32p, C.c_int32,
+                                      _i64p, C.c_int32, C.POINTER(C.c_int32), C.c_int32]
         L.orc_static_greedy.argtypes = [C.c_void_p, C.c_int, _i32p]
         L.orc_validate.restype = C.c_int64
         L.orc_validate.argtypes = [C.c_void_p, _i32p, C.POINTER(C.c_int32)]
@@ -121,6 +124,18 @@ class OracleGraph:
         phases = trace[: 4 * min(nph.value, trace_cap)].reshape(-1, 4)
         if rc != 0:
             raise RuntimeError(f"oracle: recolor failed ({rc})")
+        return phases
+
+    def recolor_par(self, rule, threads, max_loop=100000, trace_cap=100000):
+        """The independent rule's rounds split over `threads` OpenMP threads
+        (same colours and trace as recolor)."""
+        trace = np.zeros(4 * trace_cap, dtype=np.int64)
+        nph = C.c_int32(0)
+        rc = lib().orc_recolor_par(self._h, rule, self.color, self.prev, max_loop,
+                                   trace, trace_cap, C.byref(nph), int(threads))
+        phases = trace[: 4 * min(nph.value, trace_cap)].reshape(-1, 4)
+        if rc != 0:
+            raise RuntimeError(f"oracle: parallel recolor failed ({rc})")
         return phases
 
     def validate(self):

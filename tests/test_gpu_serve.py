@@ -48,15 +48,8 @@ def test_union_step_equals_alone(oracle_mod, driver):
         assert (got[off[i]:off[i + 1]] == og.color).all(), f"graph {i}"
 
 
-def test_serving_isolation(oracle_mod):
-    say("serving isolation")
-    graphs = _graphs()
-    n, U, V, off = d.union_edges(graphs)
-    arena = d.Graph(n, U, V)
-    arena.set_driver(d.DRIVER_FUSED)
-    arena.static_color(0)
-    rng = np.random.default_rng(5)
-    nreq = 80
+def _requests(graphs, nreq, rate, seed):
+    rng = np.random.default_rng(seed)
     gid = rng.integers(0, len(graphs), nreq).astype(np.int32)
     sizes = np.clip(np.exp(rng.normal(np.log(40), 1.0, nreq)), 1, 400).astype(np.int64)
     roff = np.zeros(nreq + 1, np.int64)
@@ -74,14 +67,34 @@ def test_serving_isolation(oracle_mod):
             else:
                 e = rng.integers(0, gu.size)
                 u[j], v[j], ins[j] = gu[e], gv[e], 0
-    arrival = np.cumsum(rng.exponential(1.0 / 2000.0, nreq))
-    srv = d.Server(arena, off, budget_lines=3000)
-    lat, step, st = srv.run(gid, roff, u, v, ins, arrival)
-    assert st.served == nreq and st.rejected == 0
-    assert (lat >= 0).all()
-    assert st.steps >= 2
-    got = arena.colors()
-    assert arena.validate()[0] == 0
+    arrival = np.cumsum(rng.exponential(1.0 / rate, nreq))
+    return gid, roff, u, v, ins, arrival
+
+
+def _lanes(graphs, nlanes, driver=d.DRIVER_PLAIN):
+    """Graph g goes to lane g % nlanes; returns (arenas, lane_of, first, size)."""
+    arenas, lane_of, first, size = [], [], [], []
+    place = {}
+    for l in range(nlanes):
+        mine = [g for g in range(len(graphs)) if g % nlanes == l]
+        n, U, V, off = d.union_edges([graphs[g] for g in mine])
+        a = d.Graph(n, U, V)
+        a.set_driver(driver)
+        a.static_color(0)
+        arenas.append(a)
+        for k, g in enumerate(mine):
+            place[g] = (l, int(off[k]), int(off[k + 1] - off[k]))
+    for g in range(len(graphs)):
+        lane_of.append(place[g][0])
+        first.append(place[g][1])
+        size.append(place[g][2])
+    return arenas, lane_of, first, size
+
+
+def _check_alone(oracle_mod, graphs, arenas, lane_of, first, size, req, step):
+    # every graph's colours equal those of its coalesced batches replayed
+    # alone on the oracle, in the order its lane ran them
+    gid, roff, u, v, ins, _ = req
     for g, (gn, gu, gv) in enumerate(graphs):
         og = oracle_mod.OracleGraph(gn, gu, gv)
         og.static_color(0)
@@ -91,7 +104,46 @@ def test_serving_isolation(oracle_mod):
             idx = np.concatenate([np.arange(roff[r], roff[r + 1]) for r in rs])
             og.apply_batch(u[idx], v[idx], ins[idx])
             og.recolor(0)
-        assert (got[off[g]:off[g + 1]] == og.color).all(), f"graph {g}"
+        got = arenas[lane_of[g]].colors()[first[g]:first[g] + size[g]]
+        assert (got == og.color).all(), f"graph {g}"
+
+
+@pytest.mark.parametrize("nlanes,driver", [(1, d.DRIVER_FUSED), (2, d.DRIVER_PLAIN),
+                                           (3, d.DRIVER_PLAIN)])
+def test_serving_isolation(oracle_mod, nlanes, driver):
+    # requests arrive faster than steps finish: they are admitted while
+    # steps run and coalesced; each graph ends with the colours of its
+    # coalesced batches applied alone
+    say("serving isolation", nlanes, driver)
+    graphs = _graphs()
+    arenas, lane_of, first, size = _lanes(graphs, nlanes, driver)
+    req = _requests(graphs, 120, 3000.0, 5)
+    budget = d.request_bytes(3000, 3000)
+    srv = d.Server(arenas, lane_of, first, size, budget_bytes=budget)
+    lat, step, st = srv.run(*req)
+    assert st.served == req[0].size and st.rejected == 0 and st.lanes == nlanes
+    assert (lat >= 0).all()
+    assert st.steps >= 2 and st.max_inflight_bytes <= budget
+    assert any(np.unique(step[req[0] == g]).size < (req[0] == g).sum() for g in range(len(graphs))), \
+        "no coalescing happened"
+    for a in arenas:
+        assert a.validate()[0] == 0
+    _check_alone(oracle_mod, graphs, arenas, lane_of, first, size, req, step)
+
+
+def test_serving_admission_waits_for_memory(oracle_mod):
+    # a budget of about two requests: later arrivals wait until a step frees
+    # its bytes; results stay those of the graphs alone
+    say("serving admission waits")
+    graphs = _graphs()
+    arenas, lane_of, first, size = _lanes(graphs, 2)
+    req = _requests(graphs, 60, 20000.0, 9)
+    budget = d.request_bytes(800, 800)
+    srv = d.Server(arenas, lane_of, first, size, budget_bytes=budget)
+    lat, step, st = srv.run(*req)
+    assert st.served == req[0].size and st.rejected == 0
+    assert st.admit_waits > 0 and st.max_inflight_bytes <= budget
+    _check_alone(oracle_mod, graphs, arenas, lane_of, first, size, req, step)
 
 
 def test_serving_rejects_oversized_request():
@@ -100,7 +152,7 @@ def test_serving_rejects_oversized_request():
     n, U, V, off = d.union_edges(graphs)
     arena = d.Graph(n, U, V)
     arena.static_color(0)
-    srv = d.Server(arena, off, budget_lines=10)
+    srv = d.Server.one_lane(arena, off, budget_bytes=d.request_bytes(10, 10))
     gid = np.array([0, 1], np.int32)
     roff = np.array([0, 5, 25], np.int64)
     u = np.array([0, 1, 2, 3, 4] + list(range(20)), np.int32)
@@ -109,3 +161,21 @@ def test_serving_rejects_oversized_request():
     lat, step, st = srv.run(gid, roff, u, v, ins, np.array([0.0, 0.0]))
     assert st.served == 1 and st.rejected == 1
     assert lat[1] == -1.0 and lat[0] >= 0
+
+
+def test_server_argument_checks():
+    graphs = _graphs()[:2]
+    n, U, V, off = d.union_edges(graphs)
+    arena = d.Graph(n, U, V)
+    with pytest.raises(d.DynColorError):
+        d.Server([arena], [0, 0], [0, int(off[1])], [int(off[1]) + 5, 10])   # past the arena
+    with pytest.raises(d.DynColorError):
+        d.Server([arena], [0, 0], [0, 2], [int(off[1]), 10])                # overlapping graphs
+    srv = d.Server.one_lane(arena, off, budget_bytes=1 << 20)
+    # a non-empty request with null line arrays
+    import ctypes as C
+    lat = np.zeros(1)
+    st = d.ServeStats()
+    rc = d.lib().dc_server_run(srv._h, 1, np.zeros(1, np.int32), np.array([0, 3], np.int64),
+                               None, None, None, np.zeros(1), lat, np.zeros(1, np.int32), C.byref(st))
+    assert rc == d.ERR_INVALID

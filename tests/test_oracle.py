@@ -261,3 +261,49 @@ def test_partition_sample_fixture(oracle_mod):
     assert og.color.tolist() == p["expected"]["colors"]
     assert tr.tolist() == p["expected"]["trace"]
     assert og.validate() == (p["expected"]["monochromatic"], p["expected"]["max_color"])
+
+
+@pytest.mark.parametrize("threads", [1, 3, 8])
+def test_parallel_replay_equals_single_thread(oracle_mod, threads):
+    # the all-cores CPU baseline: the same colours, prev values and traces as
+    # the single-thread oracle, on multi-batch R-MAT streams and on
+    # partition shares with frozen ghosts
+    import dyncolor_amd as d
+    u, v = d.gen_rmat(12, 16, seed=11)
+    n = 1 << 12
+    batches = d.gen_batches(n, u, v, 3, 1500, 0.5, 12)
+    a = oracle_mod.OracleGraph(n, u, v)
+    b = oracle_mod.OracleGraph(n, u, v)
+    a.static_color(0)
+    b.static_color(0)
+    for bu, bv, bi in batches:
+        a.apply_batch(bu, bv, bi)
+        b.apply_batch(bu, bv, bi)
+        ta = a.recolor(0)
+        tb = b.recolor_par(0, threads)
+        assert ta.tolist() == tb.tolist()
+        assert (a.color == b.color).all() and (a.prev == b.prev).all()
+    rng = random.Random(5 + threads)
+    for trial in range(6):
+        nn = rng.randint(50, 2000)
+        no = rng.randint(10, nn - 5)
+        uu, vv = d.gen_er(nn, 4 * nn, 60 + trial)
+        keep = (uu < no) | (vv < no)
+        uu, vv = uu[keep], vv[keep]
+        init = np.array([rng.randint(0, 5) for _ in range(nn)], np.int32)
+        lines = []
+        while len(lines) < nn // 4:
+            x, y = rng.sample(range(nn), 2)
+            if x >= no and y >= no:
+                continue
+            lines.append((x, y, rng.randint(0, 1)))
+        la = np.array(lines, np.int32)
+        a = oracle_mod.OracleGraph(nn, uu, vv, n_owned=no)
+        b = oracle_mod.OracleGraph(nn, uu, vv, n_owned=no)
+        for g in (a, b):
+            g.set_colors(init)
+            g.apply_batch(la[:, 0], la[:, 1], la[:, 2].astype(np.uint8))
+        ta = a.recolor(0)
+        tb = b.recolor_par(0, threads)
+        assert ta.tolist() == tb.tolist()
+        assert (a.color == b.color).all()
