@@ -46,7 +46,8 @@ class Options(C.Structure):
     _fields_ = [("slack_frac", C.c_float), ("slack_min", C.c_int32), ("max_rounds", C.c_int32),
                 ("driver", C.c_int32), ("device", C.c_int32), ("spare_slots", C.c_int32),
                 ("exact_rounds", C.c_int32), ("list_limit", C.c_int32),
-                ("pull_mode", C.c_int32), ("reserved0", C.c_int32), ("heavy_slots", C.c_int64)]
+                ("pull_mode", C.c_int32), ("group_deletions", C.c_int32),
+                ("heavy_slots", C.c_int64)]
 
 
 class Stats(C.Structure):

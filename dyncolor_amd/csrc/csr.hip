@@ -15,6 +15,8 @@
 #include <mutex>
 #include <string>
 
+#include <hipcub/hipcub.hpp>
+
 #include "dc_device.h"
 #include "dc_internal.h"
 
@@ -412,6 +414,7 @@ void free_state(Graph* g) {
   f(g->hidx); f(g->hist); f(g->hrowv); f(g->hch);
   f(g->fbuf); f(g->fcnt); f(g->hbuf); f(g->partial);
   f(g->del); f(g->ins); f(g->xdel); f(g->xins); f(g->ovf);
+  f(g->tkeys); f(g->tlong); f(g->sort_tmp);
   f(g->ctr); f(g->aux); f(g->tail); f(g->scan_tmp);
   f(g->scratch_batch.u); f(g->scratch_batch.v); f(g->scratch_batch.ins);
   if (g->hctr) (void)hipHostFree(g->hctr);
@@ -935,6 +938,221 @@ __global__ void k_tombstone(const int2* del, uint32_t ndel, const int2* xdel, ui
   wave_add_u64(dslots, &ctr->delslots);
 }
 
+// ---- deletions grouped by scanned row ----------------------------------------
+// A row that many deletion lines scan (a hub of an R-MAT graph whose edges a
+// batch deletes by the hundred) is read once for all of them instead of once
+// per line: each line becomes the key (scanned row << 32 | other endpoint),
+// the keys are sorted, and the first line of each row's run scans the row,
+// matching every slot's neighbour against the run's sorted targets by binary
+// search.  Runs over rows of kLongRow or more slots go to whole workgroups.
+// The slots retired are those the per-line pass retires: every copy of each
+// deleted pair in both rows.  The scanned row of a pair does not depend on the
+// line's orientation (the shorter row, the lower ID on a tie), so every copy
+// of a pair, and its twin, belongs to exactly one run: slots are retired with
+// plain stores (only the histogram counts, which runs of different rows
+// share, stay atomic).
+constexpr uint32_t kLongRow = 4096;
+
+__device__ __forceinline__ uint64_t tomb_key(int32_t x, int32_t y) {
+  return ((uint64_t)(uint32_t)x << 32) | (uint32_t)y;
+}
+
+// scanned row and other endpoint of each line (the shorter row, the lower
+// ID on a tie; the owned endpoint of a cross line)
+__global__ void k_tomb_keys(const int2* del, uint32_t ndel, const int2* xdel, uint32_t nxdel,
+                            GraphView G, uint64_t* keys, Counters* ctr) {
+  const int64_t items = (int64_t)ndel + nxdel;
+  unsigned long long dslots = 0;
+  for (int64_t it = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; it < items;
+       it += (int64_t)gridDim.x * blockDim.x) {
+    const bool cross = it >= (int64_t)ndel;
+    const int2 e = cross ? xdel[it - ndel] : del[it];
+    int32_t x = e.x, y = e.y;
+    const uint32_t rx = G.row[x].y;
+    uint32_t ry = 0;
+    if (!cross) {
+      ry = G.row[y].y;
+      if (ry < rx || (ry == rx && y < x)) {
+        x = e.y;
+        y = e.x;
+      }
+    }
+    dslots += (unsigned long long)rx + ry;
+    keys[it] = tomb_key(x, y);
+  }
+  wave_add_u64(dslots, &ctr->delslots);
+}
+
+// a live slot of row x (owner hx) whose neighbour w is a target: retire it
+// and its twin (the run of row x is their only writer)
+__device__ __forceinline__ uint32_t tomb_retire(const GraphView& G, int32_t x, int32_t hx,
+                                                uint64_t s, int32_t w) {
+  const int32_t c = G.ccol[s];
+  G.col[s] = -1;
+  G.ccol[s] = -1;
+  if (hx >= 0 && c >= 0 && c < kHistW) atomicSub(&G.hist[(uint64_t)hx * kHistW + c], 1u);
+  if (w >= G.n_owned) return 1;
+  const uint64_t s2 = (uint64_t)G.row[w].x + (uint32_t)G.twin[s];
+  const int32_t c2 = G.ccol[s2];
+  G.col[s2] = -1;
+  G.ccol[s2] = -1;
+  const int32_t hw = G.hidx[w];
+  if (hw >= 0 && c2 >= 0 && c2 < kHistW) atomicSub(&G.hist[(uint64_t)hw * kHistW + c2], 1u);
+  return 2;
+}
+
+// is w one of the run's targets (keys[lo, hi) of row x, sorted)?
+__device__ __forceinline__ bool tomb_target(const uint64_t* keys, uint32_t lo, uint32_t hi,
+                                            int32_t x, int32_t w) {
+  const uint64_t k = tomb_key(x, w);
+  const uint32_t end = hi;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (keys[mid] < k) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo < end && keys[lo] == k;
+}
+
+// Runs of the sorted keys.  An 8-lane group owns 8 consecutive lines: its
+// lanes flag the lines that start a run (a ballot gives the group's starts),
+// then the group takes those runs in turn; a run that goes on past the 8
+// lines is followed to its end (8 keys per step).  A short row (fewer than
+// kMidRow slots) is read by the group in one round trip (8 lanes x 8
+// consecutive slots); runs over longer rows go to k_tomb_mid (a wave per run)
+// or k_tomb_long (a workgroup per run).
+constexpr uint32_t kMidRow = 64;
+
+__global__ __launch_bounds__(kBlock) void k_tomb_runs(const uint64_t* keys, uint32_t items,
+                                                      GraphView G, int2* mids, uint32_t* nmid,
+                                                      int2* longs, uint32_t* nlong, Counters* ctr) {
+  constexpr int kL = 8;
+  const int lane = lane_id();
+  const int sub = lane & (kL - 1);
+  const int gl0 = lane & ~(kL - 1);   // the group's first lane
+  const uint64_t grp = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / kL;
+  const uint64_t ngrp = ((uint64_t)gridDim.x * blockDim.x) / kL;
+  unsigned long long tomb = 0;
+  for (uint64_t base = grp * kL; base < items; base += ngrp * kL) {
+    const uint64_t i = base + sub;
+    const bool valid = i < items;
+    const int64_t xi = valid ? (int64_t)(keys[i] >> 32) : -1;
+    const int64_t xp = valid && i > 0 ? (int64_t)(keys[i - 1] >> 32) : -2;
+    const bool start = valid && xp != xi;
+    uint32_t m = (uint32_t)(__ballot(start) >> gl0) & 0xffu;
+    while (m) {
+      const int b = __ffs(m) - 1;
+      m &= m - 1u;
+      const int32_t x = (int32_t)__shfl((int)xi, gl0 + b);
+      const uint32_t p = (uint32_t)(base + b);
+      // the run's end: the next start in the group, else followed past it
+      uint32_t end;
+      if (m) {
+        end = (uint32_t)base + (uint32_t)(__ffs(m) - 1);
+      } else {
+        end = (uint32_t)min<uint64_t>(base + kL, items);
+        for (;;) {
+          const uint32_t j = end + (uint32_t)sub;
+          const bool same = j < items && (int32_t)(keys[j] >> 32) == x;
+          uint32_t stop = same ? 0xffffffffu : j;
+          for (int off = 1; off < kL; off <<= 1)
+            stop = min(stop, (uint32_t)__shfl_xor((int)stop, off, kL));
+          if (stop != 0xffffffffu) {
+            end = stop;
+            break;
+          }
+          end += kL;
+        }
+      }
+      const uint2 r = G.row[x];
+      if (r.y >= kMidRow) {
+        if (sub == 0) {
+          if (r.y >= kLongRow) longs[atomicAdd(nlong, 1u)] = make_int2((int32_t)p, (int32_t)end);
+          else mids[atomicAdd(nmid, 1u)] = make_int2((int32_t)p, (int32_t)end);
+        }
+        continue;
+      }
+      const int32_t hx = G.hidx[x];
+      const bool one = end == p + 1;
+      const int32_t y0 = (int32_t)(uint32_t)keys[p];
+      int32_t cv[kMidRow / kL];
+#pragma unroll
+      for (int q = 0; q < (int)(kMidRow / kL); ++q) {
+        const uint32_t jj = (uint32_t)sub + q * kL;
+        cv[q] = jj < r.y ? G.col[(uint64_t)r.x + jj] : -2;
+      }
+#pragma unroll
+      for (int q = 0; q < (int)(kMidRow / kL); ++q) {
+        if (cv[q] < 0) continue;
+        if (one ? cv[q] != y0 : !tomb_target(keys, p, end, x, cv[q])) continue;
+        tomb += tomb_retire(G, x, hx, (uint64_t)r.x + (uint32_t)sub + q * kL, cv[q]);
+      }
+    }
+  }
+  wave_add_u64(tomb, &ctr->tomb);
+}
+
+// runs over rows of kMidRow to kLongRow slots: one wave per run
+__global__ __launch_bounds__(kBlock) void k_tomb_mid(const uint64_t* keys, const int2* mids,
+                                                     const uint32_t* nmid, GraphView G,
+                                                     Counters* ctr) {
+  constexpr int kUnroll = 4;
+  const int lane = lane_id();
+  const uint32_t gwave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
+  unsigned long long tomb = 0;
+  const uint32_t nm = *nmid;
+  for (uint32_t q = gwave; q < nm; q += nwaves) {
+    const int2 run = mids[q];
+    const int32_t x = (int32_t)(keys[run.x] >> 32);
+    const uint2 r = G.row[x];
+    const int32_t hx = G.hidx[x];
+    for (uint32_t j0 = lane; j0 < r.y; j0 += kUnroll * 64) {
+      int32_t cv[kUnroll];
+#pragma unroll
+      for (int u = 0; u < kUnroll; ++u) {
+        const uint32_t jj = j0 + u * 64;
+        cv[u] = jj < r.y ? G.col[(uint64_t)r.x + jj] : -2;
+      }
+#pragma unroll
+      for (int u = 0; u < kUnroll; ++u) {
+        if (cv[u] < 0 || !tomb_target(keys, (uint32_t)run.x, (uint32_t)run.y, x, cv[u])) continue;
+        tomb += tomb_retire(G, x, hx, (uint64_t)r.x + j0 + u * 64, cv[u]);
+      }
+    }
+  }
+  wave_add_u64(tomb, &ctr->tomb);
+}
+
+// runs over long rows: one workgroup per run
+__global__ __launch_bounds__(kBlock) void k_tomb_long(const uint64_t* keys, const int2* longs,
+                                                      const uint32_t* nlong, GraphView G,
+                                                      Counters* ctr) {
+  constexpr int kUnroll = 8;
+  unsigned long long tomb = 0;
+  const uint32_t nl = *nlong;
+  for (uint32_t q = blockIdx.x; q < nl; q += gridDim.x) {
+    const int2 run = longs[q];
+    const int32_t x = (int32_t)(keys[run.x] >> 32);
+    const uint2 r = G.row[x];
+    const int32_t hx = G.hidx[x];
+    for (uint32_t j0 = threadIdx.x; j0 < r.y; j0 += kUnroll * kBlock) {
+      int32_t cv[kUnroll];
+#pragma unroll
+      for (int u = 0; u < kUnroll; ++u) {
+        const uint32_t jj = j0 + u * kBlock;
+        cv[u] = jj < r.y ? G.col[(uint64_t)r.x + jj] : -2;
+      }
+#pragma unroll
+      for (int u = 0; u < kUnroll; ++u) {
+        if (cv[u] < 0 || !tomb_target(keys, (uint32_t)run.x, (uint32_t)run.y, x, cv[u])) continue;
+        tomb += tomb_retire(G, x, hx, (uint64_t)r.x + j0 + u * kBlock, cv[u]);
+      }
+    }
+  }
+  wave_add_u64(tomb, &ctr->tomb);
+}
+
 // regrow: copy every row's slots [0, min(used, old cap)) to the new layout
 __global__ void k_copy_rows(int32_t n, const uint32_t* old_start, const uint32_t* old_cap,
                             const int32_t* old_col, const int32_t* old_ccol,
@@ -1105,6 +1323,18 @@ static int ensure_lists(Graph* g, int64_t k) {
   DC_HIP(hipMalloc(&g->xins, sizeof(int2) * (size_t)c));
   DC_HIP(hipMalloc(&g->ovf, sizeof(int4) * (size_t)(2 * c)));
   g->list_cap = c;
+  // grouped deletions: keys, sorted keys, long-row runs, radix-sort storage
+  f(g->tkeys); f(g->tlong); f(g->sort_tmp);
+  g->tkeys = nullptr;
+  g->tlong = nullptr;
+  g->sort_tmp = nullptr;
+  DC_HIP(hipMalloc(&g->tkeys, sizeof(uint64_t) * 2 * (size_t)c));
+  DC_HIP(hipMalloc(&g->tlong, sizeof(int2) * 2 * (size_t)c));   // mid runs, then long runs
+  size_t bytes = 0;
+  DC_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, bytes, g->tkeys, g->tkeys + c, (int)c, 0, 63,
+                                           g->stream));
+  DC_HIP(hipMalloc(&g->sort_tmp, bytes));
+  g->sort_tmp_bytes = bytes;
   return DC_OK;
 }
 
@@ -1138,9 +1368,34 @@ int csr_apply(Graph* g, const Batch& b, int rule) {
     g->staged_moves = 1;
   }
   uint32_t items = c.n_del + c.n_xdel;
-  if (items > 0) {
-    // row slots (with slack) per owned vertex: 32 or more -> wide groups
-    const bool wide = g->rows_end >= 32ull * (uint64_t)std::max(g->n_owned, 1);
+  // row slots (with slack) per owned vertex: 32 or more -> wide groups
+  const bool wide = g->rows_end >= 32ull * (uint64_t)std::max(g->n_owned, 1);
+  const uint32_t group_at = g->opt.group_deletions == 0  ? (1u << 16)
+                            : g->opt.group_deletions < 0 ? 0xffffffffu
+                                                         : (uint32_t)g->opt.group_deletions;
+  if (items > 0 && items >= group_at) {
+    // many lines: grouped by scanned row (each row read once)
+    uint64_t* keys = g->tkeys;
+    uint64_t* sorted = g->tkeys + g->list_cap;
+    uint32_t* nlong = g->aux + 12;
+    DC_HIP(hipMemsetAsync(nlong, 0, sizeof(uint32_t), g->stream));
+    hipLaunchKernelGGL(k_tomb_keys, dim3(grid_for(items)), dim3(kBlock), 0, g->stream, g->del, c.n_del,
+                       g->xdel, c.n_xdel, g->view(), keys, g->ctr);
+    DC_HIP(hipGetLastError());
+    size_t bytes = g->sort_tmp_bytes;
+    DC_HIP(hipcub::DeviceRadixSort::SortKeys(g->sort_tmp, bytes, keys, sorted, (int)items, 0, 63,
+                                             g->stream));
+    uint32_t* nmid = g->aux + 13;
+    DC_HIP(hipMemsetAsync(nmid, 0, sizeof(uint32_t), g->stream));
+    hipLaunchKernelGGL(k_tomb_runs, dim3(grid_for(((int64_t)items + 7) / 8 * 8)), dim3(kBlock), 0,
+                       g->stream, sorted, items,
+                       g->view(), g->tlong, nmid, g->tlong + g->list_cap, nlong, g->ctr);
+    hipLaunchKernelGGL(k_tomb_mid, dim3(2048), dim3(kBlock), 0, g->stream, sorted, g->tlong, nmid,
+                       g->view(), g->ctr);
+    hipLaunchKernelGGL(k_tomb_long, dim3(1024), dim3(kBlock), 0, g->stream, sorted,
+                       g->tlong + g->list_cap, nlong, g->view(), g->ctr);
+    DC_HIP(hipGetLastError());
+  } else if (items > 0) {
     if (wide)
       hipLaunchKernelGGL((k_tombstone<16, 8>), dim3(grid_for((int64_t)items * 16)), dim3(kBlock), 0,
                          g->stream, g->del, c.n_del, g->xdel, c.n_xdel, g->view(), g->ctr);

@@ -148,6 +148,12 @@ struct Graph {
   int2* xins = nullptr;
   int4* ovf = nullptr;        // insertion overflow {vertex, pos, value, 0}
   int64_t list_cap = 0;
+  // deletions grouped by scanned row (csr_apply): keys [2][list_cap], runs
+  // over long rows, radix-sort storage
+  uint64_t* tkeys = nullptr;
+  int2* tlong = nullptr;
+  void* sort_tmp = nullptr;
+  size_t sort_tmp_bytes = 0;
   bool staged = false;
   int64_t staged_k = 0, staged_ins = 0, staged_del = 0;
   int32_t staged_regrows = 0;

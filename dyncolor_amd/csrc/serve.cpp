@@ -40,10 +40,11 @@ namespace {
 
 // Device bytes a request may occupy while in flight: per line its staged copy
 // (u, v, flag: 9 B), its entries in the four change lists and the overflow
-// list (64 B of list capacity), and per insertion the two row slots it adds
+// list (64 B of list capacity), its two deletion sort keys (16 B), and per
+// insertion the two row slots it adds
 // (neighbour, slot colour, reverse link: 2 x 12 B) in case they land in the
 // spare tail.
-constexpr int64_t kBytesPerLine = 9 + 64;
+constexpr int64_t kBytesPerLine = 9 + 64 + 16;
 constexpr int64_t kBytesPerInsertion = 24;
 
 struct Lane {

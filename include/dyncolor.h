@@ -77,7 +77,10 @@ typedef struct {
                           than heavy_slots slots and whose candidate set more than
                           doubled; 1: every round pulls; -1: no round pulls.
                           Results do not depend on it.                               */
-  int32_t reserved0;
+  int32_t group_deletions; /* batches with at least this many deletion lines tombstone
+                          them grouped by scanned row (lines sorted by row; each row
+                          read once for all its lines); 0 (default): 65536; -1:
+                          never.  Results do not depend on it.                       */
   int64_t heavy_slots; /* the pull threshold in row slots; 0 (default): 1/8 of the
                           laid-out row slots, at least 65536                         */
 } dc_options;
