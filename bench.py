@@ -99,8 +99,19 @@ def roofline(kernel, algo_bytes, kernel_ms, launches):
     return out
 
 
-def set_driver(g, d, args):
-    g.set_driver(d.DRIVER_FUSED if args.driver == "fused" else d.DRIVER_PLAIN, args.blocks_per_cu)
+TUNING_PATH = os.path.join(ROOT, "profiles", "tuning.json")
+
+
+def set_driver(g, d, args, workload=None):
+    """The requested driver, or the tuned parameters saved for this device
+    and workload (driver included) when --driver is not given."""
+    tuned = getattr(args, "tuned", {}).get(workload) if workload else None
+    if tuned and args.driver is None:
+        from dyncolor_amd import tune
+        tune.apply(g, tuned)
+        return
+    drv = {"fused": d.DRIVER_FUSED, "graph": d.DRIVER_GRAPH}.get(args.driver, d.DRIVER_PLAIN)
+    g.set_driver(drv, args.blocks_per_cu)
 
 
 def cpu_info():
@@ -138,7 +149,7 @@ def run_stream(args, d, rule, n, u, v, workload):
         f"colors {maxc0 + 1}, monochromatic {mono0}")
     dev_batches = [g.upload_batch(*b) for b in batches]
     g.set_profiling(False)
-    set_driver(g, d, args)
+    set_driver(g, d, args, args.stream_key)
     prof = {"ms": 0.0, "bytes": 0, "launches": 0, "kms": 0.0, "rbytes": 0, "rl": 0}
     terms = {"updates": 0, "del_row_slots": 0, "lc_vertices": 0, "lc_slots": 0, "loop_rounds": 0,
              "sum_C": 0, "batches": 0, "kernel_ms": 0.0}
@@ -160,6 +171,16 @@ def run_stream(args, d, rule, n, u, v, workload):
         terms["batches"] += 1
         terms["kernel_ms"] += st.ms_recolor_kernel
 
+    if args.tune:
+        from dyncolor_amd import tune
+        for i in range(args.warmup):
+            g.apply_device_batch(dev_batches[i])
+            g.recolor(rule)
+        tb = d.gen_batches(n, u, v, 4, args.batch, 0.5, args.seed + 2000)
+        best, sweep = tune.tune(g, tb, rule, reps=args.tune_reps, log=log)
+        tune.save(args.tuning_path, args.stream_key, best, sweep, {"batch": args.batch})
+        log(f"{args.stream_key} tuned: {best}")
+        return {"tuned": best, "sweep": sweep}
     colors_after_first = None
     for i in range(args.warmup):
         g.apply_device_batch(dev_batches[i])
@@ -205,7 +226,7 @@ def run_stream(args, d, rule, n, u, v, workload):
         gc.collect()
         g = d.Graph(n, u, v)
         g.static_color(rule)
-        set_driver(g, d, args)
+        set_driver(g, d, args, args.stream_key)
         dev_batches = [g.upload_batch(*b) for b in batches]
         replay_rounds = 0
         last_log = time.perf_counter()
@@ -231,10 +252,10 @@ def run_stream(args, d, rule, n, u, v, workload):
     lat_ms = np.array(lat) * 1e3
     out = base_record(value, elapsed, args.steps, args.warmup, {
         "workload": workload, "edges": int(u.size), "vertices": int(n), "batch": args.batch,
-        "rule": args.rule, "driver": args.driver, "colors": int(maxc + 1),
-        "monochromatic": int(mono)})
+        "rule": args.rule, "driver": args.driver or "plain", "colors": int(maxc + 1),
+        "monochromatic": int(mono), "tuning": args.tuned.get(args.stream_key) or "default"})
     out["latency_ms"] = {"p50": pct(lat_ms, 50), "p99": pct(lat_ms, 99), "max": float(lat_ms.max())}
-    if args.driver == "plain" and rule == d.RULE_INDEPENDENT and prof["launches"]:
+    if args.driver != "fused" and rule == d.RULE_INDEPENDENT and prof["launches"]:
         out["roofline"] = roofline("k_round_c (count pass)", prof["bytes"], prof["ms"],
                                    prof["launches"])
     else:
@@ -360,8 +381,17 @@ def run_c4(args, d, rule, steps, warmup):
     n, U, V, off = d.union_edges(graphs)
     arena = d.Graph(n, U, V)
     arena.static_color(rule)
-    set_driver(arena, d, args)
+    set_driver(arena, d, args, "c4")
     ub = [arena.upload_batch(*d.union_batch([p[s] for p in per], off)) for s in range(nsteps)]
+    if args.tune:
+        from dyncolor_amd import tune
+        arena.apply_device_batch(ub[0])
+        arena.recolor(rule)
+        tb = [d.union_batch([p[s] for p in per], off) for s in range(1, min(3, nsteps))]
+        best, sweep = tune.tune(arena, tb, rule, reps=args.tune_reps, log=log)
+        tune.save(args.tuning_path, "c4", best, sweep, {"graphs": len(graphs), "lines_per_step": int(step_lines[-1])})
+        log(f"c4 tuned: {best}")
+        return {"tuned": best, "sweep": sweep}
     for s in range(warmup):
         arena.apply_device_batch(ub[s])
         arena.recolor(rule)
@@ -390,6 +420,7 @@ def run_c4(args, d, rule, steps, warmup):
     for i, (gn, gu, gv) in enumerate(graphs):
         g = d.Graph(gn, gu, gv)
         g.static_color(rule)
+        set_driver(g, d, args, "c4")
         handles.append((g, [g.upload_batch(*per[i][s]) for s in range(nsteps)]))
     for s in range(nsteps):
         tb = time.perf_counter()
@@ -411,6 +442,7 @@ def run_c4(args, d, rule, steps, warmup):
         "graphs": len(graphs), "vertices": int(n), "edges": int(tot_edges),
         "lines_per_step": int(step_lines[-1]), "loop_rounds": rounds, "monochromatic": int(mono),
         "plain_value": plain, "plain_seconds": t_plain, "plain_calls_per_step": len(graphs),
+        "tuning": args.tuned.get("c4") or "default",
         "speedup_vs_plain": batched / plain, "bit_equal_to_plain": bool(same)}
 
 
@@ -617,7 +649,8 @@ def main():
     ap.add_argument("--edge-factor", type=int, default=16)
     ap.add_argument("--batch", type=int, default=100000)
     ap.add_argument("--rule", choices=["independent", "ldf"], default="independent")
-    ap.add_argument("--driver", choices=["plain", "fused"], default="plain")
+    ap.add_argument("--driver", choices=["plain", "fused", "graph"], default=None,
+                    help="recolour driver (default: the tuned one, else plain)")
     ap.add_argument("--blocks-per-cu", type=int, default=0)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--graphs", type=int, default=512)
@@ -630,6 +663,13 @@ def main():
     ap.add_argument("--c3-scale", type=int, default=24)
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--tune", action="store_true",
+                    help="run the autotuner on the workload (c2: the batch stream; c4: the "
+                         "arena) and save its choice and sweep to profiles/tuning.json")
+    ap.add_argument("--tune-reps", type=int, default=4)
+    ap.add_argument("--tuning-path", default=TUNING_PATH,
+                    help="where tuned parameters are read from (and written by --tune)")
+    ap.add_argument("--no-tuning", action="store_true", help="ignore profiles/tuning.json")
     ap.add_argument("--no-batched", action="store_true", help="default run: skip the c4 sub-record")
     ap.add_argument("--no-served", action="store_true", help="default run: skip the c5 sub-record")
     ap.add_argument("--events-after", action="store_true",
@@ -658,6 +698,16 @@ def main():
 
     import dyncolor_amd as d
     rule = d.RULE_INDEPENDENT if args.rule == "independent" else d.RULE_LDF
+    args.tuned = {}
+    if not args.no_tuning and not args.tune and d.device_count() > 0:
+        from dyncolor_amd import tune
+        for w in ("c2", "c4"):
+            t = tune.load(args.tuning_path, w)
+            if t:
+                args.tuned[w] = t
+                log(f"tuned parameters for {w}: {t}")
+    args.stream_key = "c2" if args.workload == "c2" and args.scale == 22 and args.batch == 100000 \
+        and rule == d.RULE_INDEPENDENT else "none"
     if d.device_count() < 1:
         print("no HIP device visible", file=sys.stderr)
         sys.exit(1)
@@ -675,6 +725,10 @@ def main():
                          f"rmat{args.scale}_ef{args.edge_factor}_{args.steps}x{args.batch}_mixed50")
         del u, v
         gc.collect()
+        if args.tune:
+            out = {"c2": out, "c4": run_c4(args, d, rule, 2, 1)}
+            print(json.dumps(out), flush=True)
+            return
         if not args.no_batched:
             b = run_c4(args, d, rule, args.c4_steps, 1)
             out["batched"] = b
@@ -689,6 +743,9 @@ def main():
         out = run_c3(args, d, rule)
     elif args.workload == "c4":
         b = run_c4(args, d, rule, args.steps, args.warmup)
+        if args.tune:
+            print(json.dumps(b), flush=True)
+            return
         out = base_record(b["value"], b["ms_per_step"] / 1e3 * args.steps, args.steps, args.warmup,
                           {"workload": b["workload"], "graphs": b["graphs"], "rule": args.rule,
                            "driver": args.driver})

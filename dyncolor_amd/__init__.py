@@ -18,6 +18,7 @@ RULE_INDEPENDENT = 0
 RULE_LDF = 1
 DRIVER_PLAIN = 0
 DRIVER_FUSED = 1
+DRIVER_GRAPH = 2
 
 OK = 0
 ERR_INVALID = -1
@@ -34,7 +35,8 @@ EXPORTS = [
     "dc_graph_info", "dc_colors_set", "dc_colors_get", "dc_static_color", "dc_apply_batch",
     "dc_batch_create", "dc_batch_destroy", "dc_apply_device_batch", "dc_recolor",
     "dc_process_batch", "dc_validate", "dc_trace_get", "dc_neighbors", "dc_prop_get",
-    "dc_compact", "dc_compact_noalloc", "dc_check_consistency", "dc_set_profiling", "dc_set_driver", "dc_set_parts", "dc_synchronize", "dc_last_error", "dc_device_count",
+    "dc_compact", "dc_compact_noalloc", "dc_check_consistency",
+    "dc_tuning_default", "dc_set_tuning", "dc_get_tuning", "dc_device_info", "dc_set_profiling", "dc_set_driver", "dc_set_parts", "dc_synchronize", "dc_last_error", "dc_device_count",
     "dc_gen_er", "dc_gen_rmat", "dc_gen_batches",
     "dc_server_create", "dc_server_destroy", "dc_server_run", "dc_server_request_bytes",
     "dc_partition_plan_create", "dc_partition_plan_info", "dc_partition_plan_arrays",
@@ -50,6 +52,15 @@ class Options(C.Structure):
                 ("heavy_slots", C.c_int64)]
 
 
+class Tuning(C.Structure):
+    _fields_ = [("loop_blocks", C.c_int32), ("light_blocks", C.c_int32), ("light_c", C.c_int32),
+                ("list_c", C.c_int32), ("list_slots", C.c_int64), ("heavy_div", C.c_int32),
+                ("group_deletions", C.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if not k.startswith("reserved")}
+
+
 class Stats(C.Structure):
     _fields_ = [("updates", C.c_int64), ("insertions", C.c_int64), ("deletions", C.c_int64),
                 ("phases", C.c_int32), ("loop_rounds", C.c_int32), ("evaluated", C.c_int64),
@@ -61,7 +72,7 @@ class Stats(C.Structure):
                 ("ms_count_pass", C.c_double), ("count_pass_bytes", C.c_int64),
                 ("count_pass_launches", C.c_int32), ("loop_settled", C.c_int32),
                 ("pull_rounds", C.c_int32), ("heavy_push_rounds", C.c_int32),
-                ("heavy_push_listed", C.c_int32), ("reserved0", C.c_int32),
+                ("heavy_push_listed", C.c_int32), ("graph_captures", C.c_int32),
                 ("lc_vertices", C.c_int64), ("lc_slots", C.c_int64), ("del_row_slots", C.c_int64)]
 
     def as_dict(self):
@@ -135,6 +146,11 @@ def lib():
         "dc_prop_get": (C.c_int, [_vp, _i32p]),
         "dc_compact": (C.c_int, [_vp]),
         "dc_compact_noalloc": (C.c_int, [_vp]),
+        "dc_tuning_default": (None, [C.POINTER(Tuning)]),
+        "dc_device_info": (C.c_int, [C.c_int32, C.c_char_p, C.c_int32, C.c_char_p, C.c_int32,
+                                     C.POINTER(C.c_int32)]),
+        "dc_set_tuning": (C.c_int, [_vp, C.POINTER(Tuning)]),
+        "dc_get_tuning": (C.c_int, [_vp, C.POINTER(Tuning)]),
         "dc_check_consistency": (C.c_int, [_vp, C.POINTER(C.c_int64)]),
         "dc_set_profiling": (C.c_int, [_vp, C.c_int32]),
         "dc_set_driver": (C.c_int, [_vp, C.c_int32, C.c_int32]),
@@ -188,6 +204,23 @@ def device_count() -> int:
     c = C.c_int32(0)
     rc = lib().dc_device_count(C.byref(c))
     return int(c.value) if rc == OK else 0
+
+
+def device_info(device=0):
+    """(name, architecture, compute units) of a HIP device."""
+    name = C.create_string_buffer(256)
+    arch = C.create_string_buffer(256)
+    cus = C.c_int32(0)
+    _check(lib().dc_device_info(device, name, 256, arch, 256, C.byref(cus)))
+    return name.value.decode(errors="replace"), arch.value.decode(errors="replace"), int(cus.value)
+
+
+def tuning(**kw) -> Tuning:
+    t = Tuning()
+    lib().dc_tuning_default(C.byref(t))
+    for k, v in kw.items():
+        setattr(t, k, v)
+    return t
 
 
 def options(**kw) -> Options:
@@ -365,6 +398,19 @@ class Graph:
         length nparts + 1; an empty array clears)."""
         off = _arr(offsets, np.int32)
         _check(lib().dc_set_parts(self._h, max(off.size - 1, 0), off))
+
+    def set_tuning(self, t: Tuning | None = None, **kw):
+        """dc_set_tuning: t, or the current tuning with the given fields changed."""
+        if t is None:
+            t = self.get_tuning()
+        for k, v in kw.items():
+            setattr(t, k, v)
+        _check(lib().dc_set_tuning(self._h, C.byref(t)))
+
+    def get_tuning(self) -> Tuning:
+        t = Tuning()
+        _check(lib().dc_get_tuning(self._h, C.byref(t)))
+        return t
 
     def set_profiling(self, on=True):
         _check(lib().dc_set_profiling(self._h, 1 if on else 0))

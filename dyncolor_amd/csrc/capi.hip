@@ -62,6 +62,7 @@ static int create_common(int32_t n, int32_t n_owned, int64_t m, const int32_t* u
   if (!g) return DC_ERR_NOMEM;
   dc_options_default(&g->opt);
   if (opt) g->opt = *opt;
+  dc_tuning_default(&g->tuning);
   if (g->opt.slack_frac < 0.f) g->opt.slack_frac = 0.f;
   if (g->opt.slack_min < 1) g->opt.slack_min = 1;
   if (g->opt.spare_slots < 0) g->opt.spare_slots = 0;
@@ -123,6 +124,38 @@ void dc_options_default(dc_options* opt) {
 
 const char* dc_last_error(void) { return dc::g_last_error.c_str(); }
 
+void dc_tuning_default(dc_tuning* t) {
+  if (!t) return;
+  std::memset(t, 0, sizeof(*t));
+  t->loop_blocks = 2048;
+  t->light_blocks = 0;
+  t->light_c = 4096;
+  t->list_c = 1 << 17;
+  t->list_slots = 1ll << 22;
+  t->heavy_div = 8;
+}
+
+int dc_set_tuning(dc_graph* h, const dc_tuning* t) {
+  return dc::guarded("dc_set_tuning", [&]() -> int {
+    if (!h || !t || t->loop_blocks < 1 || t->loop_blocks > 4096 || t->light_blocks < 0 ||
+        t->light_blocks > 4096 || t->light_c < 0 || t->list_c < -1 || t->list_slots < 0 ||
+        t->heavy_div < 1) {
+      dc::set_error("dc_set_tuning: parameter out of range");
+      return DC_ERR_INVALID;
+    }
+    G_(h)->tuning = *t;
+    return DC_OK;
+  });
+}
+
+int dc_get_tuning(const dc_graph* h, dc_tuning* t) {
+  return dc::guarded("dc_get_tuning", [&]() -> int {
+    if (!h || !t) return DC_ERR_INVALID;
+    *t = reinterpret_cast<const dc::Graph*>(h)->tuning;
+    return DC_OK;
+  });
+}
+
 int dc_device_count(int32_t* count) {
   return dc::guarded("dc_device_count", [&]() -> int {
     if (!count) return DC_ERR_INVALID;
@@ -133,6 +166,24 @@ int dc_device_count(int32_t* count) {
       return dc::hip_fail(e, "hipGetDeviceCount");
     }
     *count = c;
+    return DC_OK;
+  });
+}
+
+int dc_device_info(int32_t device, char* name, int32_t cap, char* arch, int32_t arch_cap,
+                   int32_t* compute_units) {
+  return dc::guarded("dc_device_info", [&]() -> int {
+    hipDeviceProp_t p;
+    DC_HIP(hipGetDeviceProperties(&p, device));
+    if (name && cap > 0) {
+      std::strncpy(name, p.name, (size_t)cap - 1);
+      name[cap - 1] = 0;
+    }
+    if (arch && arch_cap > 0) {
+      std::strncpy(arch, p.gcnArchName, (size_t)arch_cap - 1);
+      arch[arch_cap - 1] = 0;
+    }
+    if (compute_units) *compute_units = p.multiProcessorCount;
     return DC_OK;
   });
 }
@@ -497,7 +548,8 @@ int dc_compact_noalloc(dc_graph* h) {
 
 int dc_set_driver(dc_graph* h, int32_t driver, int32_t blocks_per_cu) {
   return dc::guarded("dc_set_driver", [&]() -> int {
-    if (!h || (driver != DC_DRIVER_PLAIN && driver != DC_DRIVER_FUSED) || blocks_per_cu < 0)
+    if (!h || (driver != DC_DRIVER_PLAIN && driver != DC_DRIVER_FUSED && driver != DC_DRIVER_GRAPH) ||
+        blocks_per_cu < 0)
       return DC_ERR_INVALID;
     Graph* g = G_(h);
     g->opt.driver = driver;

@@ -1370,9 +1370,9 @@ int csr_apply(Graph* g, const Batch& b, int rule) {
   uint32_t items = c.n_del + c.n_xdel;
   // row slots (with slack) per owned vertex: 32 or more -> wide groups
   const bool wide = g->rows_end >= 32ull * (uint64_t)std::max(g->n_owned, 1);
-  const uint32_t group_at = g->opt.group_deletions == 0  ? (1u << 16)
-                            : g->opt.group_deletions < 0 ? 0xffffffffu
-                                                         : (uint32_t)g->opt.group_deletions;
+  const int32_t gd = g->tuning.group_deletions != 0 ? g->tuning.group_deletions
+                                                   : g->opt.group_deletions;
+  const uint32_t group_at = gd == 0 ? (1u << 16) : gd < 0 ? 0xffffffffu : (uint32_t)gd;
   if (items > 0 && items >= group_at) {
     // many lines: grouped by scanned row (each row read once)
     uint64_t* keys = g->tkeys;

@@ -88,6 +88,7 @@ struct Batch {
 struct Graph {
   int device = 0;
   dc_options opt{};
+  dc_tuning tuning{};
   int32_t n = 0, n_owned = 0;
   uint64_t total_cap = 0;     // col[] slots (rows, then a spare tail for relocated rows)
   uint64_t rows_end = 0;      // end of the laid-out rows = start of the spare tail
@@ -247,6 +248,7 @@ struct LoopResult {
   int32_t pull_rounds = 0;      // rounds recounted by pull
   int32_t heavy_push_rounds = 0;  // pushed rounds whose D rows exceeded heavy_slots
   int32_t heavy_push_listed = 0;  // ... that also listed R
+  int32_t graph_captures = 0;   // round blocks captured as HIP graphs (graph driver)
   uint64_t lc_vertices = 0;     // sum over rounds of |L_r u C_r| (profiled batches)
   uint64_t lc_slots = 0;        // sum over rounds of the row slots of L_r u C_r
 };

@@ -41,7 +41,8 @@
 // a list costs what the set holds.  Appends are wave-aggregated atomics on
 // one counter per list, and on this GPU a few thousand atomics on one address
 // serialise into tens of microseconds, so rounds with large sets (more than
-// kListC vertices in L, or more than kListSlots slots in D's rows) scan.
+// dc_tuning.list_c vertices in L, or more than dc_tuning.list_slots slots in
+// D's rows) scan.
 //
 // One round = two phases:
 //   A(r): candidates = L_r | D_{r-1} | R_{r-1}.  K > 0 makes a member of C_r,
@@ -71,23 +72,24 @@
 
 #include <algorithm>
 #include <climits>
+#include <cstring>
 #include <vector>
 
 #include "dc_phases.h"
 
 namespace dc {
 
-constexpr int kMaxBlocks = 2048;   // grid of the loop kernels (per-block partials)
+constexpr int kMaxBlocks = 4096;   // largest grid of the loop kernels (per-block partials)
 constexpr int kAcc = 32;           // phase A sums: workgroup b adds into slot b % kAcc
 constexpr int kAccW = 8;           // words per slot (five used)
-constexpr int32_t kLightC = 4096;  // rounds after one with fewer members use the light grid
+// the loop's grids and thresholds come from dc_tuning (dc_tuning_default)
 constexpr int kWaveList = 64 * 16; // vertices of 128 state words (16 per lane)
 // Worklists: a round lists its sets when L has at most list_c vertices (the
-// smaller of kListC and 1/64 of the words that the scans pass: below that an
-// append's atomic costs less than the scan words it saves), and R too when
-// D's rows hold at most kListSlots slots (and the round is not a pull round).
-constexpr int32_t kListC = 1 << 17;
-constexpr unsigned long long kListSlots = 1ull << 22;
+// smaller of dc_tuning.list_c and 1/64 of the words that the scans pass:
+// below that an append's atomic costs less than the scan words it saves), and
+// R too when D's rows hold at most dc_tuning.list_slots slots (and the round
+// is not a pull round).
+constexpr int kGraphCache = 48;   // captured round blocks kept per graph (graph driver)
 constexpr uint32_t kQInvalid = 0xffffffffu;  // touched count of a round that did not list
 constexpr int kQT = 0;      // qc[kQT + (r & 3)]: entries of round r's touched list
 constexpr int kQD = 4;      // qc[kQD + 3 * (r & 1) + k]: D_r's short rows, mid rows, hub chunks
@@ -200,6 +202,14 @@ struct LoopLds {
 };
 static_assert(sizeof(LoopLds) <= 24576, "loop LDS exceeds 3/20 of a CU");
 
+// A captured block of rounds (graph driver) and the launch state it was
+// captured with.
+struct GraphBlock {
+  int32_t r0 = 0, nb = 0, grid = 0;
+  std::vector<char> state;
+  hipGraphExec_t exec = nullptr;
+};
+
 struct IncState {
   LoopState S{};
   std::vector<hipEvent_t> cev;   // profiling: start/stop around each count-pass launch
@@ -209,6 +219,8 @@ struct IncState {
   unsigned long long* hstat = nullptr;
   int32_t* tqbuf = nullptr;     // tq[0], tq[1]
   unsigned long long* terms = nullptr;   // LoopState::terms (device, 4 words)
+  std::vector<GraphBlock> graphs;        // graph driver: captured round blocks
+  int64_t captures = 0;
 };
 
 // Membership flags of one vertex in a state array (one gather tells a
@@ -1484,6 +1496,7 @@ void loop_free(Graph* g) {
   f(s->S.ctl); f(s->tqbuf); f(s->S.dq_s); f(s->S.dq_m); f(s->S.dq_h); f(s->S.qc); f(s->terms);
   if (s->hctl) (void)hipHostFree(s->hctl);
   for (hipEvent_t e : s->cev) (void)hipEventDestroy(e);
+  for (auto& e : s->graphs) (void)hipGraphExecDestroy(e.exec);
   delete[] s->hstat;
   delete s;
   g->inc = nullptr;
@@ -1568,6 +1581,7 @@ int loop_hi(Graph* g, Frontier L0, int32_t nl0, bool persistent, LoopResult* res
   int rc = inc_trace(g, s, 256);
   if (rc) return rc;
   const int32_t max_rounds = g->opt.max_rounds;
+  const dc_tuning tun = g->tuning;
 
   LoopState& S = s->S;
   S.G = g->view();
@@ -1604,14 +1618,14 @@ int loop_hi(Graph* g, Frontier L0, int32_t nl0, bool persistent, LoopResult* res
   // 22 against 1/2 and 1/32) recount by pull when C also more than doubled
   S.heavy_slots = g->opt.heavy_slots > 0
                       ? (unsigned long long)g->opt.heavy_slots
-                      : std::max<unsigned long long>(g->rows_end / 8, 1ull << 16);
+                      : std::max<unsigned long long>(g->rows_end / (uint64_t)tun.heavy_div, 1ull << 16);
   S.pull_mode = g->opt.pull_mode > 0 ? 1 : g->opt.pull_mode < 0 ? -1 : 0;
   // the reference's work terms are counted in profiled batches only (one
   // more row-bounds gather per candidate)
   S.terms = g->profiling ? s->terms : nullptr;
   S.list_c = (int32_t)std::min<uint64_t>(
-      kListC, ((uint64_t)S.swords + g->n_hrows + g->n_hch) / 64);
-  S.list_slots = kListSlots;
+      (uint64_t)tun.list_c, ((uint64_t)S.swords + g->n_hrows + g->n_hch) / 64);
+  S.list_slots = (unsigned long long)tun.list_slots;
   if (g->opt.list_limit != 0) {
     S.list_c = g->opt.list_limit < 0 ? -1 : g->opt.list_limit;
     S.list_slots = ~0ull;
@@ -1627,16 +1641,19 @@ int loop_hi(Graph* g, Frontier L0, int32_t nl0, bool persistent, LoopResult* res
   DC_HIP(hipMemsetAsync(S.acc, 0, sizeof(unsigned long long) * 2 * kAcc * kAccW, g->stream));
   if (S.terms) DC_HIP(hipMemsetAsync(S.terms, 0, sizeof(unsigned long long) * 4, g->stream));
   const dim3 blk(kBlock);
-  hipLaunchKernelGGL(k_loop_init, dim3(kMaxBlocks), blk, 0, g->stream, S);
+  hipLaunchKernelGGL(k_loop_init, dim3(tun.loop_blocks), blk, 0, g->stream, S);
   DC_HIP(hipGetLastError());
   int64_t launches = 1;
-  int grid = kMaxBlocks;
+  int grid = tun.loop_blocks;
   if (persistent && (rc = persistent_grid(g, &grid))) return rc;
+  // captured blocks replace the launches in unprofiled batches of the graph
+  // driver (per-launch events cannot be recorded inside a capture)
+  const bool use_graphs = !persistent && g->opt.driver == DC_DRIVER_GRAPH && !g->profiling;
   // light rounds (small C) launch fewer workgroups: their work is a few
   // thousand vertices, and dispatching 2048 workgroups twice a round costs
   // more than it brings
-  int light_grid = kMaxBlocks;
-  {
+  int light_grid = tun.light_blocks;
+  if (light_grid <= 0) {
     int dev = 0, cus = 0;
     DC_HIP(hipGetDevice(&dev));
     DC_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
@@ -1652,6 +1669,7 @@ int loop_hi(Graph* g, Frontier L0, int32_t nl0, bool persistent, LoopResult* res
   int settle_left = g->opt.exact_rounds ? 0 : 3;
   bool settled = false;
   int32_t rounds = -1, r = 0;
+  const int64_t captures0 = s->captures;
   while (r < max_rounds && rounds < 0) {
     const int32_t nb =
         std::min(persistent ? block_end_persistent(r) : block_end_plain(r), max_rounds) - r;
@@ -1664,24 +1682,20 @@ int loop_hi(Graph* g, Frontier L0, int32_t nl0, bool persistent, LoopResult* res
       }
       DC_HIP(hipEventRecord(e0, g->stream));
     }
-    if (persistent) {
-      int32_t r0 = r, r1 = r + nb;
-      void* args[] = {&S, &r0, &r1};
-      DC_HIP(hipLaunchCooperativeKernel((const void*)k_loop_persistent, dim3(grid), blk, args, 0,
-                                        g->stream));
-      ++launches;
-    } else {
+    // rounds r .. r + nb - 1 of the plain driver: two launches per round, and
+    // after the check rounds the queued settle (and part) checks
+    auto enqueue_block = [&](bool events) -> int {
       for (int32_t i = 0; i < nb; ++i) {
         hipLaunchKernelGGL(k_round_a, dim3(round_grid), blk, 0, g->stream, S, r + i);
-        if (g->profiling) DC_HIP(hipEventRecord(s->cev[2 * i], g->stream));
+        if (events) DC_HIP(hipEventRecord(s->cev[2 * i], g->stream));
         hipLaunchKernelGGL(k_round_c, dim3(round_grid), blk, 0, g->stream, S, r + i);
-        if (g->profiling) DC_HIP(hipEventRecord(s->cev[2 * i + 1], g->stream));
+        if (events) DC_HIP(hipEventRecord(s->cev[2 * i + 1], g->stream));
         if (settle_point(r + i + 1) && !g->opt.exact_rounds) {
-          hipLaunchKernelGGL(k_settle_check_if, dim3(kMaxBlocks), blk, 0, g->stream, S);
+          hipLaunchKernelGGL(k_settle_check_if, dim3(tun.loop_blocks), blk, 0, g->stream, S);
           hipLaunchKernelGGL(k_settle_decide, dim3(1), dim3(64), 0, g->stream, S, r + i);
           launches += 2;
           if (S.nparts) {
-            hipLaunchKernelGGL(k_part_check_if, dim3(kMaxBlocks), blk, 0, g->stream, S, r + i);
+            hipLaunchKernelGGL(k_part_check_if, dim3(tun.loop_blocks), blk, 0, g->stream, S, r + i);
             hipLaunchKernelGGL(k_part_decide, dim3(1), dim3(1024), 0, g->stream, S, r + i);
             hipLaunchKernelGGL(k_part_clear,
                                dim3((uint32_t)std::min<uint64_t>(2048, (S.swords + kBlock - 1) / kBlock)),
@@ -1692,6 +1706,54 @@ int loop_hi(Graph* g, Frontier L0, int32_t nl0, bool persistent, LoopResult* res
       }
       DC_HIP(hipGetLastError());
       launches += 2 * nb;
+      return DC_OK;
+    };
+    if (persistent) {
+      int32_t r0 = r, r1 = r + nb;
+      void* args[] = {&S, &r0, &r1};
+      DC_HIP(hipLaunchCooperativeKernel((const void*)k_loop_persistent, dim3(grid), blk, args, 0,
+                                        g->stream));
+      ++launches;
+    } else if (use_graphs) {
+      // the block as a captured HIP graph, replayed while the loop state, the
+      // round range and the grid are the same (kernel arguments are baked in)
+      GraphBlock* gb = nullptr;
+      for (auto& e : s->graphs)
+        if (e.r0 == r && e.nb == nb && e.grid == round_grid &&
+            std::memcmp(e.state.data(), &S, sizeof(S)) == 0) {
+          gb = &e;
+          break;
+        }
+      if (!gb) {
+        if (s->graphs.size() >= kGraphCache) {
+          for (auto& e : s->graphs) (void)hipGraphExecDestroy(e.exec);
+          s->graphs.clear();
+        }
+        hipGraph_t graph = nullptr;
+        DC_HIP(hipStreamBeginCapture(g->stream, hipStreamCaptureModeRelaxed));
+        const int64_t l0 = launches;
+        const int erc = enqueue_block(false);
+        const hipError_t ec = hipStreamEndCapture(g->stream, &graph);
+        if (erc) return erc;
+        if (ec != hipSuccess) return hip_fail(ec, "hipStreamEndCapture");
+        launches = l0;
+        GraphBlock nbk;
+        nbk.r0 = r;
+        nbk.nb = nb;
+        nbk.grid = round_grid;
+        nbk.state.resize(sizeof(S));
+        std::memcpy(nbk.state.data(), &S, sizeof(S));
+        const hipError_t ei = hipGraphInstantiate(&nbk.exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (ei != hipSuccess) return hip_fail(ei, "hipGraphInstantiate");
+        s->graphs.push_back(std::move(nbk));
+        gb = &s->graphs.back();
+        ++s->captures;
+      }
+      DC_HIP(hipGraphLaunch(gb->exec, g->stream));
+      ++launches;
+    } else {
+      if ((rc = enqueue_block(g->profiling))) return rc;
     }
     if (g->profiling) DC_HIP(hipEventRecord(e1, g->stream));
     DC_HIP(hipMemcpyAsync(s->hctl, S.ctl, sizeof(LoopCtl), hipMemcpyDeviceToHost, g->stream));
@@ -1723,7 +1785,7 @@ int loop_hi(Graph* g, Frontier L0, int32_t nl0, bool persistent, LoopResult* res
       --settle_left;
       int32_t* flag = reinterpret_cast<int32_t*>(g->aux);
       DC_HIP(hipMemsetAsync(flag, 0, sizeof(int32_t), g->stream));
-      hipLaunchKernelGGL(k_settle_check, dim3(kMaxBlocks), blk, 0, g->stream, S, flag);
+      hipLaunchKernelGGL(k_settle_check, dim3(tun.loop_blocks), blk, 0, g->stream, S, flag);
       DC_HIP(hipGetLastError());
       ++launches;
       int32_t h = 1;
@@ -1735,7 +1797,7 @@ int loop_hi(Graph* g, Frontier L0, int32_t nl0, bool persistent, LoopResult* res
         if ((rc = loop_reset(g, s))) return rc;
       }
     }
-    round_grid = s->hctl->last_nc < kLightC ? light_grid : grid;
+    round_grid = s->hctl->last_nc < tun.light_c ? light_grid : grid;
   }
   if (rounds < 0) {
     int rc2 = loop_reset(g, s);
@@ -1755,6 +1817,7 @@ int loop_hi(Graph* g, Frontier L0, int32_t nl0, bool persistent, LoopResult* res
   res->pull_rounds = s->hctl->pull_rounds;
   res->heavy_push_rounds = s->hctl->heavy_push_rounds;
   res->heavy_push_listed = s->hctl->heavy_push_listed;
+  res->graph_captures = (int32_t)(s->captures - captures0);
   res->lc_vertices = terms[0];
   res->lc_slots = terms[1];
   res->rounds = rounds;

@@ -492,6 +492,7 @@ int recolor_plain(Graph* g, int rule, dc_stats* st, bool persistent_loop) {
       st->pull_rounds = lr.pull_rounds;
       st->heavy_push_rounds = lr.heavy_push_rounds;
       st->heavy_push_listed = lr.heavy_push_listed;
+      st->graph_captures = lr.graph_captures;
       st->lc_vertices = (int64_t)lr.lc_vertices;
       st->lc_slots = (int64_t)lr.lc_slots;
       st->del_row_slots = g->profiling ? del_row_slots : 0;

@@ -46,6 +46,11 @@ extern "C" {
  * cooperative launch runs every round of a batch with device-side barriers. */
 #define DC_DRIVER_PLAIN 0
 #define DC_DRIVER_FUSED 1
+/* GRAPH: the plain driver's launches, each host block of loop rounds captured
+ * once as a HIP graph and replayed (per graph handle, while the round range,
+ * the grid and the loop's device state are the same); profiled batches use the
+ * plain launches. */
+#define DC_DRIVER_GRAPH 2
 
 typedef struct dc_graph dc_graph;
 typedef struct dc_batch dc_batch;
@@ -113,7 +118,8 @@ typedef struct {
   int32_t heavy_push_rounds;  /* loop rounds that pushed count changes from more
                                than heavy_slots changed row slots             */
   int32_t heavy_push_listed;  /* ... and took their next candidates from lists */
-  int32_t reserved0;
+  int32_t graph_captures;   /* round blocks captured as HIP graphs in this batch
+                               (graph driver; the others were replays)         */
   /* Work terms of the reference's algorithm for this batch (profiling on;
    * zero otherwise): the reference rescans every row of L_r and C_r each loop
    * round (HI/gpuFunctions_undir.cuh:229-284) and both endpoint rows per
@@ -122,6 +128,30 @@ typedef struct {
   int64_t lc_slots;         /* sum over loop rounds of the row slots of L_r u C_r */
   int64_t del_row_slots;    /* sum over deletion lines of both rows' slots    */
 } dc_stats;
+
+/* Work-split parameters of the independent-rule loop and the batch apply
+ * (results never depend on them; the tuner in dyncolor_amd/tune.py picks them
+ * per device and workload by timing).  No counterpart in the reference, whose
+ * launch shapes are fixed (numberOfBlocks = 32 x SMs, HI/Color_main.cu:54). */
+typedef struct {
+  int32_t loop_blocks;   /* workgroups of a loop round (1..4096, default 2048)           */
+  int32_t light_blocks;  /* workgroups of a light round (0 = two per CU)                  */
+  int32_t light_c;       /* a round after one with fewer members is light (default 4096) */
+  int32_t list_c;        /* rounds with at most this many vertices in L take their sets
+                            from worklists (default 131072; also capped at 1/64 of the
+                            words a scan passes)                                          */
+  int64_t list_slots;    /* ... and list R when D's rows hold at most this many slots
+                            (default 2^22)                                                */
+  int32_t heavy_div;     /* pull threshold = laid-out row slots / heavy_div, at least 65536
+                            (default 8; dc_options.heavy_slots overrides)                 */
+  int32_t group_deletions; /* deletion lines from which the batch apply groups them by
+                            scanned row (0: dc_options.group_deletions decides)          */
+} dc_tuning;
+
+void dc_tuning_default(dc_tuning* t);
+/* Set / read a graph's tuning (validated: DC_ERR_INVALID when out of range). */
+int dc_set_tuning(dc_graph* g, const dc_tuning* t);
+int dc_get_tuning(const dc_graph* g, dc_tuning* t);
 
 /* ---- graph lifetime ------------------------------------------------------ */
 void dc_options_default(dc_options* opt);
@@ -308,6 +338,10 @@ int dc_set_parts(dc_graph* g, int32_t nparts, const int32_t* offsets);
 int dc_synchronize(dc_graph* g);
 const char* dc_last_error(void);
 int dc_device_count(int32_t* count);
+/* Name (NUL-terminated, at most cap bytes), architecture and compute units of
+ * a HIP device, for keying tuned parameters per device. */
+int dc_device_info(int32_t device, char* name, int32_t cap, char* arch, int32_t arch_cap,
+                   int32_t* compute_units);
 
 /* ---- serving: continuous batching over resident graphs --------------------
  * No counterpart in the reference (it recolors one batch per process run,

@@ -118,7 +118,8 @@ def test_header_documents_reference_citations():
 def test_struct_layouts_match_header(tmp_path):
     # the ctypes mirrors of dc_options, dc_stats and dc_serve_stats against
     # the C header: size and every field offset
-    structs = {"dc_options": d.Options, "dc_stats": d.Stats, "dc_serve_stats": d.ServeStats}
+    structs = {"dc_options": d.Options, "dc_stats": d.Stats, "dc_serve_stats": d.ServeStats,
+               "dc_tuning": d.Tuning, "dc_partition_info": d.PartitionInfo}
     lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "dyncolor.h"', "int main(void) {"]
     for cname, py in structs.items():
         lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')
