@@ -55,7 +55,8 @@ class Options(C.Structure):
 class Tuning(C.Structure):
     _fields_ = [("loop_blocks", C.c_int32), ("light_blocks", C.c_int32), ("light_c", C.c_int32),
                 ("list_c", C.c_int32), ("list_slots", C.c_int64), ("heavy_div", C.c_int32),
-                ("group_deletions", C.c_int32)]
+                ("group_deletions", C.c_int32), ("ldf_loop", C.c_int32),
+                ("reserved0", C.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_ if not k.startswith("reserved")}

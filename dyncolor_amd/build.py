@@ -25,7 +25,7 @@ HIP_FLAGS = [
 ]
 HOST_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-Wall"]
 
-SOURCES = ["csr.hip", "recolor.hip", "loop.hip", "fused.hip", "capi.hip", "serve.cpp", "gen.cpp",
+SOURCES = ["csr.hip", "recolor.hip", "loop.hip", "loop_ldf.hip", "fused.hip", "capi.hip", "serve.cpp", "gen.cpp",
            "partition.cpp"]
 
 

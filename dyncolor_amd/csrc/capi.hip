@@ -139,7 +139,7 @@ int dc_set_tuning(dc_graph* h, const dc_tuning* t) {
   return dc::guarded("dc_set_tuning", [&]() -> int {
     if (!h || !t || t->loop_blocks < 1 || t->loop_blocks > 4096 || t->light_blocks < 0 ||
         t->light_blocks > 4096 || t->light_c < 0 || t->list_c < -1 || t->list_slots < 0 ||
-        t->heavy_div < 1) {
+        t->heavy_div < 1 || t->ldf_loop < 0 || t->ldf_loop > 1) {
       dc::set_error("dc_set_tuning: parameter out of range");
       return DC_ERR_INVALID;
     }

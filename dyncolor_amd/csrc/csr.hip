@@ -406,6 +406,7 @@ void free_state(Graph* g) {
   };
   fused_free(g);
   loop_free(g);
+  ldf_free(g);
   for (hipEvent_t e : g->evpool) (void)hipEventDestroy(e);
   g->evpool.clear();
   f(g->row); f(g->cap); f(g->col); f(g->prop); f(g->ccol); f(g->twin);

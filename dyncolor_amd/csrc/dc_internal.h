@@ -199,6 +199,8 @@ struct Graph {
 
   // incremental independent-rule loop state (loop.hip)
   void* inc = nullptr;
+  // heuristic-based loop state (loop_ldf.hip)
+  void* ldf = nullptr;
 
   GraphView view() const {
     return GraphView{n, n_owned, row, cap, col, prop, ccol, twin, mexc, hidx, hist,
@@ -254,6 +256,10 @@ struct LoopResult {
 };
 int loop_hi(Graph* g, Frontier L0, int32_t nl0, bool persistent, LoopResult* res);
 void loop_free(Graph* g);
+// loop_ldf.hip: the heuristic-based loop from the marked set L0 (both colour
+// buffers written; no catch-up follows)
+int loop_ldf(Graph* g, Frontier L0, LoopResult* res);
+void ldf_free(Graph* g);
 int loop_check(Graph* g, int64_t* bad);   // candidate counts / state arrays clean at rest
 // fused.hip
 int recolor_fused(Graph* g, int rule, dc_stats* st);

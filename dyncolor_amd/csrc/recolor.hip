@@ -465,11 +465,16 @@ int recolor_plain(Graph* g, int rule, dc_stats* st, bool persistent_loop) {
   // the tombstone pass's row-slot sum stays in the counters through the
   // pre-loop rounds (no round resets it)
   const int64_t del_row_slots = npre > 0 ? (int64_t)pre_h[npre - 1].c.delslots : 0;
-  if (rule == DC_RULE_INDEPENDENT && change) {
-    // candidate counts kept incrementally (loop.hip); both colour buffers
-    // are written there, so no catch-up follows
+  const bool ldf_counts = rule == DC_RULE_LDF && g->tuning.ldf_loop == 0;
+  if ((rule == DC_RULE_INDEPENDENT || ldf_counts) && change) {
+    // candidate counts kept incrementally (loop.hip, loop_ldf.hip); both
+    // colour buffers are written there, so no catch-up follows
     LoopResult lr;
-    if ((rc = loop_hi(g, Mk, (int32_t)nL, persistent_loop, &lr))) return rc;
+    if (rule == DC_RULE_INDEPENDENT) {
+      if ((rc = loop_hi(g, Mk, (int32_t)nL, persistent_loop, &lr))) return rc;
+    } else if ((rc = loop_ldf(g, Mk, &lr))) {
+      return rc;
+    }
     evaluated += lr.evaluated;
     changed_total += lr.changed;
     frontier_sum += lr.frontier_sum;

@@ -146,6 +146,11 @@ typedef struct {
                             (default 8; dc_options.heavy_slots overrides)                 */
   int32_t group_deletions; /* deletion lines from which the batch apply groups them by
                             scanned row (0: dc_options.group_deletions decides)          */
+  int32_t ldf_loop;      /* heuristic-based (ldf) loop: 0 = its sets kept as counts that
+                            move with the rows entering or leaving L and the changed
+                            colours (default), 1 = every round re-derives C from the rows
+                            of L (the reference's form); same colours and traces          */
+  int32_t reserved0;
 } dc_tuning;
 
 void dc_tuning_default(dc_tuning* t);

@@ -20,8 +20,9 @@ def say(*a):
 
 
 def run_both(oracle_mod, n, u, v, batches, rule, colors=None, opts=None, n_owned=None,
-             ghost_colors=None, device_batches=False, driver=d.DRIVER_PLAIN):
-    """Replay batches on GPU and oracle; assert identical colors + traces."""
+             ghost_colors=None, device_batches=False, driver=d.DRIVER_PLAIN, tuning=None):
+    """Replay batches on GPU and oracle; assert identical colors + traces.
+    tuning: dc_tuning fields to set on the graph before the first batch."""
     u = np.asarray(u, np.int32)
     v = np.asarray(v, np.int32)
     if n_owned is None:
@@ -31,6 +32,11 @@ def run_both(oracle_mod, n, u, v, batches, rule, colors=None, opts=None, n_owned
         g = d.Graph(n_owned, u, v, opts, n_ghost=n - n_owned, ghost_colors=ghost_colors)
         og = oracle_mod.OracleGraph(n, u, v, n_owned=n_owned)
     g.set_driver(driver)
+    if tuning:
+        t = g.get_tuning()
+        for k, x in tuning.items():
+            setattr(t, k, x)
+        g.set_tuning(t)
     if colors is None:
         if n_owned is not None:
             init = np.zeros(n, np.int32)
