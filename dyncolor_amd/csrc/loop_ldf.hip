@@ -37,6 +37,7 @@ namespace dc {
 
 constexpr int kLdfBlocks = 2048;
 constexpr int kLdfAcc = 32;        // round sums: workgroup b adds into slot b % kLdfAcc
+constexpr int kWin32 = kWinBits / 32;   // 32-bit words of a wave's mex window
 constexpr uint8_t kFlD = 1, kFlX = 2, kFlPm = 4;   // per-vertex flags of a round
 
 struct LdfCtl {
@@ -44,6 +45,10 @@ struct LdfCtl {
   int32_t pad[3];
 };
 
+// Rows by length: short (cap < kSmallLen) by their own thread; longer rows
+// (the histogram-row table) by a wave for their mex, and for their row
+// passes by a wave (cap < kMidLen) or a workgroup per kChunk-slot chunk of
+// the hub chunk table (cap >= kMidLen).
 struct LdfState {
   GraphView G;
   int32_t* c0;          // colour buffers (both written at commit: they agree)
@@ -52,75 +57,105 @@ struct LdfState {
   int32_t* cc;
   int32_t* pm;
   int32_t* newc;
+  int32_t* corig;       // colour at the loop's start of every vertex that changed (-1: none)
+  uint8_t* pdir;        // slot -> its neighbour precedes the row's vertex (set up per batch)
   uint8_t* F[2];        // F[r & 1] = L_r, F[(r + 1) & 1] = C_r = L_{r+1}
   uint8_t* fl;          // kFlD / kFlX / kFlPm of the current round
-  const int32_t* hrowv; // rows with cap >= kSmallLen (the long-row table)
+  const int32_t* hrowv; // rows with cap >= kSmallLen
   uint32_t n_hrows;
+  const int2* hch;      // {hub, chunk} for every kChunk slots of every hub's capacity
+  uint32_t n_hch;
   unsigned long long* acc;   // [2][kLdfAcc][4]: |L|, |C|, changed
   int4* trace;
   LdfCtl* ctl;
 };
 
 // ---------------------------------------------------------------------------
+// Predecessor colour bits below 64 are kept as two 32-bit words.
+__device__ __forceinline__ void add_colour_bit(uint32_t c, uint32_t& lo, uint32_t& hi) {
+  lo |= c < 32u ? 1u << c : 0u;
+  hi |= c - 32u < 32u ? 1u << (c - 32u) : 0u;
+}
+// mex of colour bits 0..63, -1 when all are present
+__device__ __forceinline__ int32_t mex64(uint32_t lo, uint32_t hi) {
+  return lo != ~0u ? __ffs(~lo) - 1 : hi != ~0u ? 32 + __ffs(~hi) - 1 : -1;
+}
+
 // A short row (used < kSmallLen) of v by its own thread: mex of the
 // predecessors' colours and the predecessor slots with colour cv.  At most 63
-// predecessors, so the mex is below 64.
-__device__ __forceinline__ void short_pred_scan(const GraphView& G, int32_t v, int32_t cv,
-                                                int32_t& pm, int32_t& cc) {
+// predecessors, so the mex is below 64.  INIT classifies the slots by the
+// priorities and records the predecessor slots in pdir (fixed for the batch:
+// no priority moves inside the loop); the loop's scans read pdir.
+template <bool INIT>
+__device__ __forceinline__ void short_pred_scan(const GraphView& G, uint8_t* pdir, int32_t v,
+                                                int32_t cv, int32_t& pm, int32_t& cc) {
   const uint2 r = G.row[v];
   const int32_t pv = G.prop[v];
-  uint32_t lo = 0u, hi = 0u;   // colours 0..31, 32..63
+  uint32_t lo = 0u, hi = 0u;
   int32_t k = 0;
   for (uint32_t j0 = 0; j0 < r.y; j0 += 4) {
-    int32_t w[4], c[4], p[4];
+    int32_t c[4];
+    bool pr[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const bool in = j0 + q < r.y;
-      w[q] = in ? G.col[(uint64_t)r.x + j0 + q] : -1;
-      c[q] = in ? G.ccol[(uint64_t)r.x + j0 + q] : -1;
+      const uint64_t j = (uint64_t)r.x + j0 + q;
+      c[q] = in ? G.ccol[j] : -1;
+      if (INIT) {
+        const int32_t w = in ? G.col[j] : -1;
+        pr[q] = w >= 0 && precedes(nullptr, w, G.prop[w], v, pv);
+        if (in) pdir[j] = pr[q] ? 1 : 0;
+      } else {
+        pr[q] = in && pdir[j] != 0;
+      }
     }
 #pragma unroll
-    for (int q = 0; q < 4; ++q) p[q] = w[q] >= 0 ? G.prop[w[q]] : 0;
-#pragma unroll
     for (int q = 0; q < 4; ++q) {
-      if (w[q] < 0 || !precedes(nullptr, w[q], p[q], v, pv)) continue;
-      const uint32_t cq = (uint32_t)c[q];
-      lo |= cq < 32u ? 1u << cq : 0u;
-      hi |= cq - 32u < 32u ? 1u << (cq - 32u) : 0u;
+      if (!pr[q]) continue;
+      add_colour_bit((uint32_t)c[q], lo, hi);
       k += c[q] == cv ? 1 : 0;
     }
   }
-  pm = lo != ~0u ? __ffs(~lo) - 1 : 32 + __ffs(~hi) - 1;
+  pm = mex64(lo, hi);
   cc = k;
 }
 
-// The same for a long row by a whole wave: the mex over LDS windows of
-// kWinBits colours (more than kWinBits predecessors only if the window is
-// full), cc by a wave reduction.  Every lane returns both.
-__device__ __forceinline__ void wave_pred_scan(const GraphView& G, int32_t v, int32_t cv,
-                                               unsigned long long* ww, int32_t& pm, int32_t& cc) {
+// A whole row of v by one wave: the mex over LDS windows of kWinBits colours
+// (a window past the first only when it is full) and the predecessor slots
+// with colour cv.  Every lane returns both.  INIT as for short_pred_scan.
+template <bool INIT>
+__device__ __forceinline__ void wave_pred_scan(const GraphView& G, uint8_t* pdir, int32_t v,
+                                               int32_t cv, uint32_t* ww, int32_t& pm,
+                                               int32_t& cc) {
   const int lane = lane_id();
   const uint2 r = G.row[v];
   const int32_t pv = G.prop[v];
   int32_t k = 0;
   for (int32_t base = 0;; base += kWinBits) {
-    for (int i = lane; i < kWinWords; i += 64) ww[i] = 0ull;
+    for (int i = lane; i < kWin32; i += 64) ww[i] = 0u;
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
     for (uint32_t j = lane; j < r.y; j += 64) {
-      const int32_t w = G.col[(uint64_t)r.x + j];
-      if (w < 0) continue;
-      if (!precedes(nullptr, w, G.prop[w], v, pv)) continue;
-      const int32_t c = G.ccol[(uint64_t)r.x + j];
+      const uint64_t s = (uint64_t)r.x + j;
+      bool pr;
+      if (INIT && base == 0) {
+        const int32_t w = G.col[s];
+        pr = w >= 0 && precedes(nullptr, w, G.prop[w], v, pv);
+        pdir[s] = pr ? 1 : 0;
+      } else {
+        pr = pdir[s] != 0;
+      }
+      if (!pr) continue;
+      const int32_t c = G.ccol[s];
       if (base == 0 && c == cv) ++k;
-      const int32_t cw = c - base;
-      if (cw >= 0 && cw < kWinBits) atomicOr(&ww[cw >> 6], 1ull << (cw & 63));
+      const uint32_t cw = (uint32_t)(c - base);
+      if (cw < (uint32_t)kWinBits) atomicOr(&ww[cw >> 5], 1u << (cw & 31u));
     }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
     int32_t found = INT_MAX;
-    for (int i = lane; i < kWinWords; i += 64)
-      if (ww[i] != ~0ull && found == INT_MAX) found = i * 64 + __ffsll((long long)~ww[i]) - 1;
+    for (int i = lane; i < kWin32; i += 64)
+      if (ww[i] != ~0u && found == INT_MAX) found = i * 32 + __ffs(~ww[i]) - 1;
     for (int off = 32; off > 0; off >>= 1) found = min(found, __shfl_xor(found, off));
     if (base == 0)
       for (int off = 32; off > 0; off >>= 1) k += __shfl_xor(k, off);
@@ -134,6 +169,56 @@ __device__ __forceinline__ void wave_pred_scan(const GraphView& G, int32_t v, in
   }
 }
 
+// Rows of the histogram-row table (cap >= kSmallLen; with MID only those
+// below kMidLen) for which pick(v) holds, each by a whole wave: the lanes
+// test 64 table entries at once, then the wave takes the picked rows one
+// after another.
+template <bool MID, class Pick, class Body>
+__device__ __forceinline__ void each_long_row(const LdfState& S, Pick pick, Body body) {
+  const int lane = lane_id();
+  const uint32_t gwave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
+  // entries strided over the waves: the table lists the largest rows
+  // together, which one wave would otherwise take in a row
+  for (uint32_t base = 0; base < S.n_hrows; base += nwaves * 64u) {
+    const uint32_t i = base + (uint32_t)lane * nwaves + gwave;
+    int32_t v = -1;
+    if (i < S.n_hrows) {
+      v = S.hrowv[i];
+      if ((MID && S.G.cap[v] >= kMidLen) || !pick(v)) v = -1;
+    }
+    unsigned long long act = __ballot(v >= 0);
+    while (act) {
+      const int src = __ffsll((long long)act) - 1;
+      act &= act - 1ull;
+      body(__shfl(v, src));
+    }
+  }
+}
+
+// Hub chunks whose hub satisfies pick(v), each by the whole workgroup: the
+// threads test kBlock table entries at once, then the workgroup takes the
+// picked chunks one after another.
+template <class Pick, class Body>
+__device__ __forceinline__ void each_hub_chunk(const LdfState& S, Pick pick, Body body) {
+  __shared__ int2 list[kBlock];
+  __shared__ uint32_t nlist;
+  // entries strided over the workgroups: a hub's chunks are listed together
+  for (uint32_t base = 0; base < S.n_hch; base += gridDim.x * (uint32_t)kBlock) {
+    __syncthreads();
+    if (threadIdx.x == 0) nlist = 0u;
+    __syncthreads();
+    const uint32_t i = base + threadIdx.x * gridDim.x + blockIdx.x;
+    if (i < S.n_hch) {
+      const int2 e = S.hch[i];
+      if (pick(e.x)) list[atomicAdd(&nlist, 1u)] = e;
+    }
+    __syncthreads();
+    const uint32_t m = nlist;
+    for (uint32_t q = 0; q < m; ++q) body(list[q]);
+  }
+}
+
 // Adds the workgroup's sums of (a, b, c) into round slot set `slot`.
 __device__ __forceinline__ void ldf_sums(const LdfState& S, int slot, uint32_t a, uint32_t b,
                                          uint32_t c) {
@@ -144,6 +229,7 @@ __device__ __forceinline__ void ldf_sums(const LdfState& S, int slot, uint32_t a
     c += __shfl_xor(c, off);
   }
   const int wib = threadIdx.x >> 6;
+  __syncthreads();
   if (lane_id() == 0) {
     red[wib][0] = a;
     red[wib][1] = b;
@@ -161,30 +247,27 @@ __device__ __forceinline__ void ldf_sums(const LdfState& S, int slot, uint32_t a
 }
 
 // ---------------------------------------------------------------------------
-// batch set-up: every owned vertex's cc and (short rows) cached mex
+// batch set-up: every owned vertex's cc and cached mex
 __global__ __launch_bounds__(kBlock) void k_ldf_init_cc(LdfState S) {
-  __shared__ unsigned long long win[kBlock / 64][kWinWords];
+  __shared__ uint32_t win[kBlock / 64][kWin32];
   const GraphView& G = S.G;
   for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < G.n_owned;
        v += (int64_t)gridDim.x * blockDim.x) {
-    if (G.hidx[v] >= 0) continue;   // long rows below
+    if (G.hidx[v] >= 0) continue;
     int32_t pm, cc;
-    short_pred_scan(G, (int32_t)v, S.c0[v], pm, cc);
+    short_pred_scan<true>(G, S.pdir, (int32_t)v, S.c0[v], pm, cc);
     S.pm[v] = pm;
     S.cc[v] = cc;
   }
-  const int lane = lane_id();
-  const uint32_t gwave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
-  for (uint32_t i = gwave; i < S.n_hrows; i += nwaves) {
-    const int32_t v = S.hrowv[i];
+  uint32_t* ww = win[threadIdx.x >> 6];
+  each_long_row<false>(S, [](int32_t) { return true; }, [&](int32_t v) {
     int32_t pm, cc;
-    wave_pred_scan(G, v, S.c0[v], win[threadIdx.x >> 6], pm, cc);
-    if (lane == 0) {
+    wave_pred_scan<true>(G, S.pdir, v, S.c0[v], ww, pm, cc);
+    if (lane_id() == 0) {
       S.pm[v] = pm;
       S.cc[v] = cc;
     }
-  }
+  });
 }
 
 // L_0 = the marked set (round -1's C): its flags, and its rows as D of round
@@ -229,7 +312,7 @@ __global__ __launch_bounds__(kBlock) void k_ldf_members(LdfState S, int32_t r) {
       if (m < 0) {
         if (G.hidx[v] < 0) {
           int32_t cc;
-          short_pred_scan(G, (int32_t)v, S.c0[v], m, cc);
+          short_pred_scan<false>(G, S.pdir, (int32_t)v, S.c0[v], m, cc);
           S.pm[v] = m;
         } else {
           f |= kFlPm;   // long row: the next launch
@@ -247,56 +330,78 @@ __global__ __launch_bounds__(kBlock) void k_ldf_members(LdfState S, int32_t r) {
   ldf_sums(S, r & 1, nl, nc, nx);
 }
 
-// launch 2: stale mex of long rows, a wave per row of the long-row table
+// launch 2: stale mex of mid rows (a wave per row) and hub rows (a
+// workgroup per chunk), and their colour changes
 __global__ __launch_bounds__(kBlock) void k_ldf_long_mex(LdfState S, int32_t r) {
   if (S.ctl->done_round < r) return;
-  __shared__ unsigned long long win[kBlock / 64][kWinWords];
+  __shared__ uint32_t win[kBlock / 64][kWin32];
   const GraphView& G = S.G;
-  const int lane = lane_id();
-  const uint32_t gwave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
+  uint32_t* ww = win[threadIdx.x >> 6];
   uint32_t nx = 0;
-  for (uint32_t i = gwave; i < S.n_hrows; i += nwaves) {
-    const int32_t v = S.hrowv[i];
-    const uint8_t f = S.fl[v];
-    if (!(f & kFlPm)) continue;
+  auto stale = [&](int32_t v) { return (S.fl[v] & kFlPm) != 0; };
+  each_long_row<false>(S, stale, [&](int32_t v) {
     int32_t m, cc;
-    wave_pred_scan(G, v, S.c0[v], win[threadIdx.x >> 6], m, cc);
-    if (lane == 0) {
+    wave_pred_scan<false>(G, S.pdir, v, S.c0[v], ww, m, cc);
+    if (lane_id() == 0) {
       S.pm[v] = m;
       if (m != S.c0[v]) {
         S.newc[v] = m;
         S.cc[v] = 0;
-        S.fl[v] = f | kFlX;
+        S.fl[v] = S.fl[v] | kFlX;
         ++nx;
       }
     }
-  }
+  });
   ldf_sums(S, r & 1, 0u, 0u, nx);
 }
 
-// One lane's share of the row pass of x (slots lo, lo + stride, ...): to
-// each successor y, D adds x's +-1 to K[y]; X moves cc[y] and drops y's
-// cached mex.
+// x's colour c (was old) as owned neighbour w keeps it: with SLOT the slot
+// of x in w's row (position t), and w's histogram and cached mex over all its
+// neighbours, as propagate_slots keeps them.
+template <bool SLOT>
+__device__ __forceinline__ void nbr_colour(const GraphView& G, int32_t w, int32_t t, int32_t c,
+                                           int32_t old) {
+  if (w >= G.n_owned) return;
+  const int32_t h = G.hidx[w];
+  const int32_t m = G.mexc[w];
+  if (SLOT) G.ccol[(uint64_t)G.row[w].x + (uint32_t)t] = c;
+  uint32_t before = 2u;
+  if (h >= 0) {
+    uint32_t* hr = G.hist + (uint64_t)h * kHistW;
+    if (old >= 0 && old < kHistW) before = atomicSub(&hr[old], 1u);
+    if (c < kHistW) atomicAdd(&hr[c], 1u);
+  }
+  const bool stale = h >= 0 ? (old < m && (old >= kHistW || before == 1u)) || c == m
+                            : old < m || c == m;
+  if (m >= 0 && stale) G.mexc[w] = -1;
+}
+
+// One lane's share of the row pass of x (slots lo, lo + stride, ...), over
+// x's successors y only: D adds x's +-1 to K[y]; X (colour a -> b) moves
+// cc[y], drops y's predecessor mex and writes b into x's slot in y's row.
+// The loop's scans read predecessors' slots only, and no histogram or
+// all-neighbour mex: x's slots in its predecessors' rows, and its
+// neighbours' histograms and all-neighbour mex caches, take x's final colour
+// once, after the loop (k_ldf_settle).
 __device__ __forceinline__ void ldf_row_part(const LdfState& S, int32_t x, uint8_t f, int32_t ksign,
                                              int32_t a, int32_t b, uint64_t rbase, uint32_t lo,
                                              uint32_t hi, uint32_t stride) {
   const GraphView& G = S.G;
-  const int32_t px = G.prop[x];
+  const bool xc = (f & kFlX) != 0;
   for (uint32_t j0 = lo; j0 < hi; j0 += 4 * stride) {
-    int32_t y[4], p[4];
+    int32_t y[4], t[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const uint32_t j = j0 + q * stride;
-      y[q] = j < hi ? G.col[rbase + j] : -1;
+      // successors only: live slots whose neighbour does not precede x
+      y[q] = j < hi && S.pdir[rbase + j] == 0 ? G.col[rbase + j] : -1;
+      t[q] = xc && y[q] >= 0 ? G.twin[rbase + j] : 0;
     }
 #pragma unroll
-    for (int q = 0; q < 4; ++q) p[q] = y[q] >= 0 ? G.prop[y[q]] : 0;
-#pragma unroll
     for (int q = 0; q < 4; ++q) {
-      if (y[q] < 0 || !precedes(nullptr, x, px, y[q], p[q])) continue;   // x precedes y
+      if (y[q] < 0) continue;
       if (f & kFlD) atomicAdd(&S.K[y[q]], ksign);
-      if (f & kFlX) {
+      if (xc) {
         const uint8_t fy = S.fl[y[q]];
         int32_t dlt;
         if (fy & kFlX) dlt = b == S.newc[y[q]] ? 1 : 0;
@@ -306,41 +411,67 @@ __device__ __forceinline__ void ldf_row_part(const LdfState& S, int32_t x, uint8
         }
         if (dlt) atomicAdd(&S.cc[y[q]], dlt);
         S.pm[y[q]] = -1;
+        G.ccol[(uint64_t)G.row[y[q]].x + (uint32_t)t[q]] = b;
       }
     }
   }
 }
 
-// launch 3: the row passes of D and X (short rows by their own thread, long
-// rows by a wave), and each X member's new colour into its neighbours' slot
-// caches, histograms and all-neighbour mex caches
+// launch 3: the row passes of D and X
 __global__ __launch_bounds__(kBlock) void k_ldf_rows(LdfState S, int32_t r) {
   if (S.ctl->done_round < r) return;
   const GraphView& G = S.G;
   const uint8_t* Cf = S.F[(r + 1) & 1];
+  auto pass = [&](int32_t x, uint32_t lo, uint32_t hi_cap, uint32_t stride) {
+    const uint8_t f = S.fl[x];
+    const uint2 rw = G.row[x];
+    const uint32_t hi = min(rw.y, hi_cap);
+    const int32_t a = S.c0[x];
+    const int32_t b = (f & kFlX) ? S.newc[x] : a;
+    ldf_row_part(S, x, f, Cf[x] ? 1 : -1, a, b, rw.x, lo, hi, stride);
+  };
   for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < G.n_owned;
        v += (int64_t)gridDim.x * blockDim.x) {
-    const uint8_t f = S.fl[v];
-    if (!(f & (kFlD | kFlX)) || G.hidx[v] >= 0) continue;
-    const uint2 rw = G.row[v];
-    const int32_t a = S.c0[v];
-    const int32_t b = (f & kFlX) ? S.newc[v] : a;
-    ldf_row_part(S, (int32_t)v, f, Cf[v] ? 1 : -1, a, b, rw.x, 0, rw.y, 1);
-    if (f & kFlX) propagate_slots(G, b, a, rw.x, 0, rw.y, 1);
+    if (!(S.fl[v] & (kFlD | kFlX)) || G.hidx[v] >= 0) continue;
+    pass((int32_t)v, 0u, 0xffffffffu, 1u);
   }
-  const int lane = lane_id();
-  const uint32_t gwave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
-  for (uint32_t i = gwave; i < S.n_hrows; i += nwaves) {
-    const int32_t v = S.hrowv[i];
-    const uint8_t f = S.fl[v];
-    if (!(f & (kFlD | kFlX))) continue;
-    const uint2 rw = G.row[v];
-    const int32_t a = S.c0[v];
-    const int32_t b = (f & kFlX) ? S.newc[v] : a;
-    ldf_row_part(S, v, f, Cf[v] ? 1 : -1, a, b, rw.x, lane, rw.y, 64);
-    if (f & kFlX) propagate_slots(G, b, a, rw.x, lane, rw.y, 64);
+  auto moved = [&](int32_t v) { return (S.fl[v] & (kFlD | kFlX)) != 0; };
+  each_long_row<true>(S, moved, [&](int32_t v) { pass(v, (uint32_t)lane_id(), 0xffffffffu, 64u); });
+  each_hub_chunk(S, moved, [&](int2 e) {
+    const uint32_t lo = (uint32_t)e.y * kChunk;
+    pass(e.x, lo + threadIdx.x, lo + kChunk, kBlock);
+  });
+}
+
+// after the loop: every vertex whose colour differs from its colour at the
+// loop's start writes its final colour into its slots in its predecessors'
+// rows (the rows its round passes skipped) and moves all its neighbours'
+// histograms and all-neighbour mex caches from the start colour to it
+__global__ __launch_bounds__(kBlock) void k_ldf_settle(LdfState S) {
+  const GraphView& G = S.G;
+  auto pass = [&](int32_t x, uint32_t lo, uint32_t hi_cap, uint32_t stride) {
+    const uint2 rw = G.row[x];
+    const uint32_t hi = min(rw.y, hi_cap);
+    const int32_t a = S.corig[x], b = S.c0[x];
+    for (uint32_t j = lo; j < hi; j += stride) {
+      const uint64_t sl = (uint64_t)rw.x + j;
+      const int32_t y = G.col[sl];
+      if (y < 0) continue;
+      if (S.pdir[sl]) nbr_colour<true>(G, y, G.twin[sl], b, a);
+      else nbr_colour<false>(G, y, 0, b, a);
+    }
+  };
+  auto changed = [&](int32_t v) { return S.corig[v] >= 0 && S.corig[v] != S.c0[v]; };
+  for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < G.n_owned;
+       v += (int64_t)gridDim.x * blockDim.x) {
+    if (G.hidx[v] >= 0 || !changed((int32_t)v)) continue;
+    pass((int32_t)v, 0u, 0xffffffffu, 1u);
   }
+  each_long_row<true>(S, changed, [&](int32_t v) { pass(v, (uint32_t)lane_id(), 0xffffffffu, 64u); });
+  each_hub_chunk(S, changed, [&](int2 e) {
+    const uint32_t lo = (uint32_t)e.y * kChunk;
+    pass(e.x, lo + threadIdx.x, lo + kChunk, kBlock);
+  });
 }
 
 // launch 4: commit the new colours; round r's counts into the trace; the
@@ -350,6 +481,7 @@ __global__ __launch_bounds__(kBlock) void k_ldf_commit(LdfState S, int32_t r) {
   for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < S.G.n_owned;
        v += (int64_t)gridDim.x * blockDim.x)
     if (S.fl[v] & kFlX) {
+      if (S.corig[v] < 0) S.corig[v] = S.c0[v];
       S.c0[v] = S.newc[v];
       S.c1[v] = S.newc[v];
     }
@@ -380,6 +512,7 @@ struct LdfHost {
   int32_t trace_cap = 0;
   LdfCtl* hctl = nullptr;
   uint8_t* flags = nullptr;   // F[0], F[1], fl
+  uint64_t pdir_cap = 0;      // slots pdir holds
 };
 
 static LdfHost* ldf_state(Graph* g) { return static_cast<LdfHost*>(g->ldf); }
@@ -390,8 +523,8 @@ void ldf_free(Graph* g) {
   auto f = [](void* p) {
     if (p) (void)hipFree(p);
   };
-  f(s->S.K); f(s->S.cc); f(s->S.pm); f(s->S.newc); f(s->flags); f(s->S.acc); f(s->S.trace);
-  f(s->S.ctl);
+  f(s->S.K); f(s->S.cc); f(s->S.pm); f(s->S.newc); f(s->S.corig); f(s->flags); f(s->S.acc); f(s->S.trace);
+  f(s->S.ctl); f(s->S.pdir);
   if (s->hctl) (void)hipHostFree(s->hctl);
   delete s;
   g->ldf = nullptr;
@@ -404,6 +537,7 @@ static int ldf_alloc(Graph* g, LdfHost* s) {
   DC_HIP(hipMalloc(&S.cc, sizeof(int32_t) * n));
   DC_HIP(hipMalloc(&S.pm, sizeof(int32_t) * n));
   DC_HIP(hipMalloc(&S.newc, sizeof(int32_t) * n));
+  DC_HIP(hipMalloc(&S.corig, sizeof(int32_t) * n));
   DC_HIP(hipMalloc(&s->flags, 3 * n));
   S.F[0] = s->flags;
   S.F[1] = s->flags + n;
@@ -452,8 +586,18 @@ int loop_ldf(Graph* g, Frontier L0, LoopResult* res) {
   S.c1 = g->color[g->cur ^ 1];
   S.hrowv = g->hrowv;
   S.n_hrows = g->n_hrows;
+  S.hch = g->hch;
+  S.n_hch = g->n_hch;
+  if (s->pdir_cap < g->total_cap) {
+    if (S.pdir) DC_HIP(hipFree(S.pdir));
+    S.pdir = nullptr;
+    s->pdir_cap = 0;
+    DC_HIP(hipMalloc(&S.pdir, g->total_cap));
+    s->pdir_cap = g->total_cap;
+  }
   const size_t n = (size_t)g->n + 1;
   DC_HIP(hipMemsetAsync(S.K, 0, sizeof(int32_t) * n, g->stream));
+  DC_HIP(hipMemsetAsync(S.corig, 0xff, sizeof(int32_t) * n, g->stream));
   DC_HIP(hipMemsetAsync(s->flags, 0, 3 * n, g->stream));
   DC_HIP(hipMemsetAsync(S.acc, 0, sizeof(unsigned long long) * 2 * kLdfAcc * 4, g->stream));
   *s->hctl = LdfCtl{};
@@ -484,7 +628,11 @@ int loop_ldf(Graph* g, Frontier L0, LoopResult* res) {
     r += nb;
     if (s->hctl->done_round != INT_MAX) rounds = s->hctl->done_round + 1;
   }
+  hipLaunchKernelGGL(k_ldf_settle, grid, blk, 0, g->stream, S);
+  DC_HIP(hipGetLastError());
+  ++launches;
   if (rounds < 0) {
+    DC_HIP(hipStreamSynchronize(g->stream));
     set_error("recolor loop hit max_rounds");
     return DC_ERR_ROUNDS;
   }
