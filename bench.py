@@ -258,13 +258,22 @@ def run_stream(args, d, rule, n, u, v, workload):
     if args.driver != "fused" and rule == d.RULE_INDEPENDENT and prof["launches"]:
         out["roofline"] = roofline("k_round_c (count pass)", prof["bytes"], prof["ms"],
                                    prof["launches"])
+    elif args.driver != "fused" and prof["launches"]:
+        # ldf rule: the loop's rounds as the unit (four launches each); bytes are
+        # the reference's for those rounds (it rescans every row of L_r u C_r)
+        out["roofline"] = roofline("ldf loop round", prof["bytes"], prof["ms"], prof["launches"])
+        out["roofline"]["basis"] = (
+            "achieved and frac: the reference's bytes per loop round (16 per vertex of L_r u C_r, "
+            "12 per row slot of L_r u C_r: neighbour, colour, priority; n/8 bitmap; 4 per member "
+            "of C_r) / mean round time (k_ldf_members + k_ldf_long_mex + k_ldf_rows + "
+            "k_ldf_commit, hipEvents around each block of rounds)")
     else:
         out["roofline"] = roofline("k_loop_persistent" if args.driver == "fused" else
                                    "k_recolor", prof["rbytes"], prof["kms"], prof["rl"])
     out["roofline"]["measured_on"] = (
         f"{nprof} batches after the timed ones" if args.events_after else
         f"replay of the warm-up and timed batches (same {replay_rounds} rounds)")
-    out["reference_model"] = reference_model(terms, n)
+    out["reference_model"] = reference_model(terms, n, rule == d.RULE_LDF)
     out["work"] = {"loop_rounds": tot["rounds"], "evaluated": tot["evaluated"],
                    "changed": tot["changed"], "row_slots_scanned": tot["slots"],
                    "batches_with_row_growth": tot["regrows"],
@@ -275,25 +284,27 @@ def run_stream(args, d, rule, n, u, v, workload):
     return out
 
 
-def reference_model(t, n):
+def reference_model(t, n, ldf=False):
     """Bytes the reference's algorithm moves for the same batches (per batch):
     12 per update record + 8 per adjacency write or tombstone + 4 per row slot
     of both endpoints per deletion (its linear deletion search) + per loop round
     16 + 8 per row slot for every vertex of L_r u C_r (it rescans each of their
-    rows) + a bitmap over the vertices and 4 per member of C_r (compaction),
-    counted over the rounds this build ran (rounds after settled colours are
-    not counted: the figure is a lower bound).  achieved = those bytes over the
-    summed recolour-kernel time of the same batches."""
+    rows; 12 per slot for the ldf rule, whose scans also gather priorities) + a
+    bitmap over the vertices and 4 per member of C_r (compaction), counted over
+    the rounds this build ran (rounds after settled colours are not counted:
+    the figure is a lower bound).  achieved = those bytes over the summed
+    recolour-kernel time of the same batches."""
     b = max(t["batches"], 1)
-    B = (20 * t["updates"] + 4 * t["del_row_slots"] + 16 * t["lc_vertices"] + 8 * t["lc_slots"]
-         + t["loop_rounds"] * (n / 8.0) + 4 * t["sum_C"])
+    per_slot = 12 if ldf else 8
+    B = (20 * t["updates"] + 4 * t["del_row_slots"] + 16 * t["lc_vertices"]
+         + per_slot * t["lc_slots"] + t["loop_rounds"] * (n / 8.0) + 4 * t["sum_C"])
     s = t["kernel_ms"] / 1e3
     ach = B / s / 1e9 if s > 0 else None
     return {"terms_per_batch": {k: t[k] / b for k in ("updates", "del_row_slots", "lc_vertices",
                                                      "lc_slots", "loop_rounds", "sum_C")},
             "bytes_per_batch": B / b, "recolor_kernel_ms_per_batch": t["kernel_ms"] / b,
             "achieved_GBs": ach, "frac_of_peak": ach / HBM_PEAK_GBS if ach else None,
-            "formula": "20*U + 4*del_row_slots + 16*|L u C| + 8*slots(L u C) + rounds*n/8 + 4*sum|C|"}
+            "formula": f"20*U + 4*del_row_slots + 16*|L u C| + {per_slot}*slots(L u C) + rounds*n/8 + 4*sum|C|"}
 
 
 def cpu_baseline(args, n, u, v, batch, rule, gpu_colors_after_first):

@@ -68,6 +68,10 @@ struct LdfState {
   unsigned long long* acc;   // [2][kLdfAcc][4]: |L|, |C|, changed
   int4* trace;
   LdfCtl* ctl;
+  // profiling: the reference's work terms, summed over rounds: |L_r u C_r| and
+  // the row slots of L_r u C_r (the rows it rescans every round)
+  unsigned long long* terms;
+  int32_t prof;
 };
 
 // ---------------------------------------------------------------------------
@@ -478,13 +482,29 @@ __global__ __launch_bounds__(kBlock) void k_ldf_settle(LdfState S) {
 // loop ends after a round without a colour change
 __global__ __launch_bounds__(kBlock) void k_ldf_commit(LdfState S, int32_t r) {
   if (S.ctl->done_round < r) return;
+  unsigned long long lcv = 0ull, lcs = 0ull;
   for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < S.G.n_owned;
-       v += (int64_t)gridDim.x * blockDim.x)
+       v += (int64_t)gridDim.x * blockDim.x) {
     if (S.fl[v] & kFlX) {
       if (S.corig[v] < 0) S.corig[v] = S.c0[v];
       S.c0[v] = S.newc[v];
       S.c1[v] = S.newc[v];
     }
+    if (S.prof && (S.F[0][v] | S.F[1][v])) {   // v in L_r u C_r
+      ++lcv;
+      lcs += S.G.row[v].y;
+    }
+  }
+  if (S.prof) {
+    for (int off = 32; off > 0; off >>= 1) {
+      lcv += __shfl_xor(lcv, off);
+      lcs += __shfl_xor(lcs, off);
+    }
+    if (lane_id() == 0 && lcv) {
+      atomicAdd(&S.terms[0], lcv);
+      atomicAdd(&S.terms[1], lcs);
+    }
+  }
   if (blockIdx.x == 0) {
     unsigned long long* nxt = S.acc + (size_t)((r + 1) & 1) * kLdfAcc * 4;
     if (threadIdx.x < 64) {
@@ -513,6 +533,7 @@ struct LdfHost {
   LdfCtl* hctl = nullptr;
   uint8_t* flags = nullptr;   // F[0], F[1], fl
   uint64_t pdir_cap = 0;      // slots pdir holds
+  hipEvent_t ev[2] = {nullptr, nullptr};   // around a block of rounds (profiling)
 };
 
 static LdfHost* ldf_state(Graph* g) { return static_cast<LdfHost*>(g->ldf); }
@@ -524,8 +545,10 @@ void ldf_free(Graph* g) {
     if (p) (void)hipFree(p);
   };
   f(s->S.K); f(s->S.cc); f(s->S.pm); f(s->S.newc); f(s->S.corig); f(s->flags); f(s->S.acc); f(s->S.trace);
-  f(s->S.ctl); f(s->S.pdir);
+  f(s->S.ctl); f(s->S.pdir); f(s->S.terms);
   if (s->hctl) (void)hipHostFree(s->hctl);
+  for (hipEvent_t e : s->ev)
+    if (e) (void)hipEventDestroy(e);
   delete s;
   g->ldf = nullptr;
 }
@@ -544,7 +567,10 @@ static int ldf_alloc(Graph* g, LdfHost* s) {
   S.fl = s->flags + 2 * n;
   DC_HIP(hipMalloc(&S.acc, sizeof(unsigned long long) * 2 * kLdfAcc * 4));
   DC_HIP(hipMalloc(&S.ctl, sizeof(LdfCtl)));
+  DC_HIP(hipMalloc(&S.terms, sizeof(unsigned long long) * 2));
   DC_HIP(hipHostMalloc(&s->hctl, sizeof(LdfCtl), hipHostMallocDefault));
+  DC_HIP(hipEventCreate(&s->ev[0]));
+  DC_HIP(hipEventCreate(&s->ev[1]));
   s->n = g->n;
   return DC_OK;
 }
@@ -600,6 +626,8 @@ int loop_ldf(Graph* g, Frontier L0, LoopResult* res) {
   DC_HIP(hipMemsetAsync(S.corig, 0xff, sizeof(int32_t) * n, g->stream));
   DC_HIP(hipMemsetAsync(s->flags, 0, 3 * n, g->stream));
   DC_HIP(hipMemsetAsync(S.acc, 0, sizeof(unsigned long long) * 2 * kLdfAcc * 4, g->stream));
+  S.prof = g->profiling ? 1 : 0;
+  if (S.prof) DC_HIP(hipMemsetAsync(S.terms, 0, sizeof(unsigned long long) * 2, g->stream));
   *s->hctl = LdfCtl{};
   s->hctl->done_round = INT_MAX;
   DC_HIP(hipMemcpyAsync(S.ctl, s->hctl, sizeof(LdfCtl), hipMemcpyHostToDevice, g->stream));
@@ -612,9 +640,11 @@ int loop_ldf(Graph* g, Frontier L0, LoopResult* res) {
   int64_t launches = 3;
   const int32_t max_rounds = g->opt.max_rounds;
   int32_t rounds = -1, r = 0;
+  double loop_ms = 0.0;
   while (r < max_rounds && rounds < 0) {
     const int32_t nb = std::min(ldf_block_end(r), max_rounds) - r;
     if ((rc = ldf_trace(g, s, r + nb))) return rc;
+    if (S.prof) DC_HIP(hipEventRecord(s->ev[0], g->stream));
     for (int32_t i = 0; i < nb; ++i) {
       hipLaunchKernelGGL(k_ldf_members, grid, blk, 0, g->stream, S, r + i);
       hipLaunchKernelGGL(k_ldf_long_mex, grid, blk, 0, g->stream, S, r + i);
@@ -623,8 +653,14 @@ int loop_ldf(Graph* g, Frontier L0, LoopResult* res) {
     }
     DC_HIP(hipGetLastError());
     launches += 4 * nb;
+    if (S.prof) DC_HIP(hipEventRecord(s->ev[1], g->stream));
     DC_HIP(hipMemcpyAsync(s->hctl, S.ctl, sizeof(LdfCtl), hipMemcpyDeviceToHost, g->stream));
     DC_HIP(hipStreamSynchronize(g->stream));
+    if (S.prof) {
+      float ms = 0.f;
+      DC_HIP(hipEventElapsedTime(&ms, s->ev[0], s->ev[1]));
+      loop_ms += ms;
+    }
     r += nb;
     if (s->hctl->done_round != INT_MAX) rounds = s->hctl->done_round + 1;
   }
@@ -649,6 +685,22 @@ int loop_ldf(Graph* g, Frontier L0, LoopResult* res) {
     res->changed += (uint64_t)tr[i].z;
   }
   res->launches = launches;
+  if (S.prof) {
+    // the rounds' kernel time, and the reference's bytes for them: per
+    // vertex of L_r u C_r its list entry, row bounds and own colour (16),
+    // per row slot the neighbour ID, its colour and its priority (12), per
+    // round a compaction bitmap over the vertices (n/8) and 4 per member of
+    // C_r
+    unsigned long long t[2];
+    DC_HIP(hipMemcpy(t, S.terms, sizeof(t), hipMemcpyDeviceToHost));
+    res->lc_vertices = t[0];
+    res->lc_slots = t[1];
+    res->kernel_ms = loop_ms;
+    res->count_ms = loop_ms;
+    res->count_launches = rounds;
+    res->count_bytes = 16ull * t[0] + 12ull * t[1] +
+                       (uint64_t)rounds * ((uint64_t)g->n_owned / 8) + 4ull * res->evaluated;
+  }
   return DC_OK;
 }
 

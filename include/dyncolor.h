@@ -110,9 +110,12 @@ typedef struct {
   int32_t recolor_launches; /* recolor kernel launches in the batch           */
   int32_t regrows;          /* batches whose rows outgrew their capacity      */
   double ms_count_pass;     /* summed duration of the independent-rule loop's
-                               count-pass launches (profiling on, plain driver) */
-  int64_t count_pass_bytes; /* algorithmic bytes of those launches            */
-  int32_t count_pass_launches;
+                               count-pass launches (profiling on, plain driver);
+                               for the ldf rule: of the loop's rounds (four
+                               launches each)                                 */
+  int64_t count_pass_bytes; /* algorithmic bytes of those launches (ldf rule:
+                               the reference's bytes for those rounds)        */
+  int32_t count_pass_launches;  /* launches (ldf rule: rounds)                */
   int32_t loop_settled;     /* 1 if the loop stopped at settled colours       */
   int32_t pull_rounds;      /* loop rounds whose counts were recounted by pull */
   int32_t heavy_push_rounds;  /* loop rounds that pushed count changes from more

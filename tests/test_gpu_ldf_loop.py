@@ -120,3 +120,27 @@ def test_ldf_tuning_range():
     t.ldf_loop = 2
     with pytest.raises(d.DynColorError):
         g.set_tuning(t)
+
+
+def test_rules_alternate_on_one_graph(oracle_mod):
+    # the heuristic-based loop leaves the histograms, the all-neighbour mex
+    # caches and the slot colours of predecessors' rows to one pass after the
+    # loop; an independent-rule batch next reads all of them
+    say("ldf / independent alternating")
+    from traces import same_trace
+    u, v = d.gen_rmat(14, 16, seed=9)
+    n = 1 << 14
+    batches = d.gen_batches(n, u, v, 6, 2000, 0.5, 31)
+    g = d.Graph(n, u, v)
+    og = oracle_mod.OracleGraph(n, u, v)
+    g.static_color(1)
+    og.static_color(1)
+    for i, (bu, bv, bi) in enumerate(batches):
+        rule = 1 if i % 2 == 0 else 0
+        g.process_batch(bu, bv, bi, rule)
+        og.apply_batch(bu, bv, bi)
+        ph = og.recolor(rule)
+        assert (g.colors() == og.color).all(), f"batch {i} rule {rule}"
+        assert same_trace(g.trace(), ph), f"batch {i} rule {rule}: trace"
+        assert g.check_consistency() == 0
+    assert g.validate()[0] == 0
