@@ -424,6 +424,16 @@ def run_c4(args, d, rule, steps, warmup):
     lines = sum(step_lines[warmup:])
     batched = lines / t_batched if mono == 0 else 0.0
     log(f"c4 batched: {t_batched:.3f}s for {steps} steps, rounds {rounds}, mono {mono}")
+    rec = {
+        "workload": f"c4_{len(graphs)}graphs_lognormal20k_er_rmat_1to10pct",
+        "value": batched, "unit": "updates/s", "steps": steps, "warmup": warmup,
+        "ms_per_step": t_batched / steps * 1e3, "p50_ms": pct(np.array(lat) * 1e3, 50),
+        "p99_ms": pct(np.array(lat) * 1e3, 99),
+        "graphs": len(graphs), "vertices": int(n), "edges": int(tot_edges),
+        "lines_per_step": int(step_lines[-1]), "loop_rounds": rounds, "monochromatic": int(mono),
+        "tuning": args.tuned.get("c4") or "default"}
+    if args.no_plain:   # profiling the arena alone
+        return rec
 
     # plain per-batch baseline: one graph handle and one plain call per graph
     t_plain = 0.0
@@ -445,16 +455,9 @@ def run_c4(args, d, rule, steps, warmup):
     gc.collect()
     plain = lines / t_plain
     log(f"c4 plain: {t_plain:.3f}s, speedup {batched / plain:.1f}x, colors equal: {same}")
-    lat_ms = np.array(lat) * 1e3
-    return {
-        "workload": f"c4_{len(graphs)}graphs_lognormal20k_er_rmat_1to10pct",
-        "value": batched, "unit": "updates/s", "steps": steps, "warmup": warmup,
-        "ms_per_step": t_batched / steps * 1e3, "p50_ms": pct(lat_ms, 50), "p99_ms": pct(lat_ms, 99),
-        "graphs": len(graphs), "vertices": int(n), "edges": int(tot_edges),
-        "lines_per_step": int(step_lines[-1]), "loop_rounds": rounds, "monochromatic": int(mono),
-        "plain_value": plain, "plain_seconds": t_plain, "plain_calls_per_step": len(graphs),
-        "tuning": args.tuned.get("c4") or "default",
-        "speedup_vs_plain": batched / plain, "bit_equal_to_plain": bool(same)}
+    rec.update({"plain_value": plain, "plain_seconds": t_plain, "plain_calls_per_step": len(graphs),
+                "speedup_vs_plain": batched / plain, "bit_equal_to_plain": bool(same)})
+    return rec
 
 
 # ---------------------------------------------------------------------- c5 --
@@ -666,6 +669,8 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--graphs", type=int, default=512)
     ap.add_argument("--c4-steps", type=int, default=3, help="timed c4 steps of the default run")
+    ap.add_argument("--no-plain", action="store_true",
+                    help="c4: time the arena only (no plain per-graph leg), for profiling")
     ap.add_argument("--rates", type=float, nargs="+", default=[80, 320, 1280, 5120])
     ap.add_argument("--duration", type=float, default=1.5)
     ap.add_argument("--max-requests", type=int, default=6000)
@@ -743,7 +748,7 @@ def main():
         if not args.no_batched:
             b = run_c4(args, d, rule, args.c4_steps, 1)
             out["batched"] = b
-            out["vs_baseline"] = b["speedup_vs_plain"]
+            out["vs_baseline"] = b.get("speedup_vs_plain")
             out["vs_baseline_basis"] = ("batched (512 graphs per step in one arena) updates/s over "
                                         "the plain per-batch path (one call per graph), same inputs")
         if not args.no_served:
@@ -761,7 +766,7 @@ def main():
                           {"workload": b["workload"], "graphs": b["graphs"], "rule": args.rule,
                            "driver": args.driver})
         out["batched"] = b
-        out["vs_baseline"] = b["speedup_vs_plain"]
+        out["vs_baseline"] = b.get("speedup_vs_plain")
     else:
         s = run_c5(args, d, rule)
         out = base_record(s["value"], 0.0, 0, 0, {"workload": s["workload"], "rule": args.rule,

@@ -404,8 +404,8 @@ int dc_gen_er(int32_t n, int64_t m, uint64_t seed, int32_t* u, int32_t* v);
 
 /* R-MAT: 2^scale vertices, edge_factor * 2^scale samples with quadrant
  * probabilities a, b, c (d = 1-a-b-c); self loops and duplicates removed.
- * Returns the unique edge count through *m_out; u/v need room for all samples
- * (pass NULL to only count... not supported: capacity must be >= samples). */
+ * u and v need room for edge_factor * 2^scale entries (every sample); the
+ * unique edge count is returned through *m_out. */
 int dc_gen_rmat(int32_t scale, int32_t edge_factor, double a, double b, double c,
                 uint64_t seed, int32_t* u, int32_t* v, int64_t* m_out);
 
