@@ -39,6 +39,7 @@ EXPORTS = [
     "dc_tuning_default", "dc_set_tuning", "dc_get_tuning", "dc_device_info", "dc_set_profiling", "dc_set_driver", "dc_set_parts", "dc_synchronize", "dc_last_error", "dc_device_count",
     "dc_gen_er", "dc_gen_rmat", "dc_gen_batches",
     "dc_server_create", "dc_server_destroy", "dc_server_run", "dc_server_request_bytes",
+    "dc_lanes_step",
     "dc_partition_plan_create", "dc_partition_plan_info", "dc_partition_plan_arrays",
     "dc_partition_plan_border", "dc_partition_create_global", "dc_partition_plan_destroy",
 ]
@@ -175,6 +176,7 @@ def lib():
         "dc_partition_create_global": (C.c_int, [_vp, _i32p, C.POINTER(Options), C.POINTER(_vp)]),
         "dc_partition_plan_destroy": (C.c_int, [_vp]),
         "dc_server_destroy": (C.c_int, [_vp]),
+        "dc_lanes_step": (C.c_int, [C.c_int32, _vp, _vp, C.c_int32, _vp]),
         "dc_server_run": (C.c_int, [_vp, C.c_int64, _i32p, _i64p, _vp, _vp, _vp,
                                     np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS"),
                                     np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS"),
@@ -569,6 +571,19 @@ class Server:
         _check(lib().dc_server_run(self._h, nreq, gid, roff, _ptr(u), _ptr(v), _ptr(ins),
                                    arrival, lat, step, C.byref(st)))
         return lat[:nreq], step[:nreq], st
+
+
+def lanes_step(arenas, batches, rule=RULE_INDEPENDENT):
+    """dc_lanes_step: lane l applies batches[l] (device batches of arenas[l])
+    and recolors; the lanes run concurrently.  Returns each lane's Stats."""
+    if len(arenas) != len(batches) or not arenas:
+        raise ValueError("one device batch per arena")
+    k = len(arenas)
+    ah = (C.c_void_p * k)(*[a.handle for a in arenas])
+    bh = (C.c_void_p * k)(*[b._h for b in batches])
+    st = (Stats * k)()
+    _check(lib().dc_lanes_step(k, ah, bh, rule, st))
+    return list(st)
 
 
 def request_bytes(lines, insertions):

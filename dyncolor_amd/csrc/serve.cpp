@@ -325,4 +325,52 @@ int dc_server_run(dc_server* s, int64_t nreq, const int32_t* gid, const int64_t*
   });
 }
 
+int dc_lanes_step(int32_t nlanes, dc_graph* const* arenas, const dc_batch* const* batches,
+                  int32_t rule, dc_stats* stats) {
+  return dc::guarded("dc_lanes_step", [&]() -> int {
+    if (nlanes < 1 || !arenas || !batches) {
+      dc::set_error("dc_lanes_step: invalid argument");
+      return DC_ERR_INVALID;
+    }
+    for (int32_t l = 0; l < nlanes; ++l) {
+      if (!arenas[l] || !batches[l]) {
+        dc::set_error("dc_lanes_step: null arena or batch");
+        return DC_ERR_INVALID;
+      }
+      for (int32_t k = 0; k < l; ++k)
+        if (arenas[k] == arenas[l]) {
+          dc::set_error("dc_lanes_step: an arena appears twice");
+          return DC_ERR_INVALID;
+        }
+    }
+    // lane l on its own host thread (lane 0 on the caller's): each arena has
+    // its own stream, so the lanes' launches overlap on the device
+    std::vector<int> rc((size_t)nlanes, DC_OK);
+    std::vector<std::string> msg((size_t)nlanes);
+    auto lane = [&](int32_t l) {
+      int r = dc_apply_device_batch(arenas[l], batches[l]);
+      if (r == DC_OK) r = dc_recolor(arenas[l], rule, stats ? &stats[l] : nullptr);
+      if (r == DC_OK) r = dc_synchronize(arenas[l]);
+      rc[(size_t)l] = r;
+      if (r != DC_OK) msg[(size_t)l] = dc_last_error();
+    };
+    std::vector<std::thread> th;
+    th.reserve((size_t)nlanes - 1);
+    try {
+      for (int32_t l = 1; l < nlanes; ++l) th.emplace_back(lane, l);
+    } catch (...) {
+      for (auto& t : th) t.join();
+      throw;
+    }
+    lane(0);
+    for (auto& t : th) t.join();
+    for (int32_t l = 0; l < nlanes; ++l)
+      if (rc[(size_t)l] != DC_OK) {
+        dc::set_error("dc_lanes_step: lane " + std::to_string(l) + ": " + msg[(size_t)l]);
+        return rc[(size_t)l];
+      }
+    return DC_OK;
+  });
+}
+
 }  // extern "C"

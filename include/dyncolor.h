@@ -393,6 +393,18 @@ int dc_server_run(dc_server* s, int64_t nreq, const int32_t* gid, const int64_t*
                   const int32_t* u, const int32_t* v, const uint8_t* is_insert,
                   const double* arrival, double* latency, int32_t* step_of_req,
                   dc_serve_stats* st);
+/* Batched step over lanes: lane l applies the device batch batches[l] to the
+ * arena arenas[l] (distinct handles) and recolors it with `rule`; the lanes
+ * run concurrently (one host thread and the arena's own stream each) and the
+ * call returns when every lane's colors are final.  stats: nlanes entries (or
+ * NULL).  On a failure the first failing lane's code is returned and
+ * dc_last_error names the lane.  No counterpart in the reference (one graph
+ * per process, HI/Color_main.cu:31-247); it serves the batched workload:
+ * graphs split over a few arenas overlap on the device where one arena's loop
+ * would wait for its slowest graph. */
+int dc_lanes_step(int32_t nlanes, dc_graph* const* arenas, const dc_batch* const* batches,
+                  int32_t rule, dc_stats* stats);
+
 /* The device bytes a request of `lines` lines, `insertions` of them
  * insertions, counts against the budget (staged lines, change-list entries,
  * row slots its insertions may add); -1 for bad sizes. */
