@@ -23,8 +23,14 @@
 // One round = four launches over per-vertex flag bytes (no worklists, so no
 // appends on shared counters): members and their mex (short rows by their
 // own thread), stale mex of long rows (a wave per row of the long-row
-// table), the row passes of D and X (K, cc, mex caches, and the slot colour
-// cache / histograms / all-neighbour mex cache of the other loop), commit.
+// table), the row passes of D and X over their successors (K, cc, cached
+// mexes, the changed colour into the successors' slot colour caches), commit.
+// The batch's set-up records which slots hold a predecessor (pdir), so the
+// scans and row passes read no priorities.  The loop's scans read only
+// predecessors' slots, so the changed vertices' slots in their predecessors'
+// rows, and their neighbours' histograms and all-neighbour mex caches (the
+// independent rule's), take the final colours once after the loop
+// (k_ldf_settle).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -602,7 +608,10 @@ int loop_ldf(Graph* g, Frontier L0, LoopResult* res) {
     s = new LdfHost();
     g->ldf = s;
     int rc = ldf_alloc(g, s);
-    if (rc) return rc;
+    if (rc) {
+      ldf_free(g);   // a later call starts over instead of using a partial state
+      return rc;
+    }
   }
   int rc = ldf_trace(g, s, 256);
   if (rc) return rc;
