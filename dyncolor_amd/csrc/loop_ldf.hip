@@ -45,6 +45,12 @@ constexpr int kLdfBlocks = 2048;
 constexpr int kLdfAcc = 32;        // round sums: workgroup b adds into slot b % kLdfAcc
 constexpr int kWin32 = kWinBits / 32;   // 32-bit words of a wave's mex window
 constexpr uint8_t kFlD = 1, kFlX = 2, kFlPm = 4;   // per-vertex flags of a round
+// pdir per slot: the neighbour does not precede the row's vertex, precedes it,
+// or precedes it and is a large hub (cap >= kLazyCap: its colour is read from
+// the colour array, not from the slot colour cache, during the loop, so its
+// colour changes write into none of its many successors' rows)
+constexpr uint8_t kPdNone = 0, kPdPred = 1, kPdHub = 2;
+constexpr uint32_t kLazyCap = kMidLen;
 
 struct LdfCtl {
   int32_t done_round;   // the last round run (INT_MAX while running)
@@ -65,6 +71,10 @@ struct LdfState {
   int32_t* newc;
   int32_t* corig;       // colour at the loop's start of every vertex that changed (-1: none)
   uint8_t* pdir;        // slot -> its neighbour precedes the row's vertex (set up per batch)
+  uint32_t* pslot;      // rows with cap >= kSmallLen: their predecessors, listed from the
+                        // row's start: slot position, or (top bit set) a hub's vertex ID
+                        // (set up per batch)
+  int32_t* npred;       // ... and how many
   uint8_t* F[2];        // F[r & 1] = L_r, F[(r + 1) & 1] = C_r = L_{r+1}
   uint8_t* fl;          // kFlD / kFlX / kFlPm of the current round
   const int32_t* hrowv; // rows with cap >= kSmallLen
@@ -97,15 +107,16 @@ __device__ __forceinline__ int32_t mex64(uint32_t lo, uint32_t hi) {
 // priorities and records the predecessor slots in pdir (fixed for the batch:
 // no priority moves inside the loop); the loop's scans read pdir.
 template <bool INIT>
-__device__ __forceinline__ void short_pred_scan(const GraphView& G, uint8_t* pdir, int32_t v,
-                                                int32_t cv, int32_t& pm, int32_t& cc) {
+__device__ __forceinline__ void short_pred_scan(const GraphView& G, uint8_t* pdir,
+                                                const int32_t* col0, int32_t v, int32_t cv,
+                                                int32_t& pm, int32_t& cc) {
   const uint2 r = G.row[v];
   const int32_t pv = G.prop[v];
   uint32_t lo = 0u, hi = 0u;
   int32_t k = 0;
   for (uint32_t j0 = 0; j0 < r.y; j0 += 4) {
     int32_t c[4];
-    bool pr[4];
+    uint8_t pd[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const bool in = j0 + q < r.y;
@@ -113,15 +124,21 @@ __device__ __forceinline__ void short_pred_scan(const GraphView& G, uint8_t* pdi
       c[q] = in ? G.ccol[j] : -1;
       if (INIT) {
         const int32_t w = in ? G.col[j] : -1;
-        pr[q] = w >= 0 && precedes(nullptr, w, G.prop[w], v, pv);
-        if (in) pdir[j] = pr[q] ? 1 : 0;
+        const bool pr = w >= 0 && precedes(nullptr, w, G.prop[w], v, pv);
+        pd[q] = !pr ? kPdNone : G.cap[w] >= kLazyCap ? kPdHub : kPdPred;
+        if (in) pdir[j] = pd[q];
       } else {
-        pr[q] = in && pdir[j] != 0;
+        pd[q] = in ? pdir[j] : kPdNone;
       }
+    }
+    if (!INIT) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        if (pd[q] == kPdHub) c[q] = col0[G.col[(uint64_t)r.x + j0 + q]];
     }
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      if (!pr[q]) continue;
+      if (pd[q] == kPdNone) continue;
       add_colour_bit((uint32_t)c[q], lo, hi);
       k += c[q] == cv ? 1 : 0;
     }
@@ -132,34 +149,67 @@ __device__ __forceinline__ void short_pred_scan(const GraphView& G, uint8_t* pdi
 
 // A whole row of v by one wave: the mex over LDS windows of kWinBits colours
 // (a window past the first only when it is full) and the predecessor slots
-// with colour cv.  Every lane returns both.  INIT as for short_pred_scan.
+// with colour cv.  Every lane returns both.  INIT classifies every slot by
+// the priorities (pdir, as for short_pred_scan) and lists the predecessors in
+// pslot[row start ...] (npred[v] of them: the row's own span of a slot-sized
+// array; a slot position, or a hub predecessor's vertex ID with the top bit
+// set); the loop's scans walk that list, so a hub row with few predecessors
+// costs a few loads however long it is.
 template <bool INIT>
-__device__ __forceinline__ void wave_pred_scan(const GraphView& G, uint8_t* pdir, int32_t v,
-                                               int32_t cv, uint32_t* ww, int32_t& pm,
-                                               int32_t& cc) {
+__device__ __forceinline__ void wave_pred_scan(const GraphView& G, uint8_t* pdir, uint32_t* pslot,
+                                               int32_t* npred, const int32_t* col0, int32_t v,
+                                               int32_t cv, uint32_t* ww, int32_t& pm, int32_t& cc) {
   const int lane = lane_id();
   const uint2 r = G.row[v];
-  const int32_t pv = G.prop[v];
   int32_t k = 0;
-  for (int32_t base = 0;; base += kWinBits) {
-    for (int i = lane; i < kWin32; i += 64) ww[i] = 0u;
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    for (uint32_t j = lane; j < r.y; j += 64) {
+  uint32_t np;
+  for (int i = lane; i < kWin32; i += 64) ww[i] = 0u;
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  if (INIT) {
+    // classify and list the slots; window 0's bits from the same pass
+    const int32_t pv = G.prop[v];
+    uint32_t base = 0;
+    for (uint32_t j0 = 0; j0 < r.y; j0 += 64) {
+      const uint32_t j = j0 + (uint32_t)lane;
       const uint64_t s = (uint64_t)r.x + j;
-      bool pr;
-      if (INIT && base == 0) {
-        const int32_t w = G.col[s];
-        pr = w >= 0 && precedes(nullptr, w, G.prop[w], v, pv);
-        pdir[s] = pr ? 1 : 0;
-      } else {
-        pr = pdir[s] != 0;
+      const int32_t w = j < r.y ? G.col[s] : -1;
+      const bool pr = w >= 0 && precedes(nullptr, w, G.prop[w], v, pv);
+      if (j < r.y) pdir[s] = !pr ? kPdNone : G.cap[w] >= kLazyCap ? kPdHub : kPdPred;
+      const unsigned long long b = __ballot(pr);
+      if (pr) {
+        // a hub predecessor is listed by its vertex ID (top bit set): the
+        // loop reads its colour from the colour array
+        const bool hubp = G.cap[w] >= kLazyCap;
+        pslot[(uint64_t)r.x + base + (uint32_t)__popcll(b & ((1ull << lane) - 1ull))] =
+            hubp ? (0x80000000u | (uint32_t)w) : j;
+        const int32_t c = G.ccol[s];
+        if (c == cv) ++k;
+        if ((uint32_t)c < (uint32_t)kWinBits) atomicOr(&ww[(uint32_t)c >> 5], 1u << ((uint32_t)c & 31u));
       }
-      if (!pr) continue;
-      const int32_t c = G.ccol[s];
-      if (base == 0 && c == cv) ++k;
-      const uint32_t cw = (uint32_t)(c - base);
-      if (cw < (uint32_t)kWinBits) atomicOr(&ww[cw >> 5], 1u << (cw & 31u));
+      base += (uint32_t)__popcll(b);
+    }
+    if (lane == 0) npred[v] = (int32_t)base;
+    np = base;
+    // the list is read back (by other lanes) only past a full first window
+    __threadfence_block();
+  } else {
+    np = (uint32_t)npred[v];
+  }
+  for (int32_t base = 0;; base += kWinBits) {
+    if (!INIT || base > 0) {
+      if (base > 0) {
+        for (int i = lane; i < kWin32; i += 64) ww[i] = 0u;
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+      }
+      for (uint32_t t = lane; t < np; t += 64) {
+        const uint32_t e = pslot[(uint64_t)r.x + t];
+        const int32_t c = (e & 0x80000000u) ? col0[e & 0x7fffffffu] : G.ccol[(uint64_t)r.x + e];
+        if (base == 0 && c == cv) ++k;
+        const uint32_t cw = (uint32_t)(c - base);
+        if (cw < (uint32_t)kWinBits) atomicOr(&ww[cw >> 5], 1u << (cw & 31u));
+      }
     }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -265,14 +315,14 @@ __global__ __launch_bounds__(kBlock) void k_ldf_init_cc(LdfState S) {
        v += (int64_t)gridDim.x * blockDim.x) {
     if (G.hidx[v] >= 0) continue;
     int32_t pm, cc;
-    short_pred_scan<true>(G, S.pdir, (int32_t)v, S.c0[v], pm, cc);
+    short_pred_scan<true>(G, S.pdir, S.c0, (int32_t)v, S.c0[v], pm, cc);
     S.pm[v] = pm;
     S.cc[v] = cc;
   }
   uint32_t* ww = win[threadIdx.x >> 6];
   each_long_row<false>(S, [](int32_t) { return true; }, [&](int32_t v) {
     int32_t pm, cc;
-    wave_pred_scan<true>(G, S.pdir, v, S.c0[v], ww, pm, cc);
+    wave_pred_scan<true>(G, S.pdir, S.pslot, S.npred, S.c0, v, S.c0[v], ww, pm, cc);
     if (lane_id() == 0) {
       S.pm[v] = pm;
       S.cc[v] = cc;
@@ -302,40 +352,109 @@ __global__ void k_ldf_seed(LdfState S, Frontier L0) {
 }
 
 // ---------------------------------------------------------------------------
-// round r, launch 1: members, their mex (short rows), D and X flags
+// The short rows picked among 64 vertices (one per lane; `pick` set on the
+// lanes whose row is taken), each by an 8-lane group: the lanes list the
+// picked lanes in LDS, then group g takes entries g, g + 8, ... and its lane
+// `sub` the slots sub, sub + 8, ... of the row.  body(src lane, sub) runs on
+// all 64 lanes for every step of 8 entries (src = -1 on lanes without one),
+// so it may shuffle.
+template <class Body>
+__device__ __forceinline__ void short_rows_by_groups(bool pick, int32_t* lst, Body body) {
+  const int lane = lane_id();
+  const unsigned long long b = __ballot(pick);
+  if (pick) lst[__popcll(b & ((1ull << lane) - 1ull))] = lane;
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  const uint32_t n = (uint32_t)__popcll(b);
+  const int grp = lane >> 3, sub = lane & 7;
+  for (uint32_t t = 0; t < n; t += 8) {
+    const bool mine = t + (uint32_t)grp < n;
+    body(mine ? lst[t + grp] : -1, sub);
+  }
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+// round r, launch 1: members, their mex (short rows), D and X flags.  A wave
+// takes 64 vertices at a time; the stale mexes of their short member rows are
+// taken by 8-lane groups (a lane per vertex would leave most lanes idle).
 __global__ __launch_bounds__(kBlock) void k_ldf_members(LdfState S, int32_t r) {
   if (S.ctl->done_round < r) return;
+  __shared__ int32_t lists[kBlock / 64][64];
+  __shared__ int32_t results[kBlock / 64][64];
   const GraphView& G = S.G;
   const uint8_t* Lf = S.F[r & 1];
   uint8_t* Cf = S.F[(r + 1) & 1];
+  const int lane = lane_id();
+  int32_t* lst = lists[threadIdx.x >> 6];
+  int32_t* res = results[threadIdx.x >> 6];
+  const int64_t gwave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
   uint32_t nl = 0, nc = 0, nx = 0;
-  for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < G.n_owned;
-       v += (int64_t)gridDim.x * blockDim.x) {
-    const bool inl = Lf[v] != 0;
-    const bool member = S.K[v] > 0 || (inl && S.cc[v] > 0);
-    Cf[v] = member ? 1 : 0;
-    uint8_t f = inl != member ? kFlD : 0;
-    nl += inl ? 1u : 0u;
-    nc += member ? 1u : 0u;
-    if (member) {
-      int32_t m = S.pm[v];
-      if (m < 0) {
-        if (G.hidx[v] < 0) {
-          int32_t cc;
-          short_pred_scan<false>(G, S.pdir, (int32_t)v, S.c0[v], m, cc);
-          S.pm[v] = m;
-        } else {
-          f |= kFlPm;   // long row: the next launch
+  for (int64_t base = gwave * 64; base < G.n_owned; base += nwaves * 64) {
+    const int64_t v = base + lane;
+    const bool valid = v < G.n_owned;
+    bool member = false, stale_short = false;
+    uint8_t f = 0;
+    int32_t m = -1, cv = 0;
+    if (valid) {
+      const bool inl = Lf[v] != 0;
+      member = S.K[v] > 0 || (inl && S.cc[v] > 0);
+      Cf[v] = member ? 1 : 0;
+      f = inl != member ? kFlD : 0;
+      nl += inl ? 1u : 0u;
+      nc += member ? 1u : 0u;
+      if (member) {
+        m = S.pm[v];
+        cv = S.c0[v];
+        if (m < 0) {
+          if (G.hidx[v] < 0) stale_short = true;
+          else f |= kFlPm;   // long row: the next launch
         }
       }
-      if (m >= 0 && m != S.c0[v]) {
-        S.newc[v] = m;
-        S.cc[v] = 0;   // its changed predecessors re-count it in the row pass
-        f |= kFlX;
-        ++nx;
-      }
     }
-    S.fl[v] = f;
+    short_rows_by_groups(stale_short, lst, [&](int src, int sub) {
+      const int32_t x = __shfl((int32_t)v, src < 0 ? lane : src);
+      uint32_t lo = 0u, hi = 0u;
+      if (src >= 0) {
+        // 4 slots per lane per step: their predecessor bytes and slot colours
+        // in one round trip, a hub predecessor's colour in a second
+        const uint2 rw = G.row[x];
+        for (uint32_t j0 = (uint32_t)sub; j0 < rw.y; j0 += 32) {
+          uint8_t pd[4];
+          int32_t c[4];
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const uint32_t j = j0 + 8u * q;
+            const uint64_t sl = (uint64_t)rw.x + j;
+            pd[q] = j < rw.y ? S.pdir[sl] : kPdNone;
+            c[q] = j < rw.y ? G.ccol[sl] : 0;
+          }
+#pragma unroll
+          for (int q = 0; q < 4; ++q)
+            if (pd[q] == kPdHub) c[q] = S.c0[G.col[(uint64_t)rw.x + j0 + 8u * q]];
+#pragma unroll
+          for (int q = 0; q < 4; ++q)
+            if (pd[q] != kPdNone) add_colour_bit((uint32_t)c[q], lo, hi);
+        }
+      }
+      for (int off = 1; off < 8; off <<= 1) {
+        lo |= __shfl_xor(lo, off);
+        hi |= __shfl_xor(hi, off);
+      }
+      if (src >= 0 && sub == 0) res[src] = mex64(lo, hi);
+    });
+    if (stale_short) {
+      m = res[lane];   // fewer than 64 predecessors: below 64
+      S.pm[v] = m;
+    }
+    if (member && m >= 0 && m != cv) {
+      S.newc[v] = m;
+      S.cc[v] = 0;   // its changed predecessors re-count it in the row pass
+      f |= kFlX;
+      ++nx;
+    }
+    if (valid) S.fl[v] = f;
   }
   ldf_sums(S, r & 1, nl, nc, nx);
 }
@@ -351,7 +470,7 @@ __global__ __launch_bounds__(kBlock) void k_ldf_long_mex(LdfState S, int32_t r) 
   auto stale = [&](int32_t v) { return (S.fl[v] & kFlPm) != 0; };
   each_long_row<false>(S, stale, [&](int32_t v) {
     int32_t m, cc;
-    wave_pred_scan<false>(G, S.pdir, v, S.c0[v], ww, m, cc);
+    wave_pred_scan<false>(G, S.pdir, S.pslot, S.npred, S.c0, v, S.c0[v], ww, m, cc);
     if (lane_id() == 0) {
       S.pm[v] = m;
       if (m != S.c0[v]) {
@@ -398,6 +517,8 @@ __device__ __forceinline__ void ldf_row_part(const LdfState& S, int32_t x, uint8
                                              uint32_t hi, uint32_t stride) {
   const GraphView& G = S.G;
   const bool xc = (f & kFlX) != 0;
+  // a large hub's successors read its colour from the colour array (kPdHub)
+  const bool slots = xc && G.cap[x] < kLazyCap;
   for (uint32_t j0 = lo; j0 < hi; j0 += 4 * stride) {
     int32_t y[4], t[4];
 #pragma unroll
@@ -405,7 +526,7 @@ __device__ __forceinline__ void ldf_row_part(const LdfState& S, int32_t x, uint8
       const uint32_t j = j0 + q * stride;
       // successors only: live slots whose neighbour does not precede x
       y[q] = j < hi && S.pdir[rbase + j] == 0 ? G.col[rbase + j] : -1;
-      t[q] = xc && y[q] >= 0 ? G.twin[rbase + j] : 0;
+      t[q] = slots && y[q] >= 0 ? G.twin[rbase + j] : 0;
     }
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
@@ -421,7 +542,7 @@ __device__ __forceinline__ void ldf_row_part(const LdfState& S, int32_t x, uint8
         }
         if (dlt) atomicAdd(&S.cc[y[q]], dlt);
         S.pm[y[q]] = -1;
-        G.ccol[(uint64_t)G.row[y[q]].x + (uint32_t)t[q]] = b;
+        if (slots) G.ccol[(uint64_t)G.row[y[q]].x + (uint32_t)t[q]] = b;
       }
     }
   }
@@ -440,10 +561,21 @@ __global__ __launch_bounds__(kBlock) void k_ldf_rows(LdfState S, int32_t r) {
     const int32_t b = (f & kFlX) ? S.newc[x] : a;
     ldf_row_part(S, x, f, Cf[x] ? 1 : -1, a, b, rw.x, lo, hi, stride);
   };
-  for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < G.n_owned;
-       v += (int64_t)gridDim.x * blockDim.x) {
-    if (!(S.fl[v] & (kFlD | kFlX)) || G.hidx[v] >= 0) continue;
-    pass((int32_t)v, 0u, 0xffffffffu, 1u);
+  {
+    // short rows of D and X: per wave 64 vertices at a time, the moved ones
+    // taken by 8-lane groups
+    __shared__ int32_t lists[kBlock / 64][64];
+    int32_t* lst = lists[threadIdx.x >> 6];
+    const int lane = lane_id();
+    const int64_t gwave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    for (int64_t base = gwave * 64; base < G.n_owned; base += nwaves * 64) {
+      const int64_t v = base + lane;
+      const bool pick = v < G.n_owned && (S.fl[v] & (kFlD | kFlX)) && G.hidx[v] < 0;
+      short_rows_by_groups(pick, lst, [&](int src, int sub) {
+        if (src >= 0) pass((int32_t)(base + src), (uint32_t)sub, 0xffffffffu, 8u);
+      });
+    }
   }
   auto moved = [&](int32_t v) { return (S.fl[v] & (kFlD | kFlX)) != 0; };
   each_long_row<true>(S, moved, [&](int32_t v) { pass(v, (uint32_t)lane_id(), 0xffffffffu, 64u); });
@@ -463,11 +595,12 @@ __global__ __launch_bounds__(kBlock) void k_ldf_settle(LdfState S) {
     const uint2 rw = G.row[x];
     const uint32_t hi = min(rw.y, hi_cap);
     const int32_t a = S.corig[x], b = S.c0[x];
+    const bool hub = G.cap[x] >= kLazyCap;   // wrote no slot colours in the loop
     for (uint32_t j = lo; j < hi; j += stride) {
       const uint64_t sl = (uint64_t)rw.x + j;
       const int32_t y = G.col[sl];
       if (y < 0) continue;
-      if (S.pdir[sl]) nbr_colour<true>(G, y, G.twin[sl], b, a);
+      if (hub || S.pdir[sl] != kPdNone) nbr_colour<true>(G, y, G.twin[sl], b, a);
       else nbr_colour<false>(G, y, 0, b, a);
     }
   };
@@ -538,7 +671,7 @@ struct LdfHost {
   int32_t trace_cap = 0;
   LdfCtl* hctl = nullptr;
   uint8_t* flags = nullptr;   // F[0], F[1], fl
-  uint64_t pdir_cap = 0;      // slots pdir holds
+  uint64_t pdir_cap = 0;      // slots pdir and pslot hold
   hipEvent_t ev[2] = {nullptr, nullptr};   // around a block of rounds (profiling)
 };
 
@@ -551,7 +684,7 @@ void ldf_free(Graph* g) {
     if (p) (void)hipFree(p);
   };
   f(s->S.K); f(s->S.cc); f(s->S.pm); f(s->S.newc); f(s->S.corig); f(s->flags); f(s->S.acc); f(s->S.trace);
-  f(s->S.ctl); f(s->S.pdir); f(s->S.terms);
+  f(s->S.ctl); f(s->S.pdir); f(s->S.pslot); f(s->S.npred); f(s->S.terms);
   if (s->hctl) (void)hipHostFree(s->hctl);
   for (hipEvent_t e : s->ev)
     if (e) (void)hipEventDestroy(e);
@@ -567,6 +700,7 @@ static int ldf_alloc(Graph* g, LdfHost* s) {
   DC_HIP(hipMalloc(&S.pm, sizeof(int32_t) * n));
   DC_HIP(hipMalloc(&S.newc, sizeof(int32_t) * n));
   DC_HIP(hipMalloc(&S.corig, sizeof(int32_t) * n));
+  DC_HIP(hipMalloc(&S.npred, sizeof(int32_t) * n));
   DC_HIP(hipMalloc(&s->flags, 3 * n));
   S.F[0] = s->flags;
   S.F[1] = s->flags + n;
@@ -625,9 +759,12 @@ int loop_ldf(Graph* g, Frontier L0, LoopResult* res) {
   S.n_hch = g->n_hch;
   if (s->pdir_cap < g->total_cap) {
     if (S.pdir) DC_HIP(hipFree(S.pdir));
+    if (S.pslot) DC_HIP(hipFree(S.pslot));
     S.pdir = nullptr;
+    S.pslot = nullptr;
     s->pdir_cap = 0;
     DC_HIP(hipMalloc(&S.pdir, g->total_cap));
+    DC_HIP(hipMalloc(&S.pslot, sizeof(uint32_t) * g->total_cap));
     s->pdir_cap = g->total_cap;
   }
   const size_t n = (size_t)g->n + 1;
