@@ -98,3 +98,22 @@ def test_set_tuning_rejects_out_of_range():
             g.set_tuning(**kw)
         assert e.value.code == d.ERR_INVALID
     assert g.get_tuning().loop_blocks == 2048
+
+
+def test_tuner_sweep_keeps_results(oracle_mod):
+    # a small sweep on a live graph: it returns a choice from the candidates
+    # and every trial's batch + inverse leaves a valid colouring; the tuned
+    # graph then colours the stream exactly as the oracle does
+    from dyncolor_amd import tune
+    say("tuner sweep")
+    n, u, v, batches = _stream(13, 3, 2000)
+    g = d.Graph(n, u, v)
+    g.static_color(0)
+    trial = d.gen_batches(n, u, v, 1, 1500, 0.5, 99)
+    best, sweep = tune.tune(g, trial, 0, reps=1, params=["light_c", "list_c"])
+    assert best["light_c"] in tune.CANDIDATES["light_c"]
+    assert best["list_c"] in tune.CANDIDATES["list_c"]
+    assert len(sweep) == len(tune.CANDIDATES["light_c"]) + len(tune.CANDIDATES["list_c"]) - 2
+    assert g.validate()[0] == 0 and g.check_consistency() == 0
+    params = {k: val for k, val in best.items() if k != "driver"}
+    run_both(oracle_mod, n, u, v, batches, 0, tuning=params)
