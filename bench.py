@@ -64,28 +64,34 @@ def base_record(value, elapsed_steps, steps, warmup, config):
     }
 
 
-def traffic_for(kernel):
-    """HBM bytes per launch of the kernel from the committed rocprofv3 --pmc
-    summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel factor), or
-    None when no summary covers it."""
+def traffic_for(kernel, key="bytes_per_launch"):
+    """HBM bytes per launch of the kernel (or per executed loop round: key
+    "bytes_per_round") from the committed rocprofv3 --pmc summary (FETCH_SIZE
+    + WRITE_SIZE, scaled by the copy-kernel factor); kernel may be a list,
+    whose figures are summed.  None when the summary does not cover them."""
     path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
     try:
         with open(path) as f:
             t = json.load(f)
     except (OSError, ValueError):
         return None
-    ent = t.get(kernel.split(" ")[0])
-    return None if ent is None else ent.get("bytes_per_launch")
+    tot = 0
+    for k in ([kernel] if isinstance(kernel, str) else kernel):
+        ent = t.get(k.split(" ")[0])
+        if ent is None or ent.get(key) is None:
+            return None
+        tot += ent[key]
+    return tot
 
 
-def roofline(kernel, algo_bytes, kernel_ms, launches):
+def roofline(kernel, algo_bytes, kernel_ms, launches, traffic_kernels=None, traffic_key="bytes_per_launch"):
     """Roofline of one kernel: achieved = its algorithmic bytes per launch
     over its mean launch duration (hipEvents on its stream); traffic = the
     measured HBM bytes per launch, both per launch."""
     per_launch = algo_bytes / max(launches, 1)
     avg_s = kernel_ms / 1e3 / max(launches, 1)
     achieved = per_launch / avg_s / 1e9 if avg_s > 0 else 0.0
-    traffic = traffic_for(kernel)
+    traffic = traffic_for(traffic_kernels or kernel, traffic_key)
     out = {"bound": "hbm", "kernel": kernel, "achieved": achieved, "peak": HBM_PEAK_GBS,
            "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
            "basis": "achieved and frac: the kernel's algorithmic bytes per launch (row slots "
@@ -150,12 +156,13 @@ def run_stream(args, d, rule, n, u, v, workload):
     dev_batches = [g.upload_batch(*b) for b in batches]
     g.set_profiling(False)
     set_driver(g, d, args, args.stream_key)
-    prof = {"ms": 0.0, "bytes": 0, "launches": 0, "kms": 0.0, "rbytes": 0, "rl": 0}
+    prof = {"ms": 0.0, "ams": 0.0, "bytes": 0, "launches": 0, "kms": 0.0, "rbytes": 0, "rl": 0}
     terms = {"updates": 0, "del_row_slots": 0, "lc_vertices": 0, "lc_slots": 0, "loop_rounds": 0,
              "sum_C": 0, "batches": 0, "kernel_ms": 0.0}
 
     def add_prof(st, gr):
         prof["ms"] += st.ms_count_pass
+        prof["ams"] += st.ms_round_a
         prof["bytes"] += st.count_pass_bytes
         prof["launches"] += st.count_pass_launches
         prof["kms"] += st.ms_recolor_kernel
@@ -256,8 +263,27 @@ def run_stream(args, d, rule, n, u, v, workload):
         "monochromatic": int(mono), "tuning": args.tuned.get(args.stream_key) or "default"})
     out["latency_ms"] = {"p50": pct(lat_ms, 50), "p99": pct(lat_ms, 99), "max": float(lat_ms.max())}
     if args.driver != "fused" and rule == d.RULE_INDEPENDENT and prof["launches"]:
-        out["roofline"] = roofline("k_round_c (count pass)", prof["bytes"], prof["ms"],
-                                   prof["launches"])
+        # the unit is one loop round: phase A + count pass, which together do
+        # the reference's findEligibleNeighbors + recolorNeighbor pair of the
+        # round; its algorithmic bytes are the reference's for that round
+        rb = (16 * terms["lc_vertices"] + 8 * terms["lc_slots"] + terms["loop_rounds"] * (n / 8.0)
+              + 4 * terms["sum_C"])
+        out["roofline"] = roofline("loop round (k_round_a + k_round_c)", rb,
+                                   prof["ams"] + prof["ms"], prof["launches"],
+                                   ["k_round_a", "k_round_c"], "bytes_per_round")
+        out["roofline"]["basis"] = (
+            "achieved and frac: the reference's bytes per loop round (16 per vertex of L_r u C_r: "
+            "list entry, row bounds, colour and prev; 8 per row slot of L_r u C_r: neighbour and its "
+            "colour; n/8 flag compaction; 4 per member of C_r) / mean round time (k_round_a + "
+            "k_round_c, hipEvents around each launch); traffic: rocprofv3 FETCH_SIZE + WRITE_SIZE "
+            "of both kernels per executed round")
+        out["roofline"]["rounds_by_terms"] = terms["loop_rounds"]
+        out["roofline"]["round_us"] = {"k_round_a": prof["ams"] * 1e3 / prof["launches"],
+                                       "k_round_c": prof["ms"] * 1e3 / prof["launches"]}
+        # the count pass alone against this build's own bytes (row slots it
+        # streams, rows visited, state and list words)
+        out["roofline"]["count_pass_own_model"] = roofline(
+            "k_round_c (count pass)", prof["bytes"], prof["ms"], prof["launches"])
     elif args.driver != "fused" and prof["launches"]:
         # ldf rule: the loop's rounds as the unit (four launches each); bytes are
         # the reference's for those rounds (it rescans every row of L_r u C_r)

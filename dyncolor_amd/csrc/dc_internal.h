@@ -244,6 +244,7 @@ struct LoopResult {
   double kernel_ms = 0.0;
   int64_t launches = 0;
   double count_ms = 0.0;        // summed count-pass launch durations (profiling on)
+  double round_a_ms = 0.0;      // summed phase-A launch durations (profiling on)
   int64_t count_launches = 0;
   uint64_t count_bytes = 0;     // algorithmic bytes of those launches
   bool settled = false;         // stopped at settled colours (see loop_hi)

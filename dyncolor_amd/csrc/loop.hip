@@ -69,6 +69,7 @@
 // Outside the loop K is zero everywhere and so are the state arrays.
 #include <hip/hip_runtime.h>
 #include <hip/hip_cooperative_groups.h>
+#include <hip/hip_ext.h>
 
 #include <algorithm>
 #include <climits>
@@ -212,7 +213,7 @@ struct GraphBlock {
 
 struct IncState {
   LoopState S{};
-  std::vector<hipEvent_t> cev;   // profiling: start/stop around each count-pass launch
+  std::vector<hipEvent_t> cev;   // profiling: start and stop of each phase-A and count-pass launch
   uint32_t* bits = nullptr;   // St[0], St[1]
   int32_t trace_cap = 0;
   LoopCtl* hctl = nullptr;
@@ -1660,7 +1661,7 @@ int loop_hi(Graph* g, Frontier L0, int32_t nl0, bool persistent, LoopResult* res
     light_grid = std::min(kMaxBlocks, std::max(32, 2 * cus));
   }
   int round_grid = grid;
-  double kms = 0.0, cms = 0.0;
+  double kms = 0.0, cms = 0.0, ams = 0.0;
   int64_t claunches = 0;
   hipEvent_t e0 = g->ev[2], e1 = g->ev[3];
   // settle checks: at most three per batch, after a round without a change,
@@ -1675,7 +1676,7 @@ int loop_hi(Graph* g, Frontier L0, int32_t nl0, bool persistent, LoopResult* res
         std::min(persistent ? block_end_persistent(r) : block_end_plain(r), max_rounds) - r;
     if ((rc = inc_trace(g, s, r + nb))) return rc;
     if (g->profiling) {
-      while (s->cev.size() < 2 * (size_t)nb) {
+      while (s->cev.size() < 4 * (size_t)nb) {
         hipEvent_t e;
         DC_HIP(hipEventCreate(&e));
         s->cev.push_back(e);
@@ -1686,10 +1687,19 @@ int loop_hi(Graph* g, Frontier L0, int32_t nl0, bool persistent, LoopResult* res
     // after the check rounds the queued settle (and part) checks
     auto enqueue_block = [&](bool events) -> int {
       for (int32_t i = 0; i < nb; ++i) {
-        hipLaunchKernelGGL(k_round_a, dim3(round_grid), blk, 0, g->stream, S, r + i);
-        if (events) DC_HIP(hipEventRecord(s->cev[2 * i], g->stream));
-        hipLaunchKernelGGL(k_round_c, dim3(round_grid), blk, 0, g->stream, S, r + i);
-        if (events) DC_HIP(hipEventRecord(s->cev[2 * i + 1], g->stream));
+        const int32_t ri = r + i;
+        if (events) {
+          // events taken at the launches' own start and end (an event
+          // recorded before a launch would also time the host's submission
+          // at the start of a block)
+          hipExtLaunchKernelGGL(k_round_a, dim3(round_grid), blk, 0, g->stream, s->cev[4 * i],
+                                s->cev[4 * i + 1], 0u, S, ri);
+          hipExtLaunchKernelGGL(k_round_c, dim3(round_grid), blk, 0, g->stream, s->cev[4 * i + 2],
+                                s->cev[4 * i + 3], 0u, S, ri);
+        } else {
+          hipLaunchKernelGGL(k_round_a, dim3(round_grid), blk, 0, g->stream, S, ri);
+          hipLaunchKernelGGL(k_round_c, dim3(round_grid), blk, 0, g->stream, S, ri);
+        }
         if (settle_point(r + i + 1) && !g->opt.exact_rounds) {
           hipLaunchKernelGGL(k_settle_check_if, dim3(tun.loop_blocks), blk, 0, g->stream, S);
           hipLaunchKernelGGL(k_settle_decide, dim3(1), dim3(64), 0, g->stream, S, r + i);
@@ -1764,9 +1774,12 @@ int loop_hi(Graph* g, Frontier L0, int32_t nl0, bool persistent, LoopResult* res
       DC_HIP(hipEventElapsedTime(&x, e0, e1));
       kms += x;
       if (!persistent) {
-        // count-pass launches of executed rounds (later ones return at once)
+        // phase-A and count-pass launches of executed rounds (later ones
+        // return at once)
         for (int32_t i = 0; i < nb && r + i <= done; ++i) {
-          DC_HIP(hipEventElapsedTime(&x, s->cev[2 * i], s->cev[2 * i + 1]));
+          DC_HIP(hipEventElapsedTime(&x, s->cev[4 * i], s->cev[4 * i + 1]));
+          ams += x;
+          DC_HIP(hipEventElapsedTime(&x, s->cev[4 * i + 2], s->cev[4 * i + 3]));
           cms += x;
           ++claunches;
         }
@@ -1848,6 +1861,7 @@ int loop_hi(Graph* g, Frontier L0, int32_t nl0, bool persistent, LoopResult* res
   // device); per count-pass slot col + ccol (8 B); per pulled slot col (4 B);
   // per row or chunk visit its bounds and count or saved state (16 B).
   res->count_ms = cms;
+  res->round_a_ms = ams;
   res->count_launches = claunches;
   res->count_bytes = 4ull * s->hstat[4 * kMaxBlocks] + 8ull * cnt + 4ull * pull + 16ull * rowv;
   return DC_OK;

@@ -510,6 +510,7 @@ int recolor_plain(Graph* g, int rule, dc_stats* st, bool persistent_loop) {
       st->ms_recolor_kernel = kernel_ms + lr.kernel_ms;
       st->recolor_launches = launches;
       st->ms_count_pass = lr.count_ms;
+      st->ms_round_a = lr.round_a_ms;
       st->count_pass_bytes = (int64_t)lr.count_bytes;
       st->count_pass_launches = (int32_t)lr.count_launches;
     }

@@ -130,6 +130,9 @@ typedef struct {
   int64_t lc_vertices;      /* sum over loop rounds of |L_r u C_r|            */
   int64_t lc_slots;         /* sum over loop rounds of the row slots of L_r u C_r */
   int64_t del_row_slots;    /* sum over deletion lines of both rows' slots    */
+  double ms_round_a;        /* summed duration of the independent-rule loop's
+                               phase-A launches (profiling on, plain driver);
+                               with ms_count_pass: the loop rounds' kernel time */
 } dc_stats;
 
 /* Work-split parameters of the independent-rule loop and the batch apply
