@@ -563,6 +563,26 @@ __device__ __forceinline__ int32_t group_small_mex(const GraphView& G, uint2 r, 
   return __ffsll((long long)~mask) - 1;
 }
 
+// mex of a row with fewer than kSmallLen slots by one lane: its slots 16 at a
+// time, all loads in flight, colours 0..63 kept as two 32-bit words (the mex
+// of fewer than 64 slots is below 64)
+__device__ __forceinline__ int32_t lane_small_mex(const GraphView& G, uint2 r) {
+  uint32_t lo = 0u, hi = 0u;
+  for (uint32_t j = 0; j < r.y; j += 16) {
+    int32_t c[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) c[k] = j + k < r.y ? G.ccol[(uint64_t)r.x + j + k] : -1;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      const int32_t x = c[k];
+      if (x >= 0 && x < 32) lo |= 1u << x;
+      if (x >= 32 && x < 64) hi |= 1u << (x - 32);
+    }
+  }
+  if (lo != 0xffffffffu) return __ffs(~lo) - 1;
+  return hi != 0xffffffffu ? 32 + __ffs(~hi) - 1 : 64;
+}
+
 // mex from a colour histogram by an 8-lane group: 256 counts per step (lane
 // `sub` reads 32 of them); -1 when colours 0..kHistW-1 are all present
 __device__ __forceinline__ int32_t group_hist_mex(const uint32_t* hrow, int sub) {
@@ -608,9 +628,7 @@ __device__ __forceinline__ void round_a(const LoopState& S, int r, LoopLds& lds)
   // colour changes of this round by part (parts set)
   uint32_t* pch = S.nparts ? S.pchg + (size_t)(r & 1) * S.nparts : nullptr;
   // a row put into D: its slots, and which count-pass path takes it
-  auto d_row = [&](int32_t w) {
-    const uint32_t len = G.row[w].y;
-    const int32_t h = G.hidx[w];
+  auto d_row = [&](int32_t w, uint32_t len, int32_t h) {
     const bool hub = len >= kMidLen;
     const bool mid = !hub && h >= 0;
     dsum += len;
@@ -629,14 +647,17 @@ __device__ __forceinline__ void round_a(const LoopState& S, int r, LoopLds& lds)
   };
 
   auto visit = [&](int32_t* lst, uint32_t total) {
-    // members with a cached mex commit at once; the others (stale mex) are
-    // compacted to the front of the list and evaluated by 8-lane groups
+    // members with a cached mex commit at once, and so do those with a stale
+    // mex and a row without a histogram (each lane scans its own row); the
+    // others (stale mex, histogram rows) are compacted to the front of the
+    // list and evaluated by 8-lane groups
     uint32_t nd = 0;
     for (uint32_t base = 0; base < total; base += 64) {
       const uint32_t i = base + lane;
       const bool valid = i < total;
       const int32_t w = valid ? lst[i] : 0;
-      int32_t k = 0, cm = 0, so = 0, po = 0;
+      int32_t k = 0, cm = 0, so = 0, po = 0, h = -1;
+      uint2 rw = make_uint2(0u, 0u);
       bool inl = false;
       if (valid) {
         k = S.K[w];
@@ -644,12 +665,14 @@ __device__ __forceinline__ void round_a(const LoopState& S, int r, LoopLds& lds)
         so = S.col0[w];
         po = S.prev[w];
         inl = st_has(sp, w, kFC);
+        h = G.hidx[w];
+        rw = G.row[w];
       }
       const bool member = valid && k > 0;
       const bool leaving = valid && !member && inl;
       if (S.terms && (member || inl)) {
         lc_n += 1;
-        lc_deg += G.row[w].y;
+        lc_deg += rw.y;
       }
       // w gets its first flag of the round (C or D) in this phase
       if (listing)
@@ -659,10 +682,15 @@ __device__ __forceinline__ void round_a(const LoopState& S, int r, LoopLds& lds)
         S.dold[w] = make_int2(so, (po + 1) | (int32_t)0x80000000);
         st_set(sn, w, kFD);
       }
-      if (member && cm >= 0) dv = commit_member(S, sn, w, cm, so, po, inl, cm, changed, pch);
-      if (dv) d_row(w);
+      int32_t mm = cm;
+      if (member && cm < 0 && h < 0) {
+        mm = lane_small_mex(G, rw);
+        slots += rw.y;
+      }
+      if (member && mm >= 0) dv = commit_member(S, sn, w, cm, so, po, inl, mm, changed, pch);
+      if (dv) d_row(w, rw.y, h);
       members += member ? 1u : 0u;
-      const bool defer = member && cm < 0;
+      const bool defer = member && mm < 0;
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
       __builtin_amdgcn_wave_barrier();
       const unsigned long long dm = __ballot(defer);
@@ -671,8 +699,9 @@ __device__ __forceinline__ void round_a(const LoopState& S, int r, LoopLds& lds)
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
       __builtin_amdgcn_wave_barrier();
     }
-    // stale mex: a short row's slots or the row's colour histogram; a mex
-    // past the histogram goes to the front of the list again (nw)
+    // stale mex from the row's colour histogram (a short row's slots if one
+    // is here); a mex past the histogram goes to the front of the list again
+    // (nw)
     const int grp = lane >> 3, sub = lane & 7;
     uint32_t nw = 0;
     for (uint32_t b8 = 0; b8 < nd; b8 += 8) {
@@ -693,7 +722,7 @@ __device__ __forceinline__ void round_a(const LoopState& S, int r, LoopLds& lds)
       const bool wst = valid && m < 0;
       if (valid && !wst && sub == 0 &&
           commit_member(S, sn, w, -1, S.col0[w], S.prev[w], st_has(sp, w, kFC), m, changed, pch))
-        d_row(w);
+        d_row(w, G.row[w].y, G.hidx[w]);
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
       __builtin_amdgcn_wave_barrier();
       const unsigned long long wm = __ballot(wst && sub == 0);
@@ -708,7 +737,7 @@ __device__ __forceinline__ void round_a(const LoopState& S, int r, LoopLds& lds)
       const int32_t m = wave_row_mex<M_LOOP_HI>(RA, w, win);
       if (lane == 0 &&
           commit_member(S, sn, w, -1, S.col0[w], S.prev[w], st_has(sp, w, kFC), m, changed, pch))
-        d_row(w);
+        d_row(w, G.row[w].y, G.hidx[w]);
     }
   };
   // the previous round's touched list when it has one, else the state array

@@ -113,7 +113,14 @@ def test_tuner_sweep_keeps_results(oracle_mod):
     best, sweep = tune.tune(g, trial, 0, reps=1, params=["light_c", "list_c"])
     assert best["light_c"] in tune.CANDIDATES["light_c"]
     assert best["list_c"] in tune.CANDIDATES["list_c"]
-    assert len(sweep) == len(tune.CANDIDATES["light_c"]) + len(tune.CANDIDATES["list_c"]) - 2
+    # each candidate is tried at most once, and every one but the incumbent
+    # of the moment (a taken candidate makes the old default a candidate)
+    for name in ("light_c", "list_c"):
+        tried = [e["value"] for e in sweep if e["param"] == name]
+        assert len(tried) == len(set(tried))
+        assert set(tried) <= set(tune.CANDIDATES[name])
+        assert len(tune.CANDIDATES[name]) - 1 <= len(tried) <= len(tune.CANDIDATES[name])
+    assert {e["param"] for e in sweep} <= {"light_c", "list_c"}
     assert g.validate()[0] == 0 and g.check_consistency() == 0
     params = {k: val for k, val in best.items() if k != "driver"}
     run_both(oracle_mod, n, u, v, batches, 0, tuning=params)
