@@ -507,7 +507,13 @@ __device__ __forceinline__ void nbr_colour(const GraphView& G, int32_t w, int32_
 
 // One lane's share of the row pass of x (slots lo, lo + stride, ...), over
 // x's successors y only: D adds x's +-1 to K[y]; X (colour a -> b) moves
-// cc[y], drops y's predecessor mex and writes b into x's slot in y's row.
+// cc[y], drops y's predecessor mex when the change can move it, and writes b
+// into x's slot in y's row.  A cached mex m of y's predecessors' colours
+// moves only if a < m (a may have been the only predecessor holding a colour
+// below m) or b == m; a > m leaves every colour below m present, and b != m
+// adds no colour that was missing at m.  When several predecessors of y
+// change in one round, each tests the same round-start m, so m is dropped if
+// any of their changes can move it.
 // The loop's scans read predecessors' slots only, and no histogram or
 // all-neighbour mex: x's slots in its predecessors' rows, and its
 // neighbours' histograms and all-neighbour mex caches, take x's final colour
@@ -528,21 +534,34 @@ __device__ __forceinline__ void ldf_row_part(const LdfState& S, int32_t x, uint8
       y[q] = j < hi && S.pdir[rbase + j] == 0 ? G.col[rbase + j] : -1;
       t[q] = slots && y[q] >= 0 ? G.twin[rbase + j] : 0;
     }
+    // the successors' flags, colours and cached mexes, all in flight at once
+    uint8_t fy[4];
+    int32_t cy[4], ny[4], my[4];
+    uint32_t ry[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      fy[q] = 0;
+      cy[q] = ny[q] = my[q] = -1;
+      ry[q] = 0u;
+      if (xc && y[q] >= 0) {
+        fy[q] = S.fl[y[q]];
+        cy[q] = S.c0[y[q]];
+        ny[q] = S.newc[y[q]];
+        my[q] = S.pm[y[q]];
+        if (slots) ry[q] = G.row[y[q]].x;
+      }
+    }
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       if (y[q] < 0) continue;
       if (f & kFlD) atomicAdd(&S.K[y[q]], ksign);
       if (xc) {
-        const uint8_t fy = S.fl[y[q]];
-        int32_t dlt;
-        if (fy & kFlX) dlt = b == S.newc[y[q]] ? 1 : 0;
-        else {
-          const int32_t cy = S.c0[y[q]];
-          dlt = (b == cy ? 1 : 0) - (a == cy ? 1 : 0);
-        }
+        // newc[y] is y's new colour only when y changes this round too
+        const int32_t dlt = (fy[q] & kFlX) ? (b == ny[q] ? 1 : 0)
+                                          : (b == cy[q] ? 1 : 0) - (a == cy[q] ? 1 : 0);
         if (dlt) atomicAdd(&S.cc[y[q]], dlt);
-        S.pm[y[q]] = -1;
-        if (slots) G.ccol[(uint64_t)G.row[y[q]].x + (uint32_t)t[q]] = b;
+        if (my[q] >= 0 && (a < my[q] || b == my[q])) S.pm[y[q]] = -1;
+        if (slots) G.ccol[(uint64_t)ry[q] + (uint32_t)t[q]] = b;
       }
     }
   }
