@@ -24,13 +24,15 @@
 // appends on shared counters): members and their mex (short rows by their
 // own thread), stale mex of long rows (a wave per row of the long-row
 // table), the row passes of D and X over their successors (K, cc, cached
-// mexes, the changed colour into the successors' slot colour caches), commit.
-// The batch's set-up records which slots hold a predecessor (pdir), so the
-// scans and row passes read no priorities.  The loop's scans read only
-// predecessors' slots, so the changed vertices' slots in their predecessors'
-// rows, and their neighbours' histograms and all-neighbour mex caches (the
-// independent rule's), take the final colours once after the loop
-// (k_ldf_settle).
+// mexes), commit.  A vertex's colour, its new colour in the round, its cached
+// mex and cc sit together in one 16-byte record, so the row pass gathers
+// one record per successor.  The batch's set-up records which slots hold a
+// predecessor (pdir, and for long rows a list of them), so the scans and row
+// passes read no priorities; the scans read a predecessor's colour from the
+// colour array, so a colour change writes into no slot during the loop: the
+// changed vertices' slots in their neighbours' rows, and their neighbours'
+// histograms and all-neighbour mex caches (the independent rule's), take the
+// final colours once after the loop (k_ldf_settle).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -45,12 +47,23 @@ constexpr int kLdfBlocks = 2048;
 constexpr int kLdfAcc = 32;        // round sums: workgroup b adds into slot b % kLdfAcc
 constexpr int kWin32 = kWinBits / 32;   // 32-bit words of a wave's mex window
 constexpr uint8_t kFlD = 1, kFlX = 2, kFlPm = 4;   // per-vertex flags of a round
-// pdir per slot: the neighbour does not precede the row's vertex, precedes it,
-// or precedes it and is a large hub (cap >= kLazyCap: its colour is read from
-// the colour array, not from the slot colour cache, during the loop, so its
-// colour changes write into none of its many successors' rows)
+// pdir per slot: the neighbour does not precede the row's vertex (kPdNone),
+// or precedes it (kPdHub: its colour is read from the colour array during the
+// loop, not from the slot colour cache, so a colour change writes into none
+// of its successors' rows).  kPdPred (read from the slot) is kept for
+// predecessors with cap < kLazyCap; with kLazyCap 0 there are none (measured
+// on R-MAT 22: 5 % faster than reading predecessors below kMidLen from their
+// slots, which costs the row pass a slot write per successor).
 constexpr uint8_t kPdNone = 0, kPdPred = 1, kPdHub = 2;
-constexpr uint32_t kLazyCap = kMidLen;
+constexpr uint32_t kLazyCap = 0;
+
+// Per-vertex state of the loop, gathered as one 16-byte record: the colour
+// (a copy of the colour array, kept by the commit), the new colour of a
+// vertex that changes in the round (-1 otherwise, rewritten every round),
+// the cached predecessor mex (-1 stale) and cc.
+struct alignas(16) LdfV {
+  int32_t c, nc, pm, cc;
+};
 
 struct LdfCtl {
   int32_t done_round;   // the last round run (INT_MAX while running)
@@ -66,9 +79,7 @@ struct LdfState {
   int32_t* c0;          // colour buffers (both written at commit: they agree)
   int32_t* c1;
   int32_t* K;
-  int32_t* cc;
-  int32_t* pm;
-  int32_t* newc;
+  LdfV* V;
   int32_t* corig;       // colour at the loop's start of every vertex that changed (-1: none)
   uint8_t* pdir;        // slot -> its neighbour precedes the row's vertex (set up per batch)
   uint32_t* pslot;      // rows with cap >= kSmallLen: their predecessors, listed from the
@@ -315,18 +326,16 @@ __global__ __launch_bounds__(kBlock) void k_ldf_init_cc(LdfState S) {
        v += (int64_t)gridDim.x * blockDim.x) {
     if (G.hidx[v] >= 0) continue;
     int32_t pm, cc;
-    short_pred_scan<true>(G, S.pdir, S.c0, (int32_t)v, S.c0[v], pm, cc);
-    S.pm[v] = pm;
-    S.cc[v] = cc;
+    const int32_t cv = S.c0[v];
+    short_pred_scan<true>(G, S.pdir, S.c0, (int32_t)v, cv, pm, cc);
+    S.V[v] = LdfV{cv, -1, pm, cc};
   }
   uint32_t* ww = win[threadIdx.x >> 6];
   each_long_row<false>(S, [](int32_t) { return true; }, [&](int32_t v) {
     int32_t pm, cc;
-    wave_pred_scan<true>(G, S.pdir, S.pslot, S.npred, S.c0, v, S.c0[v], ww, pm, cc);
-    if (lane_id() == 0) {
-      S.pm[v] = pm;
-      S.cc[v] = cc;
-    }
+    const int32_t cv = S.c0[v];
+    wave_pred_scan<true>(G, S.pdir, S.pslot, S.npred, S.c0, v, cv, ww, pm, cc);
+    if (lane_id() == 0) S.V[v] = LdfV{cv, -1, pm, cc};
   });
 }
 
@@ -397,16 +406,20 @@ __global__ __launch_bounds__(kBlock) void k_ldf_members(LdfState S, int32_t r) {
     bool member = false, stale_short = false;
     uint8_t f = 0;
     int32_t m = -1, cv = 0;
+    LdfV rec{0, -1, -1, 0};
+    int32_t nc0 = -1;
     if (valid) {
+      rec = S.V[v];
+      nc0 = rec.nc;
       const bool inl = Lf[v] != 0;
-      member = S.K[v] > 0 || (inl && S.cc[v] > 0);
+      member = S.K[v] > 0 || (inl && rec.cc > 0);
       Cf[v] = member ? 1 : 0;
       f = inl != member ? kFlD : 0;
       nl += inl ? 1u : 0u;
       nc += member ? 1u : 0u;
       if (member) {
-        m = S.pm[v];
-        cv = S.c0[v];
+        m = rec.pm;
+        cv = rec.c;
         if (m < 0) {
           if (G.hidx[v] < 0) stale_short = true;
           else f |= kFlPm;   // long row: the next launch
@@ -422,17 +435,19 @@ __global__ __launch_bounds__(kBlock) void k_ldf_members(LdfState S, int32_t r) {
         const uint2 rw = G.row[x];
         for (uint32_t j0 = (uint32_t)sub; j0 < rw.y; j0 += 32) {
           uint8_t pd[4];
-          int32_t c[4];
+          int32_t c[4], w[4];
 #pragma unroll
           for (int q = 0; q < 4; ++q) {
             const uint32_t j = j0 + 8u * q;
             const uint64_t sl = (uint64_t)rw.x + j;
             pd[q] = j < rw.y ? S.pdir[sl] : kPdNone;
-            c[q] = j < rw.y ? G.ccol[sl] : 0;
+            w[q] = j < rw.y ? G.col[sl] : 0;
           }
+          // a predecessor's colour from the colour array (kPdHub) or its slot
 #pragma unroll
           for (int q = 0; q < 4; ++q)
-            if (pd[q] == kPdHub) c[q] = S.c0[G.col[(uint64_t)rw.x + j0 + 8u * q]];
+            c[q] = pd[q] == kPdHub ? S.c0[w[q]]
+                 : pd[q] == kPdPred ? G.ccol[(uint64_t)rw.x + j0 + 8u * q] : 0;
 #pragma unroll
           for (int q = 0; q < 4; ++q)
             if (pd[q] != kPdNone) add_colour_bit((uint32_t)c[q], lo, hi);
@@ -446,15 +461,21 @@ __global__ __launch_bounds__(kBlock) void k_ldf_members(LdfState S, int32_t r) {
     });
     if (stale_short) {
       m = res[lane];   // fewer than 64 predecessors: below 64
-      S.pm[v] = m;
+      rec.pm = m;
     }
+    rec.nc = -1;
     if (member && m >= 0 && m != cv) {
-      S.newc[v] = m;
-      S.cc[v] = 0;   // its changed predecessors re-count it in the row pass
+      rec.nc = m;
+      rec.cc = 0;   // its changed predecessors re-count it in the row pass
       f |= kFlX;
       ++nx;
     }
-    if (valid) S.fl[v] = f;
+    if (valid) {
+      // the record is rewritten only when it moved: a new mex, or a new
+      // colour (set, or cleared after last round's)
+      if (stale_short || rec.nc != nc0) S.V[v] = rec;
+      S.fl[v] = f;
+    }
   }
   ldf_sums(S, r & 1, nl, nc, nx);
 }
@@ -470,12 +491,13 @@ __global__ __launch_bounds__(kBlock) void k_ldf_long_mex(LdfState S, int32_t r) 
   auto stale = [&](int32_t v) { return (S.fl[v] & kFlPm) != 0; };
   each_long_row<false>(S, stale, [&](int32_t v) {
     int32_t m, cc;
-    wave_pred_scan<false>(G, S.pdir, S.pslot, S.npred, S.c0, v, S.c0[v], ww, m, cc);
+    const int32_t cv = S.c0[v];
+    wave_pred_scan<false>(G, S.pdir, S.pslot, S.npred, S.c0, v, cv, ww, m, cc);
     if (lane_id() == 0) {
-      S.pm[v] = m;
-      if (m != S.c0[v]) {
-        S.newc[v] = m;
-        S.cc[v] = 0;
+      S.V[v].pm = m;
+      if (m != cv) {
+        S.V[v].nc = m;
+        S.V[v].cc = 0;
         S.fl[v] = S.fl[v] | kFlX;
         ++nx;
       }
@@ -523,7 +545,7 @@ __device__ __forceinline__ void ldf_row_part(const LdfState& S, int32_t x, uint8
                                              uint32_t hi, uint32_t stride) {
   const GraphView& G = S.G;
   const bool xc = (f & kFlX) != 0;
-  // a large hub's successors read its colour from the colour array (kPdHub)
+  // successors read x's colour from its slot only when x is below kLazyCap
   const bool slots = xc && G.cap[x] < kLazyCap;
   for (uint32_t j0 = lo; j0 < hi; j0 += 4 * stride) {
     int32_t y[4], t[4];
@@ -534,20 +556,15 @@ __device__ __forceinline__ void ldf_row_part(const LdfState& S, int32_t x, uint8
       y[q] = j < hi && S.pdir[rbase + j] == 0 ? G.col[rbase + j] : -1;
       t[q] = slots && y[q] >= 0 ? G.twin[rbase + j] : 0;
     }
-    // the successors' flags, colours and cached mexes, all in flight at once
-    uint8_t fy[4];
-    int32_t cy[4], ny[4], my[4];
+    // the successors' records, all in flight at once
+    LdfV vy[4];
     uint32_t ry[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      fy[q] = 0;
-      cy[q] = ny[q] = my[q] = -1;
+      vy[q] = LdfV{-1, -1, -1, 0};
       ry[q] = 0u;
       if (xc && y[q] >= 0) {
-        fy[q] = S.fl[y[q]];
-        cy[q] = S.c0[y[q]];
-        ny[q] = S.newc[y[q]];
-        my[q] = S.pm[y[q]];
+        vy[q] = S.V[y[q]];
         if (slots) ry[q] = G.row[y[q]].x;
       }
     }
@@ -556,11 +573,13 @@ __device__ __forceinline__ void ldf_row_part(const LdfState& S, int32_t x, uint8
       if (y[q] < 0) continue;
       if (f & kFlD) atomicAdd(&S.K[y[q]], ksign);
       if (xc) {
-        // newc[y] is y's new colour only when y changes this round too
-        const int32_t dlt = (fy[q] & kFlX) ? (b == ny[q] ? 1 : 0)
-                                          : (b == cy[q] ? 1 : 0) - (a == cy[q] ? 1 : 0);
-        if (dlt) atomicAdd(&S.cc[y[q]], dlt);
-        if (my[q] >= 0 && (a < my[q] || b == my[q])) S.pm[y[q]] = -1;
+        // y's cc counts predecessors with y's colour: its new one when y
+        // changes in this round too (nc >= 0)
+        const int32_t dlt = vy[q].nc >= 0 ? (b == vy[q].nc ? 1 : 0)
+                                          : (b == vy[q].c ? 1 : 0) - (a == vy[q].c ? 1 : 0);
+        if (dlt) atomicAdd(&S.V[y[q]].cc, dlt);
+        const int32_t my = vy[q].pm;
+        if (my >= 0 && (a < my || b == my)) S.V[y[q]].pm = -1;
         if (slots) G.ccol[(uint64_t)ry[q] + (uint32_t)t[q]] = b;
       }
     }
@@ -577,7 +596,7 @@ __global__ __launch_bounds__(kBlock) void k_ldf_rows(LdfState S, int32_t r) {
     const uint2 rw = G.row[x];
     const uint32_t hi = min(rw.y, hi_cap);
     const int32_t a = S.c0[x];
-    const int32_t b = (f & kFlX) ? S.newc[x] : a;
+    const int32_t b = (f & kFlX) ? S.V[x].nc : a;
     ldf_row_part(S, x, f, Cf[x] ? 1 : -1, a, b, rw.x, lo, hi, stride);
   };
   {
@@ -644,9 +663,11 @@ __global__ __launch_bounds__(kBlock) void k_ldf_commit(LdfState S, int32_t r) {
   for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < S.G.n_owned;
        v += (int64_t)gridDim.x * blockDim.x) {
     if (S.fl[v] & kFlX) {
+      const int32_t nc = S.V[v].nc;
       if (S.corig[v] < 0) S.corig[v] = S.c0[v];
-      S.c0[v] = S.newc[v];
-      S.c1[v] = S.newc[v];
+      S.c0[v] = nc;
+      S.c1[v] = nc;
+      S.V[v].c = nc;
     }
     if (S.prof && (S.F[0][v] | S.F[1][v])) {   // v in L_r u C_r
       ++lcv;
@@ -702,7 +723,7 @@ void ldf_free(Graph* g) {
   auto f = [](void* p) {
     if (p) (void)hipFree(p);
   };
-  f(s->S.K); f(s->S.cc); f(s->S.pm); f(s->S.newc); f(s->S.corig); f(s->flags); f(s->S.acc); f(s->S.trace);
+  f(s->S.K); f(s->S.V); f(s->S.corig); f(s->flags); f(s->S.acc); f(s->S.trace);
   f(s->S.ctl); f(s->S.pdir); f(s->S.pslot); f(s->S.npred); f(s->S.terms);
   if (s->hctl) (void)hipHostFree(s->hctl);
   for (hipEvent_t e : s->ev)
@@ -715,9 +736,7 @@ static int ldf_alloc(Graph* g, LdfHost* s) {
   const size_t n = (size_t)g->n + 1;
   LdfState& S = s->S;
   DC_HIP(hipMalloc(&S.K, sizeof(int32_t) * n));
-  DC_HIP(hipMalloc(&S.cc, sizeof(int32_t) * n));
-  DC_HIP(hipMalloc(&S.pm, sizeof(int32_t) * n));
-  DC_HIP(hipMalloc(&S.newc, sizeof(int32_t) * n));
+  DC_HIP(hipMalloc(&S.V, sizeof(LdfV) * n));
   DC_HIP(hipMalloc(&S.corig, sizeof(int32_t) * n));
   DC_HIP(hipMalloc(&S.npred, sizeof(int32_t) * n));
   DC_HIP(hipMalloc(&s->flags, 3 * n));
