@@ -524,10 +524,15 @@ __device__ __forceinline__ bool commit_member(const LoopState& S, uint32_t* sn, 
                                               int32_t cm, int32_t so, int32_t po, bool inl,
                                               int32_t m, uint32_t& changed, uint32_t* pch) {
   if (cm < 0) S.G.mexc[w] = m;
-  S.col0[w] = m;
-  S.col1[w] = m;
-  S.prev[w] = so;
+  // both colour buffers hold so and prev holds po: a member that keeps its
+  // colour (and whose prev already is its colour) stores nothing, so the
+  // many stable members of a heavy round dirty no lines
   const bool chv = m != so;
+  if (chv) {
+    S.col0[w] = m;
+    S.col1[w] = m;
+  }
+  if (po != so) S.prev[w] = so;
   changed += chv ? 1u : 0u;
   // the part changed this round (a flag: a counter atomic per change would
   // serialise on a large part's one address)
