@@ -275,7 +275,7 @@ def run_stream(args, d, rule, n, u, v, workload):
             "achieved and frac: the reference's bytes per loop round (16 per vertex of L_r u C_r: "
             "list entry, row bounds, colour and prev; 8 per row slot of L_r u C_r: neighbour and its "
             "colour; n/8 flag compaction; 4 per member of C_r) / mean round time (k_round_a + "
-            "k_round_c, hipEvents around each launch); traffic: rocprofv3 FETCH_SIZE + WRITE_SIZE "
+            "k_round_c, hipEvents attached to each launch); traffic: rocprofv3 FETCH_SIZE + WRITE_SIZE "
             "of both kernels per executed round")
         out["roofline"]["rounds_by_terms"] = terms["loop_rounds"]
         out["roofline"]["round_us"] = {"k_round_a": prof["ams"] * 1e3 / prof["launches"],
