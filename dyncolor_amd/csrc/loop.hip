@@ -870,9 +870,12 @@ __device__ __forceinline__ uint32_t count_slots(const LoopState& S, const DState
         if (to != tn) {
           atomicAdd(&S.K[y[k]], tn ? 1 : -1);
           // R_r only gains members in this phase: a flag seen set stays set.
-          // A vertex that had no flag goes on the touched list (one that had
-          // a flag when gathered has one now: no returning atomic).
-          if (tn && !(fl[k] & kFR)) {
+          // A member of C_r is a candidate of the next round already and
+          // takes no R flag (on power-law graphs thousands of rising counts
+          // hit the same hubs, which are members).  A vertex that had no
+          // flag goes on the touched list (one that had a flag when gathered
+          // has one now: no returning atomic).
+          if (tn && !(fl[k] & (kFR | kFC))) {
             if (tl.q && fl[k] == 0u)
               fresh[k] = st_set_first(st, y[k], kFR);
             else
