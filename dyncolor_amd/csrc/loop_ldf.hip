@@ -47,15 +47,13 @@ constexpr int kLdfBlocks = 2048;
 constexpr int kLdfAcc = 32;        // round sums: workgroup b adds into slot b % kLdfAcc
 constexpr int kWin32 = kWinBits / 32;   // 32-bit words of a wave's mex window
 constexpr uint8_t kFlD = 1, kFlX = 2, kFlPm = 4;   // per-vertex flags of a round
-// pdir per slot: the neighbour does not precede the row's vertex (kPdNone),
-// or precedes it (kPdHub: its colour is read from the colour array during the
-// loop, not from the slot colour cache, so a colour change writes into none
-// of its successors' rows).  kPdPred (read from the slot) is kept for
-// predecessors with cap < kLazyCap; with kLazyCap 0 there are none (measured
-// on R-MAT 22: 5 % faster than reading predecessors below kMidLen from their
-// slots, which costs the row pass a slot write per successor).
-constexpr uint8_t kPdNone = 0, kPdPred = 1, kPdHub = 2;
-constexpr uint32_t kLazyCap = 0;
+// pdir per slot: the neighbour does not precede the row's vertex (kPdNone) or
+// precedes it (kPdPred).  During the loop a predecessor's colour is read from
+// the colour array, not from the slot colour cache, so a colour change writes
+// into none of its successors' rows (measured on R-MAT 22: 5 % faster than
+// reading predecessors below 512 slots from their slots, which costs the row
+// pass a slot write per successor).
+constexpr uint8_t kPdNone = 0, kPdPred = 1;
 
 // Per-vertex state of the loop, gathered as one 16-byte record: the colour
 // (a copy of the colour array, kept by the commit), the new colour of a
@@ -82,9 +80,8 @@ struct LdfState {
   LdfV* V;
   int32_t* corig;       // colour at the loop's start of every vertex that changed (-1: none)
   uint8_t* pdir;        // slot -> its neighbour precedes the row's vertex (set up per batch)
-  uint32_t* pslot;      // rows with cap >= kSmallLen: their predecessors, listed from the
-                        // row's start: slot position, or (top bit set) a hub's vertex ID
-                        // (set up per batch)
+  uint32_t* pslot;      // rows with cap >= kSmallLen: their predecessors' vertex IDs, listed
+                        // from the row's start (set up per batch)
   int32_t* npred;       // ... and how many
   uint8_t* F[2];        // F[r & 1] = L_r, F[(r + 1) & 1] = C_r = L_{r+1}
   uint8_t* fl;          // kFlD / kFlX / kFlPm of the current round
@@ -112,15 +109,14 @@ __device__ __forceinline__ int32_t mex64(uint32_t lo, uint32_t hi) {
   return lo != ~0u ? __ffs(~lo) - 1 : hi != ~0u ? 32 + __ffs(~hi) - 1 : -1;
 }
 
-// A short row (used < kSmallLen) of v by its own thread: mex of the
-// predecessors' colours and the predecessor slots with colour cv.  At most 63
-// predecessors, so the mex is below 64.  INIT classifies the slots by the
-// priorities and records the predecessor slots in pdir (fixed for the batch:
-// no priority moves inside the loop); the loop's scans read pdir.
-template <bool INIT>
-__device__ __forceinline__ void short_pred_scan(const GraphView& G, uint8_t* pdir,
-                                                const int32_t* col0, int32_t v, int32_t cv,
-                                                int32_t& pm, int32_t& cc) {
+// Set-up of a short row (used < kSmallLen) of v by its own thread: classifies
+// the slots by the priorities into pdir (fixed for the batch: no priority
+// moves inside the loop; the loop's scans read pdir), and returns the mex of
+// the predecessors' colours and the predecessor slots with colour cv, from
+// the slot colour cache (current at the batch's start).  At most 63
+// predecessors, so the mex is below 64.
+__device__ __forceinline__ void short_pred_init(const GraphView& G, uint8_t* pdir, int32_t v,
+                                                int32_t cv, int32_t& pm, int32_t& cc) {
   const uint2 r = G.row[v];
   const int32_t pv = G.prop[v];
   uint32_t lo = 0u, hi = 0u;
@@ -133,19 +129,10 @@ __device__ __forceinline__ void short_pred_scan(const GraphView& G, uint8_t* pdi
       const bool in = j0 + q < r.y;
       const uint64_t j = (uint64_t)r.x + j0 + q;
       c[q] = in ? G.ccol[j] : -1;
-      if (INIT) {
-        const int32_t w = in ? G.col[j] : -1;
-        const bool pr = w >= 0 && precedes(nullptr, w, G.prop[w], v, pv);
-        pd[q] = !pr ? kPdNone : G.cap[w] >= kLazyCap ? kPdHub : kPdPred;
-        if (in) pdir[j] = pd[q];
-      } else {
-        pd[q] = in ? pdir[j] : kPdNone;
-      }
-    }
-    if (!INIT) {
-#pragma unroll
-      for (int q = 0; q < 4; ++q)
-        if (pd[q] == kPdHub) c[q] = col0[G.col[(uint64_t)r.x + j0 + q]];
+      const int32_t w = in ? G.col[j] : -1;
+      const bool pr = w >= 0 && precedes(nullptr, w, G.prop[w], v, pv);
+      pd[q] = pr ? kPdPred : kPdNone;
+      if (in) pdir[j] = pd[q];
     }
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
@@ -160,12 +147,12 @@ __device__ __forceinline__ void short_pred_scan(const GraphView& G, uint8_t* pdi
 
 // A whole row of v by one wave: the mex over LDS windows of kWinBits colours
 // (a window past the first only when it is full) and the predecessor slots
-// with colour cv.  Every lane returns both.  INIT classifies every slot by
-// the priorities (pdir, as for short_pred_scan) and lists the predecessors in
+// with colour cv.  Every lane returns both.  INIT (the batch's set-up, colours
+// from the slot colour cache) classifies every slot by the priorities (pdir,
+// as for short_pred_init) and lists the predecessors' vertex IDs in
 // pslot[row start ...] (npred[v] of them: the row's own span of a slot-sized
-// array; a slot position, or a hub predecessor's vertex ID with the top bit
-// set); the loop's scans walk that list, so a hub row with few predecessors
-// costs a few loads however long it is.
+// array); the loop's scans walk that list and read the colour array, so a
+// hub row with few predecessors costs a few loads however long it is.
 template <bool INIT>
 __device__ __forceinline__ void wave_pred_scan(const GraphView& G, uint8_t* pdir, uint32_t* pslot,
                                                int32_t* npred, const int32_t* col0, int32_t v,
@@ -186,14 +173,10 @@ __device__ __forceinline__ void wave_pred_scan(const GraphView& G, uint8_t* pdir
       const uint64_t s = (uint64_t)r.x + j;
       const int32_t w = j < r.y ? G.col[s] : -1;
       const bool pr = w >= 0 && precedes(nullptr, w, G.prop[w], v, pv);
-      if (j < r.y) pdir[s] = !pr ? kPdNone : G.cap[w] >= kLazyCap ? kPdHub : kPdPred;
+      if (j < r.y) pdir[s] = pr ? kPdPred : kPdNone;
       const unsigned long long b = __ballot(pr);
       if (pr) {
-        // a hub predecessor is listed by its vertex ID (top bit set): the
-        // loop reads its colour from the colour array
-        const bool hubp = G.cap[w] >= kLazyCap;
-        pslot[(uint64_t)r.x + base + (uint32_t)__popcll(b & ((1ull << lane) - 1ull))] =
-            hubp ? (0x80000000u | (uint32_t)w) : j;
+        pslot[(uint64_t)r.x + base + (uint32_t)__popcll(b & ((1ull << lane) - 1ull))] = (uint32_t)w;
         const int32_t c = G.ccol[s];
         if (c == cv) ++k;
         if ((uint32_t)c < (uint32_t)kWinBits) atomicOr(&ww[(uint32_t)c >> 5], 1u << ((uint32_t)c & 31u));
@@ -215,8 +198,7 @@ __device__ __forceinline__ void wave_pred_scan(const GraphView& G, uint8_t* pdir
         __builtin_amdgcn_wave_barrier();
       }
       for (uint32_t t = lane; t < np; t += 64) {
-        const uint32_t e = pslot[(uint64_t)r.x + t];
-        const int32_t c = (e & 0x80000000u) ? col0[e & 0x7fffffffu] : G.ccol[(uint64_t)r.x + e];
+        const int32_t c = col0[pslot[(uint64_t)r.x + t]];
         if (base == 0 && c == cv) ++k;
         const uint32_t cw = (uint32_t)(c - base);
         if (cw < (uint32_t)kWinBits) atomicOr(&ww[cw >> 5], 1u << (cw & 31u));
@@ -327,7 +309,7 @@ __global__ __launch_bounds__(kBlock) void k_ldf_init_cc(LdfState S) {
     if (G.hidx[v] >= 0) continue;
     int32_t pm, cc;
     const int32_t cv = S.c0[v];
-    short_pred_scan<true>(G, S.pdir, S.c0, (int32_t)v, cv, pm, cc);
+    short_pred_init(G, S.pdir, (int32_t)v, cv, pm, cc);
     S.V[v] = LdfV{cv, -1, pm, cc};
   }
   uint32_t* ww = win[threadIdx.x >> 6];
@@ -443,11 +425,9 @@ __global__ __launch_bounds__(kBlock) void k_ldf_members(LdfState S, int32_t r) {
             pd[q] = j < rw.y ? S.pdir[sl] : kPdNone;
             w[q] = j < rw.y ? G.col[sl] : 0;
           }
-          // a predecessor's colour from the colour array (kPdHub) or its slot
+          // a predecessor's colour from the colour array
 #pragma unroll
-          for (int q = 0; q < 4; ++q)
-            c[q] = pd[q] == kPdHub ? S.c0[w[q]]
-                 : pd[q] == kPdPred ? G.ccol[(uint64_t)rw.x + j0 + 8u * q] : 0;
+          for (int q = 0; q < 4; ++q) c[q] = pd[q] != kPdNone ? S.c0[w[q]] : 0;
 #pragma unroll
           for (int q = 0; q < 4; ++q)
             if (pd[q] != kPdNone) add_colour_bit((uint32_t)c[q], lo, hi);
@@ -506,16 +486,15 @@ __global__ __launch_bounds__(kBlock) void k_ldf_long_mex(LdfState S, int32_t r) 
   ldf_sums(S, r & 1, 0u, 0u, nx);
 }
 
-// x's colour c (was old) as owned neighbour w keeps it: with SLOT the slot
-// of x in w's row (position t), and w's histogram and cached mex over all its
-// neighbours, as propagate_slots keeps them.
-template <bool SLOT>
+// x's colour c (was old) as owned neighbour w keeps it: the slot of x in w's
+// row (position t), and w's histogram and cached mex over all its neighbours,
+// as propagate_slots keeps them.
 __device__ __forceinline__ void nbr_colour(const GraphView& G, int32_t w, int32_t t, int32_t c,
                                            int32_t old) {
   if (w >= G.n_owned) return;
   const int32_t h = G.hidx[w];
   const int32_t m = G.mexc[w];
-  if (SLOT) G.ccol[(uint64_t)G.row[w].x + (uint32_t)t] = c;
+  G.ccol[(uint64_t)G.row[w].x + (uint32_t)t] = c;
   uint32_t before = 2u;
   if (h >= 0) {
     uint32_t* hr = G.hist + (uint64_t)h * kHistW;
@@ -529,44 +508,35 @@ __device__ __forceinline__ void nbr_colour(const GraphView& G, int32_t w, int32_
 
 // One lane's share of the row pass of x (slots lo, lo + stride, ...), over
 // x's successors y only: D adds x's +-1 to K[y]; X (colour a -> b) moves
-// cc[y], drops y's predecessor mex when the change can move it, and writes b
-// into x's slot in y's row.  A cached mex m of y's predecessors' colours
+// cc[y] and drops y's predecessor mex when the change can move it (y reads
+// b from the colour array).  A cached mex m of y's predecessors' colours
 // moves only if a < m (a may have been the only predecessor holding a colour
 // below m) or b == m; a > m leaves every colour below m present, and b != m
 // adds no colour that was missing at m.  When several predecessors of y
 // change in one round, each tests the same round-start m, so m is dropped if
 // any of their changes can move it.
-// The loop's scans read predecessors' slots only, and no histogram or
-// all-neighbour mex: x's slots in its predecessors' rows, and its
-// neighbours' histograms and all-neighbour mex caches, take x's final colour
-// once, after the loop (k_ldf_settle).
+// The loop reads no slot colour, histogram or all-neighbour mex: x's slots
+// in its neighbours' rows, their histograms and all-neighbour mex caches take
+// x's final colour once, after the loop (k_ldf_settle).
 __device__ __forceinline__ void ldf_row_part(const LdfState& S, int32_t x, uint8_t f, int32_t ksign,
                                              int32_t a, int32_t b, uint64_t rbase, uint32_t lo,
                                              uint32_t hi, uint32_t stride) {
   const GraphView& G = S.G;
   const bool xc = (f & kFlX) != 0;
-  // successors read x's colour from its slot only when x is below kLazyCap
-  const bool slots = xc && G.cap[x] < kLazyCap;
   for (uint32_t j0 = lo; j0 < hi; j0 += 4 * stride) {
-    int32_t y[4], t[4];
+    int32_t y[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const uint32_t j = j0 + q * stride;
       // successors only: live slots whose neighbour does not precede x
-      y[q] = j < hi && S.pdir[rbase + j] == 0 ? G.col[rbase + j] : -1;
-      t[q] = slots && y[q] >= 0 ? G.twin[rbase + j] : 0;
+      y[q] = j < hi && S.pdir[rbase + j] == kPdNone ? G.col[rbase + j] : -1;
     }
     // the successors' records, all in flight at once
     LdfV vy[4];
-    uint32_t ry[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       vy[q] = LdfV{-1, -1, -1, 0};
-      ry[q] = 0u;
-      if (xc && y[q] >= 0) {
-        vy[q] = S.V[y[q]];
-        if (slots) ry[q] = G.row[y[q]].x;
-      }
+      if (xc && y[q] >= 0) vy[q] = S.V[y[q]];
     }
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
@@ -580,7 +550,6 @@ __device__ __forceinline__ void ldf_row_part(const LdfState& S, int32_t x, uint8
         if (dlt) atomicAdd(&S.V[y[q]].cc, dlt);
         const int32_t my = vy[q].pm;
         if (my >= 0 && (a < my || b == my)) S.V[y[q]].pm = -1;
-        if (slots) G.ccol[(uint64_t)ry[q] + (uint32_t)t[q]] = b;
       }
     }
   }
@@ -624,22 +593,19 @@ __global__ __launch_bounds__(kBlock) void k_ldf_rows(LdfState S, int32_t r) {
 }
 
 // after the loop: every vertex whose colour differs from its colour at the
-// loop's start writes its final colour into its slots in its predecessors'
-// rows (the rows its round passes skipped) and moves all its neighbours'
-// histograms and all-neighbour mex caches from the start colour to it
+// loop's start writes its final colour into its slots in all its neighbours'
+// rows and moves their histograms and all-neighbour mex caches from the
+// start colour to it
 __global__ __launch_bounds__(kBlock) void k_ldf_settle(LdfState S) {
   const GraphView& G = S.G;
   auto pass = [&](int32_t x, uint32_t lo, uint32_t hi_cap, uint32_t stride) {
     const uint2 rw = G.row[x];
     const uint32_t hi = min(rw.y, hi_cap);
     const int32_t a = S.corig[x], b = S.c0[x];
-    const bool hub = G.cap[x] >= kLazyCap;   // wrote no slot colours in the loop
     for (uint32_t j = lo; j < hi; j += stride) {
       const uint64_t sl = (uint64_t)rw.x + j;
       const int32_t y = G.col[sl];
-      if (y < 0) continue;
-      if (hub || S.pdir[sl] != kPdNone) nbr_colour<true>(G, y, G.twin[sl], b, a);
-      else nbr_colour<false>(G, y, 0, b, a);
+      if (y >= 0) nbr_colour(G, y, G.twin[sl], b, a);
     }
   };
   auto changed = [&](int32_t v) { return S.corig[v] >= 0 && S.corig[v] != S.c0[v]; };
