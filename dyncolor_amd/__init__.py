@@ -76,7 +76,8 @@ class Stats(C.Structure):
                 ("pull_rounds", C.c_int32), ("heavy_push_rounds", C.c_int32),
                 ("heavy_push_listed", C.c_int32), ("graph_captures", C.c_int32),
                 ("lc_vertices", C.c_int64), ("lc_slots", C.c_int64), ("del_row_slots", C.c_int64),
-                ("ms_round_a", C.c_double)]
+                ("ms_round_a", C.c_double), ("moved_slots", C.c_int64),
+                ("slots_rewritten", C.c_int32), ("reserved1", C.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_ if not k.startswith("reserved")}

@@ -190,14 +190,53 @@ __global__ void k_fill(int64_t m, const int32_t* u, const int32_t* v, GraphView 
   }
 }
 
-// ccol[j] <- colour of col[j] (tombstones -1) for every used slot, one wave per row
-__global__ void k_rebuild_ccol(GraphView G, const int32_t* color) {
+// ccol[j] <- colour of col[j] (tombstones -1) for every used slot of the rows
+// below kMidLen slots of capacity.  A wave takes 64 vertices at a time: rows
+// below kSmallLen by 8-lane groups (a lane per slot of a group's row), the
+// longer ones by the whole wave, one after the other.
+__global__ __launch_bounds__(kBlock) void k_ccol_rows(GraphView G, const int32_t* color) {
+  __shared__ int32_t lists[kBlock / 64][64];
+  int32_t* lst = lists[threadIdx.x >> 6];
   const int lane = lane_id();
-  const int64_t wave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+  const int grp = lane >> 3, sub = lane & 7;
+  const int64_t gwave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
   const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
-  for (int64_t v = wave; v < G.n_owned; v += nwaves) {
-    const uint2 r = G.row[v];
-    for (uint32_t j = lane; j < r.y; j += 64) {
+  auto slots = [&](uint2 r, uint32_t lo, uint32_t stride) {
+    for (uint32_t j = lo; j < r.y; j += stride) {
+      const int32_t w = G.col[(uint64_t)r.x + j];
+      G.ccol[(uint64_t)r.x + j] = w >= 0 ? color[w] : -1;
+    }
+  };
+  for (int64_t base = gwave * 64; base < G.n_owned; base += nwaves * 64) {
+    const int64_t v = base + lane;
+    const uint32_t cv = v < G.n_owned ? G.cap[v] : kMidLen;
+    const unsigned long long bs = __ballot(cv < kSmallLen);
+    if (cv < kSmallLen) lst[__popcll(bs & ((1ull << lane) - 1ull))] = lane;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    const uint32_t ns = (uint32_t)__popcll(bs);
+    for (uint32_t t = 0; t < ns; t += 8)
+      if (t + (uint32_t)grp < ns) slots(G.row[base + lst[t + grp]], (uint32_t)sub, 8u);
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    unsigned long long bm = __ballot(cv >= kSmallLen && cv < kMidLen);
+    while (bm) {
+      const int src = __ffsll((long long)bm) - 1;
+      bm &= bm - 1ull;
+      slots(G.row[base + src], (uint32_t)lane, 64u);
+    }
+  }
+}
+
+// the same for the rows of kMidLen slots of capacity or more: a workgroup per
+// entry of the hub chunk table (kChunk slots of one row)
+__global__ __launch_bounds__(kBlock) void k_ccol_hubs(GraphView G, const int32_t* color,
+                                                      const int2* hch, uint32_t n_hch) {
+  for (uint32_t e = blockIdx.x; e < n_hch; e += gridDim.x) {
+    const int2 hc = hch[e];
+    const uint2 r = G.row[hc.x];
+    const uint32_t hi = min(r.y, (uint32_t)(hc.y + 1) * kChunk);
+    for (uint32_t j = (uint32_t)hc.y * kChunk + threadIdx.x; j < hi; j += kBlock) {
       const int32_t w = G.col[(uint64_t)r.x + j];
       G.ccol[(uint64_t)r.x + j] = w >= 0 ? color[w] : -1;
     }
@@ -510,9 +549,9 @@ static int hist_fill(Graph* g) {
   return DC_OK;
 }
 
-// histogram rows, the hub chunk table and the histograms for h (from
+// histogram rows, the hub chunk table and (fill) the histograms for h (from
 // hub_counts)
-static int hist_reindex(Graph* g, const uint32_t h[3]) {
+static int hist_reindex(Graph* g, const uint32_t h[3], bool fill = true) {
   uint32_t* tmp = g->aux;
   const uint32_t nh = h[2];
   if (nh > g->hist_cap) {
@@ -533,7 +572,7 @@ static int hist_reindex(Graph* g, const uint32_t h[3]) {
   DC_HIP(hipStreamSynchronize(g->stream));
   int rc = hub_chunk_table(g, h[1]);
   if (rc) return rc;
-  return hist_fill(g);
+  return fill ? hist_fill(g) : DC_OK;
 }
 
 int hist_reindex(Graph* g) {
@@ -640,8 +679,11 @@ static int layout_rows(Graph* g, int reset_used, int32_t** new_col, int32_t** ne
 int rebuild_ccol(Graph* g) {
   // every cached mex is stale once colours are replaced wholesale
   DC_HIP(hipMemsetAsync(g->mexc, 0xff, sizeof(int32_t) * ((size_t)g->n + 1), g->stream));
-  hipLaunchKernelGGL(k_rebuild_ccol, dim3(grid_for((int64_t)g->n_owned * 64)), dim3(kBlock), 0,
-                     g->stream, g->view(), g->color[g->cur]);
+  hipLaunchKernelGGL(k_ccol_rows, dim3(grid_for(g->n_owned)), dim3(kBlock), 0, g->stream, g->view(),
+                     g->color[g->cur]);
+  if (g->n_hch)
+    hipLaunchKernelGGL(k_ccol_hubs, dim3(std::min<uint32_t>(g->n_hch, 65535u)), dim3(kBlock), 0,
+                       g->stream, g->view(), g->color[g->cur], g->hch, g->n_hch);
   DC_HIP(hipGetLastError());
   return hist_fill(g);
 }
@@ -683,8 +725,12 @@ int csr_build(Graph* g, int64_t m, const int32_t* du, const int32_t* dv) {
   if (m > 0)
     hipLaunchKernelGGL(k_fill, dim3(grid_for(m)), dim3(kBlock), 0, g->stream, m, du, dv, g->view());
   DC_HIP(hipGetLastError());
-  if ((rc = rebuild_ccol(g))) return rc;
-  return hist_reindex(g);
+  // hub rows and their chunk table first: the slot colours are written by them
+  uint32_t h[3];
+  if ((rc = hub_counts(g, h))) return rc;
+  if ((rc = hist_reindex(g, h, false))) return rc;
+  if ((rc = ensure_hub_buffers(g, h))) return rc;
+  return rebuild_ccol(g);
 }
 
 // ------------------------------------------------------------- parts -------

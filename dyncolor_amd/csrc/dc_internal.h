@@ -254,6 +254,8 @@ struct LoopResult {
   int32_t graph_captures = 0;   // round blocks captured as HIP graphs (graph driver)
   uint64_t lc_vertices = 0;     // sum over rounds of |L_r u C_r| (profiled batches)
   uint64_t lc_slots = 0;        // sum over rounds of the row slots of L_r u C_r
+  uint64_t moved_slots = 0;     // ldf: row slots of the vertices the loop changed
+  bool slots_rewritten = false; // ldf: every row's slot colours rewritten after the loop
 };
 int loop_hi(Graph* g, Frontier L0, int32_t nl0, bool persistent, LoopResult* res);
 void loop_free(Graph* g);

@@ -63,9 +63,16 @@ struct alignas(16) LdfV {
   int32_t c, nc, pm, cc;
 };
 
+// after the loop, every row's slot colours are rewritten once the changed
+// vertices' rows hold more than 1/kSettleDiv of the laid-out slots (and more
+// than kSettleMin): a row pass costs ~6x a rewrite per slot on MI355X
+constexpr uint64_t kSettleDiv = 6;
+constexpr uint64_t kSettleMin = 1ull << 16;
+
 struct LdfCtl {
   int32_t done_round;   // the last round run (INT_MAX while running)
-  int32_t pad[3];
+  int32_t pad;
+  unsigned long long moved_slots;   // row slots of the vertices changed so far
 };
 
 // Rows by length: short (cap < kSmallLen) by their own thread; longer rows
@@ -609,10 +616,21 @@ __global__ __launch_bounds__(kBlock) void k_ldf_settle(LdfState S) {
     }
   };
   auto changed = [&](int32_t v) { return S.corig[v] >= 0 && S.corig[v] != S.c0[v]; };
-  for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < G.n_owned;
-       v += (int64_t)gridDim.x * blockDim.x) {
-    if (G.hidx[v] >= 0 || !changed((int32_t)v)) continue;
-    pass((int32_t)v, 0u, 0xffffffffu, 1u);
+  {
+    // short rows: per wave 64 vertices at a time, the changed ones taken by
+    // 8-lane groups
+    __shared__ int32_t lists[kBlock / 64][64];
+    int32_t* lst = lists[threadIdx.x >> 6];
+    const int lane = lane_id();
+    const int64_t gwave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    for (int64_t base = gwave * 64; base < G.n_owned; base += nwaves * 64) {
+      const int64_t v = base + lane;
+      const bool pick = v < G.n_owned && G.hidx[v] < 0 && changed((int32_t)v);
+      short_rows_by_groups(pick, lst, [&](int src, int sub) {
+        if (src >= 0) pass((int32_t)(base + src), (uint32_t)sub, 0xffffffffu, 8u);
+      });
+    }
   }
   each_long_row<true>(S, changed, [&](int32_t v) { pass(v, (uint32_t)lane_id(), 0xffffffffu, 64u); });
   each_hub_chunk(S, changed, [&](int2 e) {
@@ -625,12 +643,15 @@ __global__ __launch_bounds__(kBlock) void k_ldf_settle(LdfState S) {
 // loop ends after a round without a colour change
 __global__ __launch_bounds__(kBlock) void k_ldf_commit(LdfState S, int32_t r) {
   if (S.ctl->done_round < r) return;
-  unsigned long long lcv = 0ull, lcs = 0ull;
+  unsigned long long lcv = 0ull, lcs = 0ull, ms = 0ull;
   for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < S.G.n_owned;
        v += (int64_t)gridDim.x * blockDim.x) {
     if (S.fl[v] & kFlX) {
       const int32_t nc = S.V[v].nc;
-      if (S.corig[v] < 0) S.corig[v] = S.c0[v];
+      if (S.corig[v] < 0) {   // v's first change in this loop
+        S.corig[v] = S.c0[v];
+        ms += S.G.row[v].y;
+      }
       S.c0[v] = nc;
       S.c1[v] = nc;
       S.V[v].c = nc;
@@ -640,6 +661,8 @@ __global__ __launch_bounds__(kBlock) void k_ldf_commit(LdfState S, int32_t r) {
       lcs += S.G.row[v].y;
     }
   }
+  for (int off = 32; off > 0; off >>= 1) ms += __shfl_xor(ms, off);
+  if (lane_id() == 0 && ms) atomicAdd(&S.ctl->moved_slots, ms);
   if (S.prof) {
     for (int off = 32; off > 0; off >>= 1) {
       lcv += __shfl_xor(lcv, off);
@@ -814,9 +837,21 @@ int loop_ldf(Graph* g, Frontier L0, LoopResult* res) {
     r += nb;
     if (s->hctl->done_round != INT_MAX) rounds = s->hctl->done_round + 1;
   }
-  hipLaunchKernelGGL(k_ldf_settle, grid, blk, 0, g->stream, S);
-  DC_HIP(hipGetLastError());
-  ++launches;
+  // the final colours into the neighbours' slots, histograms and
+  // all-neighbour mexes: by the changed vertices' rows, or by a rewrite of
+  // every row's slot colours once those rows hold more than 1/kSettleDiv of
+  // the laid-out slots (a row pass costs several scattered accesses per slot,
+  // the rewrite one gather per slot)
+  res->moved_slots = s->hctl->moved_slots;
+  res->slots_rewritten = res->moved_slots > std::max<uint64_t>(g->rows_end / kSettleDiv, kSettleMin);
+  if (res->slots_rewritten) {
+    if ((rc = rebuild_ccol(g))) return rc;
+    launches += 3;
+  } else {
+    hipLaunchKernelGGL(k_ldf_settle, grid, blk, 0, g->stream, S);
+    DC_HIP(hipGetLastError());
+    ++launches;
+  }
   if (rounds < 0) {
     DC_HIP(hipStreamSynchronize(g->stream));
     set_error("recolor loop hit max_rounds");

@@ -511,6 +511,8 @@ int recolor_plain(Graph* g, int rule, dc_stats* st, bool persistent_loop) {
       st->recolor_launches = launches;
       st->ms_count_pass = lr.count_ms;
       st->ms_round_a = lr.round_a_ms;
+      st->moved_slots = (int64_t)lr.moved_slots;
+      st->slots_rewritten = lr.slots_rewritten ? 1 : 0;
       st->count_pass_bytes = (int64_t)lr.count_bytes;
       st->count_pass_launches = (int32_t)lr.count_launches;
     }

@@ -133,6 +133,13 @@ typedef struct {
   double ms_round_a;        /* summed duration of the independent-rule loop's
                                phase-A launches (profiling on, plain driver);
                                with ms_count_pass: the loop rounds' kernel time */
+  int64_t moved_slots;      /* ldf rule: row slots of the vertices whose colour
+                               the loop changed                               */
+  int32_t slots_rewritten;  /* ldf rule: 1 if the neighbours' slot colours and
+                               histograms were rewritten for every row after the
+                               loop (moved rows holding more than a sixth of
+                               the laid-out slots), 0 if by the moved rows    */
+  int32_t reserved1;
 } dc_stats;
 
 /* Work-split parameters of the independent-rule loop and the batch apply

@@ -144,3 +144,40 @@ def test_rules_alternate_on_one_graph(oracle_mod):
         assert same_trace(g.trace(), ph), f"batch {i} rule {rule}: trace"
         assert g.check_consistency() == 0
     assert g.validate()[0] == 0
+
+
+def test_ldf_slot_colours_both_ways(oracle_mod):
+    # after the loop the neighbours' slot colours, histograms and all-neighbour
+    # mexes take the final colours: by the moved vertices' rows when few moved
+    # (small batches on a greedy colouring), by a rewrite of every row when
+    # many did (from colours far from a greedy colouring most vertices move);
+    # both leave them exact, and an independent-rule batch next reads them
+    say("ldf slot colours both ways")
+    from traces import same_trace
+    u, v = d.gen_rmat(17, 16, seed=8)
+    n = 1 << 17
+    g = d.Graph(n, u, v)
+    og = oracle_mod.OracleGraph(n, u, v)
+    g.static_color(1)
+    og.static_color(1)
+    rng = np.random.default_rng(8)
+    start = rng.integers(0, 4, n).astype(np.int32)
+    steps = [(d.gen_batches(n, u, v, 1, k, 0.5, 40 + k)[0], 1, None) for k in (4, 16, 64, 256)]
+    steps += [(d.gen_batches(n, u, v, 1, 1500, 0.5, 33)[0], 1, start),
+              (d.gen_batches(n, u, v, 1, 1500, 0.5, 35)[0], 0, None)]
+    ways = []
+    for i, ((bu, bv, bi), rule, colors) in enumerate(steps):
+        if colors is not None:
+            g.set_colors(colors)
+            og.set_colors(colors)
+        st = g.process_batch(bu, bv, bi, rule)
+        og.apply_batch(bu, bv, bi)
+        ph = og.recolor(rule)
+        assert (g.colors() == og.color).all(), f"batch {i} rule {rule}"
+        assert same_trace(g.trace(), ph), f"batch {i} rule {rule}: trace"
+        assert g.check_consistency() == 0, f"batch {i}"
+        if rule == 1:
+            ways.append((st.slots_rewritten, st.moved_slots))
+    say("ways", ways)
+    assert ways[-1][0] == 1
+    assert any(w == 0 and m > 0 for w, m in ways[:-1]), ways
