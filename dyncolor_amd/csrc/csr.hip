@@ -244,8 +244,9 @@ __global__ __launch_bounds__(kBlock) void k_ccol_hubs(GraphView G, const int32_t
 }
 
 // Count slots breaking an invariant: ccol != colour of the neighbour, or a
-// twin that does not point back; and vertices whose cached mex (below the
-// window size) is not the mex of their neighbours' colours (tests only).
+// twin that does not point back; vertices whose cached mex (below the window
+// size) is not the mex of their neighbours' colours, and rows with a
+// histogram row iff their capacity is at least kSmallLen (tests only).
 __global__ __launch_bounds__(kBlock) void k_check(GraphView G, const int32_t* color,
                                                   unsigned long long* bad) {
   __shared__ unsigned long long win[kBlock / 64][kWinWords];
@@ -257,6 +258,7 @@ __global__ __launch_bounds__(kBlock) void k_check(GraphView G, const int32_t* co
   for (int64_t v = wave; v < G.n_owned; v += nwaves) {
     const uint2 r = G.row[v];
     const int32_t mc = G.mexc[v];
+    if (lane == 0 && (G.cap[v] >= kSmallLen) != (G.hidx[v] >= 0)) ++cnt;
     if (mc >= 0 && mc < kWinBits) {
       ww[lane] = 0ull;
       ww[lane + 64] = 0ull;
@@ -453,7 +455,7 @@ void free_state(Graph* g) {
   f(g->tmark); f(g->mmark); f(g->pending); f(g->cbits[0]); f(g->mexc);
   f(g->hidx); f(g->hist); f(g->hrowv); f(g->hch);
   f(g->fbuf); f(g->fcnt); f(g->hbuf); f(g->partial);
-  f(g->del); f(g->ins); f(g->xdel); f(g->xins); f(g->ovf);
+  f(g->del); f(g->ins); f(g->xdel); f(g->xins); f(g->ovf); f(g->rch);
   f(g->tkeys); f(g->tlong); f(g->sort_tmp);
   f(g->ctr); f(g->aux); f(g->tail); f(g->scan_tmp);
   f(g->scratch_batch.u); f(g->scratch_batch.v); f(g->scratch_batch.ins);
@@ -475,21 +477,32 @@ void free_state(Graph* g) {
 // Histogram rows: cap >= kSmallLen.  hidx[v] = row of v's histogram or -1.
 __global__ void k_hist_index(int32_t n, int32_t n_owned, const uint32_t* cap, int32_t* hidx,
                              int32_t* hrowv, uint32_t* counter) {
-  for (int32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
+  const int lane = lane_id();
+  // one atomic per wave for its histogram rows (the loop bound is the same
+  // for every lane of a wave)
+  for (int64_t b = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) & ~63ll; b < n;
+       b += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t v = b + lane;
+    const bool hub = v < n_owned && cap[v] >= kSmallLen;
+    const unsigned long long bm = __ballot(hub);
+    uint32_t base = 0u;
+    if (lane == 0 && bm) base = atomicAdd(counter, (uint32_t)__popcll(bm));
+    base = __shfl(base, 0);
     int32_t h = -1;
-    if (v < n_owned && cap[v] >= kSmallLen) {
-      h = (int32_t)atomicAdd(counter, 1u);
-      hrowv[h] = v;
+    if (hub) {
+      h = (int32_t)(base + (uint32_t)__popcll(bm & ((1ull << lane) - 1ull)));
+      hrowv[h] = (int32_t)v;
     }
-    hidx[v] = h;
+    if (v < n) hidx[v] = h;
   }
 }
 
-// One workgroup per hub: count the live slots' colours below kHistW in LDS.
+// One workgroup per histogram row h0..n_hrows-1: count the live slots'
+// colours below kHistW in LDS.
 __global__ __launch_bounds__(kBlock) void k_hist_fill(GraphView G, const int32_t* hrowv,
-                                                      uint32_t n_hrows) {
+                                                      uint32_t h0, uint32_t n_hrows) {
   __shared__ uint32_t cnt[kHistW];
-  for (uint32_t h = blockIdx.x; h < n_hrows; h += gridDim.x) {
+  for (uint32_t h = h0 + blockIdx.x; h < n_hrows; h += gridDim.x) {
     for (int i = threadIdx.x; i < kHistW; i += blockDim.x) cnt[i] = 0u;
     __syncthreads();
     const uint2 r = G.row[hrowv[h]];
@@ -543,7 +556,7 @@ static int hub_chunk_table(Graph* g, uint32_t chunks);
 static int hist_fill(Graph* g) {
   if (g->n_hrows == 0) return DC_OK;
   const uint32_t blocks = std::min<uint32_t>(g->n_hrows, 8192u);
-  hipLaunchKernelGGL(k_hist_fill, dim3(blocks), dim3(kBlock), 0, g->stream, g->view(), g->hrowv,
+  hipLaunchKernelGGL(k_hist_fill, dim3(blocks), dim3(kBlock), 0, g->stream, g->view(), g->hrowv, 0u,
                      g->n_hrows);
   DC_HIP(hipGetLastError());
   return DC_OK;
@@ -1240,42 +1253,105 @@ __global__ void k_fix_overflow(const int4* ovf, uint32_t novf, GraphView G, cons
 }
 
 // Move each row that over-ran its capacity into the spare tail with room
-// used + slack (one wave per row: the first overflow entry of a row claims
-// it).  Twin links hold positions inside rows, so nothing else changes.
+// used + 2 x slack.  A thread per overflow entry: the first entry of a row
+// claims it; a wave reserves its claimed rows' room at the tail with one
+// atomic and lists their copies, one entry per kRelocChunk slots kept (one
+// more atomic), for k_reloc_copy.  A row whose capacity crosses kSmallLen
+// takes the next histogram row (filled by the caller once the row is whole).
+// Twin links hold positions inside rows, so nothing else changes.
 // flags[0]: the tail ran out (the caller relays every row instead);
-// flags[1]: a row needs a new colour histogram (cap crossed kSmallLen).
-__global__ void k_relocate(const int4* ovf, uint32_t novf, GraphView G, uint32_t* mark,
-                           uint32_t epoch, unsigned long long* tail, unsigned long long tail_end,
-                           uint32_t* flags, float frac, int32_t smin) {
+// flags[1]: the histogram rows ran out (the caller re-indexes them all);
+// flags[2]: histogram rows in use; flags[3]: copy entries listed.
+constexpr uint32_t kRelocChunk = 2048;
+
+__global__ __launch_bounds__(kBlock) void k_relocate(const int4* ovf, uint32_t novf, GraphView G,
+                                                     uint32_t* mark, uint32_t epoch,
+                                                     unsigned long long* tail,
+                                                     unsigned long long tail_end, uint32_t* flags,
+                                                     float frac, int32_t smin, int4* rch,
+                                                     uint64_t rch_cap, int32_t* hrowv,
+                                                     uint32_t hist_cap) {
   const int lane = lane_id();
-  const int64_t wave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
-  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
-  for (int64_t i = wave; i < novf; i += nwaves) {
-    const int32_t v = ovf[i].x;
-    bool lead = false;
-    if (lane == 0) lead = claim(mark, v, epoch);
-    if (!__shfl((int)lead, 0)) continue;
-    const uint2 r = G.row[v];
-    const uint32_t oldcap = G.cap[v];
-    // a row that had to move gets twice the usual slack
-    const uint32_t newcap = r.y + 2 * slack_for(r.y, frac, smin);
-    unsigned long long start = 0;
-    if (lane == 0) start = atomicAdd(tail, (unsigned long long)newcap);
-    start = __shfl(start, 0);
-    if (start + newcap > tail_end) {
-      if (lane == 0) flags[0] = 1u;
-      continue;
+  const unsigned long long below = (1ull << lane) - 1ull;
+  // the loop bound is the same for every lane of a wave
+  for (uint64_t b = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) & ~63ull; b < novf;
+       b += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t i = b + lane;
+    int32_t v = -1;
+    if (i < novf) {
+      v = ovf[i].x;
+      if (!claim(mark, v, epoch)) v = -1;
     }
-    const uint32_t keep = min(r.y, oldcap);
-    for (uint32_t j = lane; j < keep; j += 64) {
-      G.col[start + j] = G.col[(uint64_t)r.x + j];
-      G.ccol[start + j] = G.ccol[(uint64_t)r.x + j];
-      G.twin[start + j] = G.twin[(uint64_t)r.x + j];
+    uint2 r = make_uint2(0u, 0u);
+    uint32_t oldcap = 0u, newcap = 0u;
+    if (v >= 0) {
+      r = G.row[v];
+      oldcap = G.cap[v];
+      newcap = r.y + 2 * slack_for(r.y, frac, smin);
     }
-    if (lane == 0) {
+    // the wave's room at the tail: an inclusive scan of newcap
+    unsigned long long x = newcap;
+    for (int off = 1; off < 64; off <<= 1) {
+      const unsigned long long y = __shfl_up(x, off);
+      if (lane >= off) x += y;
+    }
+    unsigned long long base = 0ull;
+    const unsigned long long tot = __shfl(x, 63);
+    if (lane == 0 && tot) base = atomicAdd(tail, tot);
+    const unsigned long long start = __shfl(base, 0) + x - newcap;
+    const bool moved = v >= 0 && start + newcap <= tail_end;
+    if (v >= 0 && !moved) flags[0] = 1u;
+    const uint32_t keep = moved ? min(r.y, oldcap) : 0u;
+    // copy entries
+    const uint32_t nch = (keep + kRelocChunk - 1) / kRelocChunk;
+    uint32_t cx = nch;
+    for (int off = 1; off < 64; off <<= 1) {
+      const uint32_t y = __shfl_up(cx, off);
+      if (lane >= off) cx += y;
+    }
+    uint32_t cbase = 0u;
+    const uint32_t ctot = __shfl(cx, 63);
+    if (lane == 0 && ctot) cbase = atomicAdd(&flags[3], ctot);
+    const uint64_t e0 = (uint64_t)__shfl(cbase, 0) + cx - nch;
+    for (uint32_t c = 0; c < nch && e0 + c < rch_cap; ++c)
+      rch[e0 + c] = make_int4((int)(r.x + c * kRelocChunk), (int)(uint32_t)(start + c * kRelocChunk),
+                              (int)min(kRelocChunk, keep - c * kRelocChunk), 0);
+    // histogram rows for the rows crossing kSmallLen
+    const bool cross = moved && oldcap < kSmallLen && newcap >= kSmallLen;
+    const unsigned long long bh = __ballot(cross);
+    uint32_t hbase = 0u;
+    if (lane == 0 && bh) hbase = atomicAdd(&flags[2], (uint32_t)__popcll(bh));
+    hbase = __shfl(hbase, 0);
+    if (cross) {
+      const uint32_t h = hbase + (uint32_t)__popcll(bh & below);
+      if (h < hist_cap) {
+        hrowv[h] = v;
+        G.hidx[v] = (int32_t)h;
+      } else {
+        flags[1] = 1u;
+      }
+    }
+    if (moved) {
       G.row[v].x = (uint32_t)start;
       G.cap[v] = newcap;
-      if (oldcap < kSmallLen && newcap >= kSmallLen) flags[1] = 1u;
+    }
+  }
+}
+
+// the copies listed by k_relocate: a wave per entry of at most kRelocChunk slots
+__global__ __launch_bounds__(kBlock) void k_reloc_copy(GraphView G, const int4* rch,
+                                                       const uint32_t* nrch, uint64_t rch_cap) {
+  const int lane = lane_id();
+  const uint64_t ne = min((uint64_t)*nrch, rch_cap);
+  const uint64_t wave = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) >> 6;
+  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+  for (uint64_t e = wave; e < ne; e += nwaves) {
+    const int4 c = rch[e];
+    const uint64_t from = (uint32_t)c.x, to = (uint32_t)c.y;
+    for (uint32_t j = lane; j < (uint32_t)c.z; j += 64) {
+      G.col[to + j] = G.col[from + j];
+      G.ccol[to + j] = G.ccol[from + j];
+      G.twin[to + j] = G.twin[from + j];
     }
   }
 }
@@ -1286,18 +1362,36 @@ static int regrow(Graph* g, uint32_t novf);
 // spare tail, or relay every row when the tail is used up.
 static int make_room(Graph* g, uint32_t novf, bool* relaid) {
   *relaid = false;
+  // copy entries: at most one per claimed row plus one per kRelocChunk slots
+  const uint64_t need = (uint64_t)novf + g->total_cap / kRelocChunk + 64;
+  if (need > g->rch_cap) {
+    if (g->rch) DC_HIP(hipFree(g->rch));
+    g->rch = nullptr;
+    g->rch_cap = 0;
+    const uint64_t c = need + need / 4;
+    DC_HIP(hipMalloc(&g->rch, sizeof(int4) * (size_t)c));
+    g->rch_cap = c;
+  }
   uint32_t* flags = g->aux + 8;
-  DC_HIP(hipMemsetAsync(flags, 0, sizeof(uint32_t) * 2, g->stream));
+  DC_HIP(hipMemsetAsync(flags, 0, sizeof(uint32_t) * 4, g->stream));
+  DC_HIP(hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(flags + 2), (int)g->n_hrows, 1,
+                           g->stream));
   int rc = ensure_epochs(g, 1);
   if (rc) return rc;
   const uint32_t e = next_epoch(g);
-  hipLaunchKernelGGL(k_relocate, dim3(grid_for((int64_t)novf * 64)), dim3(kBlock), 0, g->stream,
-                     g->ovf, novf, g->view(), g->tmark, e, g->tail,
-                     (unsigned long long)g->total_cap, flags, g->opt.slack_frac, g->opt.slack_min);
+  hipLaunchKernelGGL(k_relocate, dim3(grid_for(novf)), dim3(kBlock), 0, g->stream, g->ovf, novf,
+                     g->view(), g->tmark, e, g->tail, (unsigned long long)g->total_cap, flags,
+                     g->opt.slack_frac, g->opt.slack_min, g->rch, g->rch_cap, g->hrowv, g->hist_cap);
+  hipLaunchKernelGGL(k_reloc_copy, dim3(std::min<uint64_t>((need + 3) / 4, 8192)), dim3(kBlock), 0,
+                     g->stream, g->view(), g->rch, flags + 3, g->rch_cap);
   DC_HIP(hipGetLastError());
-  uint32_t hf[2];
+  uint32_t hf[4];
   DC_HIP(hipMemcpyAsync(hf, flags, sizeof(hf), hipMemcpyDeviceToHost, g->stream));
   DC_HIP(hipStreamSynchronize(g->stream));
+  if (hf[3] > g->rch_cap) {   // cannot happen with the bound above
+    set_error("row relocation copy list overflow");
+    return DC_ERR_NOMEM;
+  }
   if (hf[0]) {
     *relaid = true;
     return regrow(g, novf);
@@ -1306,14 +1400,21 @@ static int make_room(Graph* g, uint32_t novf, bool* relaid) {
                      g->view(), g->color[g->cur]);
   DC_HIP(hipGetLastError());
   // capacities grew: frontier hub buffers and the hub chunk table follow; a
-  // row past kSmallLen needs a histogram (the existing ones already count
-  // the batch)
+  // row past kSmallLen got a histogram row above, filled here once its
+  // overflow slots are in (the existing ones already count the batch)
   uint32_t h[3];
   if ((rc = hub_counts(g, h))) return rc;
   if (hf[1]) {
     if ((rc = hist_reindex(g, h))) return rc;
-  } else if ((rc = hub_chunk_table(g, h[1]))) {
-    return rc;
+  } else {
+    if (hf[2] > g->n_hrows) {
+      const uint32_t blocks = std::min<uint32_t>(hf[2] - g->n_hrows, 8192u);
+      hipLaunchKernelGGL(k_hist_fill, dim3(blocks), dim3(kBlock), 0, g->stream, g->view(), g->hrowv,
+                         g->n_hrows, hf[2]);
+      DC_HIP(hipGetLastError());
+      g->n_hrows = hf[2];
+    }
+    if ((rc = hub_chunk_table(g, h[1]))) return rc;
   }
   return ensure_hub_buffers(g, h);
 }

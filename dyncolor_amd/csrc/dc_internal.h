@@ -148,6 +148,8 @@ struct Graph {
   int2* xdel = nullptr;
   int2* xins = nullptr;
   int4* ovf = nullptr;        // insertion overflow {vertex, pos, value, 0}
+  int4* rch = nullptr;        // row relocation copies {old start, new start, slots, 0}
+  uint64_t rch_cap = 0;
   int64_t list_cap = 0;
   // deletions grouped by scanned row (csr_apply): keys [2][list_cap], runs
   // over long rows, radix-sort storage

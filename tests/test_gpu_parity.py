@@ -171,6 +171,30 @@ def test_regrow_path(oracle_mod, spare):
         assert sorted(g.neighbors(x).tolist()) == sorted(og.neighbors(x).tolist())
 
 
+@pytest.mark.parametrize("rule", [0, 1])
+def test_relocation_into_histogram_rows(oracle_mod, rule):
+    # rows with one spare slot: insertions move them into the spare tail, rows
+    # of ~62 slots cross 64 slots of capacity there and get a colour histogram
+    # each, and a hub of ~5 000 slots moves as several copy chunks
+    say("relocation into histogram rows", rule)
+    rng = np.random.default_rng(17)
+    n = 6000
+    u, v = d.gen_er(n, 186000, 17)
+    hub_nb = rng.choice(np.arange(1, n), 5000, replace=False).astype(np.int32)
+    u = np.concatenate([u, np.zeros(5000, np.int32)])
+    v = np.concatenate([v, hub_nb])
+    batches = []
+    for bu, bv, bi in d.gen_batches(n, u, v, 3, 3000, 0.8, 18):
+        extra = rng.integers(1, n, 200).astype(np.int32)
+        batches.append((np.concatenate([bu, np.zeros(200, np.int32)]), np.concatenate([bv, extra]),
+                        np.concatenate([bi, np.ones(200, np.uint8)])))
+    opts = d.options(slack_frac=0.0, slack_min=1)
+    g, og, st = run_both(oracle_mod, n, u, v, batches, rule, opts=opts)
+    assert sum(s.regrows for s in st) >= 1
+    for x in [0] + list(range(1, n, 211)):
+        assert sorted(g.neighbors(x).tolist()) == sorted(og.neighbors(x).tolist())
+
+
 def test_hub_rows_and_mid_rows(oracle_mod):
     # star hubs with > 4096 slots (chunked hub path) and 64..4096 (wave path)
     say("hubs")
