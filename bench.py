@@ -414,21 +414,6 @@ def run_c4(args, d, rule, steps, warmup):
     log(f"c4: {len(graphs)} graphs, {sum(g[0] for g in graphs)} vertices, {tot_edges} edges, "
         f"{step_lines[-1]} lines per step ({time.perf_counter() - t:.1f}s)")
 
-    if args.tune:
-        # the work split is tuned on the graphs as one arena
-        n, U, V, off = d.union_edges(graphs)
-        arena = d.Graph(n, U, V)
-        arena.static_color(rule)
-        set_driver(arena, d, args, "c4")
-        from dyncolor_amd import tune
-        arena.apply_device_batch(arena.upload_batch(*d.union_batch([p[0] for p in per], off)))
-        arena.recolor(rule)
-        tb = [d.union_batch([p[s] for p in per], off) for s in range(1, min(3, nsteps))]
-        best, sweep = tune.tune(arena, tb, rule, reps=args.tune_reps, log=log)
-        tune.save(args.tuning_path, "c4", best, sweep, {"graphs": len(graphs), "lines_per_step": int(step_lines[-1])})
-        log(f"c4 tuned: {best}")
-        return {"tuned": best, "sweep": sweep}
-
     # batched: the graphs in K arenas (lanes) balanced by edge count; a step
     # applies every lane's batch and recolours it, the lanes concurrently
     # (dc_lanes_step: one host thread and stream per lane)
@@ -449,10 +434,25 @@ def run_c4(args, d, rule, steps, warmup):
         a = d.Graph(nk, U, V)
         a.static_color(rule)
         set_driver(a, d, args, "c4")
-        ub = [a.upload_batch(*d.union_batch([per[i][s] for i in idx], off)) for s in range(nsteps)]
+        ub = [] if args.tune else [a.upload_batch(*d.union_batch([per[i][s] for i in idx], off))
+                                   for s in range(nsteps)]
         lanes.append((idx, off, a, ub))
         del U, V
     arenas = [l[2] for l in lanes]
+
+    if args.tune:
+        # the work split is tuned on the lanes stepped together, as they run
+        from dyncolor_amd import tune
+        first = [a.upload_batch(*d.union_batch([per[i][0] for i in idx], off))
+                 for idx, off, a, _ in lanes]
+        d.lanes_step(arenas, first, rule)
+        lane_batches = [[d.union_batch([per[i][s] for i in idx], off) for s in range(1, min(3, nsteps))]
+                        for idx, off, _, _ in lanes]
+        best, sweep = tune.tune_lanes(arenas, lane_batches, rule, reps=args.tune_reps, log=log)
+        tune.save(args.tuning_path, "c4", best, sweep, {"graphs": len(graphs), "lanes": K,
+                                                        "lines_per_step": int(step_lines[-1])})
+        log(f"c4 tuned: {best}")
+        return {"tuned": best, "sweep": sweep}
     for s in range(warmup):
         d.lanes_step(arenas, [l[3][s] for l in lanes], rule)
     lat, rounds = [], 0

@@ -76,18 +76,74 @@ def tune(g, batches, rule=0, reps=4, margin=0.03, params=None, log=None):
     """Coordinate descent on the live graph g over the given host batches
     (each used as a batch + its inverse).  Returns (best params, sweep)."""
     pairs = [(g.upload_batch(*b), g.upload_batch(*_inverse(b))) for b in batches]
-    best = {"driver": DRIVER_PLAIN}
-    best.update({k: getattr(tuning(), k) for k in CANDIDATES if k != "driver"})
-    sweep = []
 
     def run(p, j):
         apply(g, p)
         return _trial(g, pairs[j % len(pairs)], rule)
 
+    state = {}
+    try:
+        return _descend(run, reps, margin, params, log, CANDIDATES, state)
+    finally:
+        if "best" in state:
+            apply(g, state["best"])
+        for a, b in pairs:
+            a.close()
+            b.close()
+
+
+def _lanes_trial(arenas, pair, rule):
+    from . import lanes_step
+    t0 = time.perf_counter()
+    for half in pair:
+        lanes_step(arenas, half, rule)
+    dt = time.perf_counter() - t0
+    for a in arenas:
+        a.compact()
+    return dt
+
+
+def tune_lanes(arenas, lane_batches, rule=0, reps=4, margin=0.03, params=None, log=None):
+    """The same descent for arenas stepped together (dc_lanes_step): every
+    arena takes the candidate, and a trial is one lanes step over each lane's
+    batch, then one over the inverses, so the lanes contend for the device as
+    they do in service.  lane_batches[l] is lane l's list of host batches (the
+    same count per lane).  The driver is not tuned (lanes use the plain one)."""
+    nb = min(len(b) for b in lane_batches)
+    pairs = [([a.upload_batch(*lb[j]) for a, lb in zip(arenas, lane_batches)],
+              [a.upload_batch(*_inverse(lb[j])) for a, lb in zip(arenas, lane_batches)])
+             for j in range(nb)]
+
+    def run(p, j):
+        for a in arenas:
+            apply(a, p)
+        return _lanes_trial(arenas, pairs[j % len(pairs)], rule)
+
+    cands = {k: v for k, v in CANDIDATES.items() if k != "driver"}
+    state = {}
+    try:
+        return _descend(run, reps, margin, params, log, cands, state)
+    finally:
+        for a in arenas:
+            if "best" in state:
+                apply(a, state["best"])
+        for fwd, inv in pairs:
+            for b in fwd + inv:
+                b.close()
+
+
+def _descend(run, reps, margin, params, log, cands, state):
+    """Coordinate descent over cands from the defaults; run(params, j) times
+    trial j.  state["best"] always holds the current choice (the caller sets
+    it on its graphs even when a trial fails)."""
+    best = {"driver": DRIVER_PLAIN}
+    best.update({k: getattr(tuning(), k) for k in CANDIDATES if k != "driver"})
+    state["best"] = best
+    sweep = []
     run(best, 0)   # warm caches and captures
     j = 0
-    for name in (params or list(CANDIDATES)):
-        for val in CANDIDATES[name]:
+    for name in (params or list(cands)):
+        for val in cands[name]:
             if val == best[name]:
                 continue
             cand = dict(best, **{name: val})
@@ -106,15 +162,12 @@ def tune(g, batches, rule=0, reps=4, margin=0.03, params=None, log=None):
                    "incumbent_ms": mi * 1e3, "candidate_ms": mc * 1e3}
             if mc < mi * (1.0 - margin):
                 best = cand
+                state["best"] = best
                 rec["taken"] = True
             sweep.append(rec)
             if log:
                 log(f"tune {name}={val}: {mc * 1e3:.2f} ms vs {mi * 1e3:.2f} ms"
                     f"{' (taken)' if rec.get('taken') else ''}")
-    apply(g, best)
-    for a, b in pairs:
-        a.close()
-        b.close()
     return best, sweep
 
 

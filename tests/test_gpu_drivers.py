@@ -124,3 +124,37 @@ def test_tuner_sweep_keeps_results(oracle_mod):
     assert g.validate()[0] == 0 and g.check_consistency() == 0
     params = {k: val for k, val in best.items() if k != "driver"}
     run_both(oracle_mod, n, u, v, batches, 0, tuning=params)
+
+
+def test_lane_tuner_keeps_results(oracle_mod):
+    # the tuner over two lanes stepped together: a choice from the candidates,
+    # set on both arenas, and the lanes still colour exactly as the oracle
+    from dyncolor_amd import tune
+    say("lane tuner")
+    graphs = [(1 << 12, *d.gen_rmat(12, 8, seed=31)), (3000, *d.gen_er(3000, 15000, 32))]
+    arenas, oracles, batches = [], [], []
+    for i, (n, u, v) in enumerate(graphs):
+        a = d.Graph(n, u, v)
+        a.static_color(0)
+        o = oracle_mod.OracleGraph(n, u, v)
+        o.static_color(0)
+        arenas.append(a)
+        oracles.append(o)
+        batches.append(d.gen_batches(n, u, v, 3, 400, 0.5, 40 + i))
+    best, sweep = tune.tune_lanes(arenas, [b[:2] for b in batches], 0, reps=1, params=["light_c"])
+    assert best["light_c"] in tune.CANDIDATES["light_c"]
+    assert all(a.get_tuning().light_c == best["light_c"] for a in arenas)
+    assert {e["param"] for e in sweep} == {"light_c"}
+    for a in arenas:
+        assert a.validate()[0] == 0 and a.check_consistency() == 0
+    # from the tuned state (each trial's batch was undone by its inverse, so
+    # the edges are the original ones): one more lanes step against the oracle
+    for a, o in zip(arenas, oracles):
+        o.set_colors(a.colors())
+    ups = [a.upload_batch(*b[2]) for a, b in zip(arenas, batches)]
+    d.lanes_step(arenas, ups, 0)
+    for a, o, b in zip(arenas, oracles, batches):
+        o.apply_batch(*b[2])
+        o.recolor(0)
+        assert (a.colors() == o.color).all()
+        assert a.check_consistency() == 0

@@ -51,3 +51,25 @@ def test_candidates_are_valid_tunings():
                 assert 0 <= v <= 4096
             else:
                 assert v >= 0
+
+
+def test_descent_takes_only_clear_wins():
+    # a fake trial time: light_c = 1024 is 20 % faster, list_c = 16384 1 %
+    # faster (inside the margin); the descent takes the first and not the
+    # second, tries each candidate at most once (the old default again after
+    # 1024 is taken) and keeps the choice in state
+    def run(p, j):
+        t = 1.0
+        if p["light_c"] == 1024:
+            t *= 0.8
+        if p["list_c"] == 16384:
+            t *= 0.99
+        return t
+    state = {}
+    cands = {k: tune.CANDIDATES[k] for k in ("light_c", "list_c")}
+    best, sweep = tune._descend(run, 2, 0.03, None, None, cands, state)
+    assert best["light_c"] == 1024 and best["list_c"] != 16384
+    assert state["best"] == best
+    tried = [e["value"] for e in sweep if e["param"] == "light_c"]
+    assert sorted(tried) == sorted(set(tried)) and 1024 in tried
+    assert [e for e in sweep if e.get("taken")] == [e for e in sweep if e["value"] == 1024]
