@@ -357,3 +357,21 @@ def test_worklist_modes(oracle_mod, limit):
         la = np.array(lines, np.int32)
         run_both(oracle_mod, nn, uu, vv, [(la[:, 0], la[:, 1], la[:, 2].astype(np.uint8))], 0,
                  opts=d.options(list_limit=limit), n_owned=n_owned, ghost_colors=ghost)
+
+
+@pytest.mark.parametrize("rule,scale", [(0, 17), (1, 16)])
+def test_rmat_stream_with_relocations(oracle_mod, rule, scale):
+    # R-MAT 16-17 with little spare row capacity and insert-heavy batches: rows
+    # of every class (hubs of thousands of slots too) move into the spare tail
+    # as several copy chunks, rows cross 64 slots there, and the ldf rule's
+    # large cascades rewrite every row's slot colours after the loop
+    say("rmat relocations", rule, scale)
+    u, v = d.gen_rmat(scale, 16, seed=12)
+    n = 1 << scale
+    batches = d.gen_batches(n, u, v, 3, 20000 >> (17 - scale), 0.8, 13)
+    opts = d.options(slack_frac=0.02, slack_min=1)
+    g, og, st = run_both(oracle_mod, n, u, v, batches, rule, opts=opts, device_batches=True)
+    assert sum(s.regrows for s in st) >= 1
+    assert g.validate()[0] == 0
+    for x in list(range(0, 64)) + list(range(64, n, 4099)):
+        assert sorted(g.neighbors(x).tolist()) == sorted(og.neighbors(x).tolist())
