@@ -91,6 +91,10 @@ constexpr int kWaveList = 64 * 16; // vertices of 128 state words (16 per lane)
 // R too when D's rows hold at most dc_tuning.list_slots slots (and the round
 // is not a pull round).
 constexpr int kGraphCache = 48;   // captured round blocks kept per graph (graph driver)
+// rounds 0..kPullSplitRounds-1 of the plain and graph drivers launch
+// k_pull_fix after their count pass, so that a pull recount among them can
+// skip about half of its neighbour-state gathers
+constexpr int32_t kPullSplitRounds = 4;
 constexpr uint32_t kQInvalid = 0xffffffffu;  // touched count of a round that did not list
 constexpr int kQT = 0;      // qc[kQT + (r & 3)]: entries of round r's touched list
 constexpr int kQD = 4;      // qc[kQD + 3 * (r & 1) + k]: D_r's short rows, mid rows, hub chunks
@@ -115,6 +119,7 @@ struct LoopCtl {
   int32_t pull_rounds;
   int32_t heavy_push_rounds;
   int32_t heavy_push_listed;
+  int32_t split_round;  // the latest round whose pull recount left work to k_pull_fix
 };
 
 // |L_r| (uniform: every workgroup reads the same words)
@@ -141,6 +146,11 @@ struct LoopState {
   uint32_t n_hrows;
   unsigned long long heavy_slots;  // D_r rows beyond this many slots: pull recount
   int32_t pull_mode;   // 0: by heavy_slots; 1: every round pulls; -1: no round pulls
+  // rounds below this one split a pull recount with k_pull_fix (launched
+  // after their count pass): the pull skips the slots whose neighbour has a
+  // higher cached colour, k_pull_fix adds the changed members' terms on them
+  // and writes the new colours into the neighbours' caches (0: no split)
+  int32_t pull_split_rounds;
   // per-batch work terms of the reference's algorithm (null: not counted):
   // {sum over rounds of |L_r u C_r|, of the row slots of L_r u C_r}
   unsigned long long* terms;
@@ -922,10 +932,16 @@ __device__ __forceinline__ uint64_t clear_state(uint32_t* b, uint32_t words, con
 // ---- heavy rounds: every K recounted by pull, no count atomics ------------
 // y's count over slots lo, lo+stride, ... of its row: members x of C_r whose
 // marking predicate holds for (x, y).
+// With split, a slot whose cached colour c is above y's colour is skipped
+// (no state gather): while the slot caches still hold every member's colour
+// from before the round (k_pull_fix writes the new ones after the pull), c is
+// a member's prev and its term (c' == cy && x < y) || c < cy can hold only
+// for a member that changed colour to c' == cy, which k_pull_fix adds.  About
+// half of all slots have the higher colour at the other end.
 template <int U>
 __device__ __forceinline__ uint32_t pull_slots(const LoopState& S, int32_t y, int32_t cy,
                                                uint64_t rbase, uint32_t lo, uint32_t hi,
-                                               uint32_t stride, const uint32_t* st,
+                                               uint32_t stride, const uint32_t* st, bool split,
                                                unsigned long long& slots) {
   const GraphView& G = S.G;
   uint32_t cnt = 0;
@@ -942,7 +958,7 @@ __device__ __forceinline__ uint32_t pull_slots(const LoopState& S, int32_t y, in
     uint32_t fl[U];
 #pragma unroll
     for (int k = 0; k < U; ++k) {
-      const bool own = x[k] >= 0 && x[k] < G.n_owned;
+      const bool own = x[k] >= 0 && x[k] < G.n_owned && !(split && c[k] > cy);
       fl[k] = own ? st_flags(st[(uint32_t)x[k] >> 3], x[k]) : 0u;
     }
     int32_t gc[U], gp[U];
@@ -971,7 +987,8 @@ __device__ __forceinline__ void set_count(const LoopState& S, int32_t y, uint32_
 }
 
 __device__ __forceinline__ void pull_all(const LoopState& S, uint32_t* st, LoopLds& lds,
-                                         unsigned long long& slots, unsigned long long& rows) {
+                                         bool split, unsigned long long& slots,
+                                         unsigned long long& rows) {
   const GraphView& G = S.G;
   const int lane = lane_id();
   const uint32_t gwave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -1004,7 +1021,7 @@ __device__ __forceinline__ void pull_all(const LoopState& S, uint32_t* st, LoopL
       const uint32_t rx = (uint32_t)__shfl((int)rw.x, src);
       const uint32_t ry_all = (uint32_t)__shfl((int)rw.y, src);   // every lane shuffles
       const uint32_t ry = mine ? ry_all : 0u;
-      uint32_t cnt = pull_slots<8>(S, yy, cyy, rx, sub, ry, kGroup, st, slots);
+      uint32_t cnt = pull_slots<8>(S, yy, cyy, rx, sub, ry, kGroup, st, split, slots);
       cnt += __shfl_xor(cnt, 1);
       cnt += __shfl_xor(cnt, 2);
       cnt += __shfl_xor(cnt, 4);
@@ -1020,7 +1037,7 @@ __device__ __forceinline__ void pull_all(const LoopState& S, uint32_t* st, LoopL
   each_mid_row(S, lds, S.hrowv, S.n_hrows, [](int32_t) { return true; },
                [&](int32_t x) { return S.col0[x]; },
                [&](int32_t x, uint2 rw, int32_t cx) {
-    uint32_t c2 = pull_slots<8>(S, x, cx, rw.x, lane, rw.y, 64, st, slots);
+    uint32_t c2 = pull_slots<8>(S, x, cx, rw.x, lane, rw.y, 64, st, split, slots);
     for (int off = 32; off > 0; off >>= 1) c2 += __shfl_xor(c2, off);
     if (lane == 0) {
       set_count(S, x, c2, st);
@@ -1034,7 +1051,8 @@ __device__ __forceinline__ void pull_all(const LoopState& S, uint32_t* st, LoopL
     const int32_t y = e.x;
     const uint2 rw = G.row[y];
     if (rw.y < kMidLen) continue;  // went by the short/mid pass (uniform)
-    const uint32_t cnt = pull_slots<8>(S, y, S.col0[y], rw.x, threadIdx.x, rw.y, kBlock, st, slots);
+    const uint32_t cnt =
+        pull_slots<8>(S, y, S.col0[y], rw.x, threadIdx.x, rw.y, kBlock, st, split, slots);
     const uint32_t tot = block_sum(cnt, lds.red);
     if (threadIdx.x == 0) {
       set_count(S, y, tot, st);
@@ -1043,14 +1061,16 @@ __device__ __forceinline__ void pull_all(const LoopState& S, uint32_t* st, LoopL
   }
 }
 
-// The new colours of D_r's members that changed into the slot cache, mex
-// cache and histograms of their neighbours.
-__device__ __forceinline__ void propagate_changed(const LoopState& S, const uint32_t* st,
-                                                  LoopLds& lds) {
+// The rows of D_r's members that changed colour (X_r), in slot ranges:
+// fn(x, new colour, old colour, row start, lo, hi, stride) with short rows by
+// a lane each, rows below kMidLen slots by a wave, hub rows by their chunks.
+// Every thread must call this (it synchronises the workgroup).
+template <class Fn>
+__device__ __forceinline__ void walk_changed(const LoopState& S, const uint32_t* st, LoopLds& lds,
+                                             Fn fn) {
   const GraphView& G = S.G;
   const int lane = lane_id();
   int32_t* list = lds.lists[threadIdx.x >> 6];
-  // the members that changed colour (X_r)
   walk_states(st, S.swords, kFX, list, [&](int32_t* lst, uint32_t total) {
     uint32_t nw = 0;
     for (uint32_t base = 0; base < total; base += 64) {
@@ -1064,7 +1084,7 @@ __device__ __forceinline__ void propagate_changed(const LoopState& S, const uint
         sn = S.col0[x];
         rw = G.row[x];
       }
-      if (chg && rw.y < kSmallLen) propagate_slots(G, sn, so, rw.x, 0, rw.y, 1);
+      if (chg && rw.y < kSmallLen) fn(x, sn, so, (uint64_t)rw.x, 0u, rw.y, 1u);
       const bool mid = chg && rw.y >= kSmallLen && rw.y < kMidLen;
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
       __builtin_amdgcn_wave_barrier();
@@ -1077,14 +1097,50 @@ __device__ __forceinline__ void propagate_changed(const LoopState& S, const uint
     for (uint32_t q = 0; q < nw; ++q) {
       const int32_t x = lst[q];
       const uint2 rw = G.row[x];
-      propagate_slots(G, S.col0[x], S.dold[x].x, rw.x, lane, rw.y, 64);
+      fn(x, S.col0[x], S.dold[x].x, (uint64_t)rw.x, (uint32_t)lane, rw.y, 64u);
     }
   });
   each_hub_chunk(S, lds, S.hch, S.n_hch, [&](int2 e) { return st_has(st, e.x, kFX); },
                  [&](int32_t x) { return make_int2(S.col0[x], S.dold[x].x); },
                  [&](int32_t x, uint2 rw, uint32_t lo, uint32_t hi, int2 cs) {
-    propagate_slots(G, cs.x, cs.y, rw.x, lo + lane, hi, 64);
+    fn(x, cs.x, cs.y, (uint64_t)rw.x, lo + (uint32_t)lane, hi, 64u);
   });
+}
+
+// The new colours of D_r's members that changed into the slot cache, mex
+// cache and histograms of their neighbours.
+__device__ __forceinline__ void propagate_changed(const LoopState& S, const uint32_t* st,
+                                                  LoopLds& lds) {
+  walk_changed(S, st, lds, [&](int32_t, int32_t sn, int32_t so, uint64_t rb, uint32_t lo,
+                               uint32_t hi, uint32_t stride) {
+    propagate_slots(S.G, sn, so, rb, lo, hi, stride);
+  });
+}
+
+// The terms a split pull skipped, for a changed member x (colour so -> sn):
+// x marks a neighbour y whose colour cy is below so iff sn == cy and x < y,
+// so only a member whose colour went down has any (sn < so).
+__device__ __forceinline__ void fix_slots(const LoopState& S, uint32_t* st, int32_t x, int32_t sn,
+                                          int32_t so, uint64_t rbase, uint32_t lo, uint32_t hi,
+                                          uint32_t stride) {
+  constexpr int U = 4;
+  const GraphView& G = S.G;
+  for (uint32_t j = lo; j < hi; j += U * stride) {
+    int32_t y[U], cy[U];
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+      const uint32_t jj = j + k * stride;
+      y[k] = jj < hi ? G.col[rbase + jj] : -1;
+    }
+#pragma unroll
+    for (int k = 0; k < U; ++k) cy[k] = y[k] > x && y[k] < G.n_owned ? S.col0[y[k]] : -1;
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+      if (cy[k] != sn) continue;   // sn >= 0, so this also drops the slots without a y
+      atomicAdd(&S.K[y[k]], 1);
+      if (!st_has(st, y[k], kFC)) st_set(st, y[k], kFR);
+    }
+  }
 }
 
 __device__ __forceinline__ void round_c(const LoopState& S, int r, LoopLds& lds) {
@@ -1145,6 +1201,7 @@ __device__ __forceinline__ void round_c(const LoopState& S, int r, LoopLds& lds)
       S.ctl->last_changed = (int32_t)lds.tot[1];
       if (!nonempty) S.ctl->done_round = r;
       if (nonempty && heavy) S.ctl->pull_rounds += 1;
+      if (nonempty && heavy && r < S.pull_split_rounds) S.ctl->split_round = r;
       if (nonempty && !heavy && big) {
         S.ctl->heavy_push_rounds += 1;
         if (listing) S.ctl->heavy_push_listed += 1;
@@ -1169,8 +1226,9 @@ __device__ __forceinline__ void round_c(const LoopState& S, int r, LoopLds& lds)
   }
   unsigned long long slots = 0, pslots = 0, rows = 0;
   if (heavy) {
-    pull_all(S, st, lds, pslots, rows);
-    propagate_changed(S, st, lds);
+    const bool split = r < S.pull_split_rounds;
+    pull_all(S, st, lds, split, pslots, rows);
+    if (!split) propagate_changed(S, st, lds);
     words += S.swords;
   } else {
   // D_r's lists when phase A made them (and none overflowed)
@@ -1282,6 +1340,20 @@ __global__ __launch_bounds__(kBlock) void k_round_c(LoopState S, int r) {
   __shared__ LoopLds lds;
   if (S.ctl->done_round < r) return;
   round_c(S, r, lds);
+}
+
+// After a split pull recount of round r (launched after each count pass of
+// the rounds below pull_split_rounds; others return at once): the changed
+// members' skipped terms, then their new colours into the neighbours' caches.
+__global__ __launch_bounds__(kBlock) void k_pull_fix(LoopState S, int r) {
+  __shared__ LoopLds lds;
+  if (S.ctl->split_round != r) return;
+  uint32_t* st = S.St[r & 1];
+  walk_changed(S, st, lds, [&](int32_t x, int32_t sn, int32_t so, uint64_t rb, uint32_t lo,
+                               uint32_t hi, uint32_t stride) {
+    if (sn < so) fix_slots(S, st, x, sn, so, rb, lo, hi, stride);
+    propagate_slots(S.G, sn, so, rb, lo, hi, stride);
+  });
 }
 
 // Persistent form: rounds r0..r1-1 in one cooperative launch.  The exit test
@@ -1674,6 +1746,7 @@ int loop_hi(Graph* g, Frontier L0, int32_t nl0, bool persistent, LoopResult* res
   s->hctl->nl0 = nl0;
   s->hctl->checks_left = g->opt.exact_rounds ? 0 : 3;
   s->hctl->nl_sub_round = -1;
+  s->hctl->split_round = -1;
   DC_HIP(hipMemcpyAsync(S.ctl, s->hctl, sizeof(LoopCtl), hipMemcpyHostToDevice, g->stream));
   DC_HIP(hipMemsetAsync(S.bstat, 0, sizeof(unsigned long long) * (4 * kMaxBlocks + 1), g->stream));
   DC_HIP(hipMemsetAsync(S.acc, 0, sizeof(unsigned long long) * 2 * kAcc * kAccW, g->stream));
@@ -1684,6 +1757,10 @@ int loop_hi(Graph* g, Frontier L0, int32_t nl0, bool persistent, LoopResult* res
   int64_t launches = 1;
   int grid = tun.loop_blocks;
   if (persistent && (rc = persistent_grid(g, &grid))) return rc;
+  // the first rounds' pull recounts (round 1 on R-MAT 22, where C grows
+  // eightfold) are split with k_pull_fix; the persistent kernel has no
+  // launch between rounds to split them with
+  S.pull_split_rounds = persistent ? 0 : kPullSplitRounds;
   // captured blocks replace the launches in unprofiled batches of the graph
   // driver (per-launch events cannot be recorded inside a capture)
   const bool use_graphs = !persistent && g->opt.driver == DC_DRIVER_GRAPH && !g->profiling;
@@ -1713,7 +1790,7 @@ int loop_hi(Graph* g, Frontier L0, int32_t nl0, bool persistent, LoopResult* res
         std::min(persistent ? block_end_persistent(r) : block_end_plain(r), max_rounds) - r;
     if ((rc = inc_trace(g, s, r + nb))) return rc;
     if (g->profiling) {
-      while (s->cev.size() < 4 * (size_t)nb) {
+      while (s->cev.size() < 6 * (size_t)nb) {
         hipEvent_t e;
         DC_HIP(hipEventCreate(&e));
         s->cev.push_back(e);
@@ -1729,14 +1806,20 @@ int loop_hi(Graph* g, Frontier L0, int32_t nl0, bool persistent, LoopResult* res
           // events taken at the launches' own start and end (an event
           // recorded before a launch would also time the host's submission
           // at the start of a block)
-          hipExtLaunchKernelGGL(k_round_a, dim3(round_grid), blk, 0, g->stream, s->cev[4 * i],
-                                s->cev[4 * i + 1], 0u, S, ri);
-          hipExtLaunchKernelGGL(k_round_c, dim3(round_grid), blk, 0, g->stream, s->cev[4 * i + 2],
-                                s->cev[4 * i + 3], 0u, S, ri);
+          hipExtLaunchKernelGGL(k_round_a, dim3(round_grid), blk, 0, g->stream, s->cev[6 * i],
+                                s->cev[6 * i + 1], 0u, S, ri);
+          hipExtLaunchKernelGGL(k_round_c, dim3(round_grid), blk, 0, g->stream, s->cev[6 * i + 2],
+                                s->cev[6 * i + 3], 0u, S, ri);
+          if (ri < S.pull_split_rounds)
+            hipExtLaunchKernelGGL(k_pull_fix, dim3(round_grid), blk, 0, g->stream,
+                                  s->cev[6 * i + 4], s->cev[6 * i + 5], 0u, S, ri);
         } else {
           hipLaunchKernelGGL(k_round_a, dim3(round_grid), blk, 0, g->stream, S, ri);
           hipLaunchKernelGGL(k_round_c, dim3(round_grid), blk, 0, g->stream, S, ri);
+          if (ri < S.pull_split_rounds)
+            hipLaunchKernelGGL(k_pull_fix, dim3(round_grid), blk, 0, g->stream, S, ri);
         }
+        if (ri < S.pull_split_rounds) ++launches;
         if (settle_point(r + i + 1) && !g->opt.exact_rounds) {
           hipLaunchKernelGGL(k_settle_check_if, dim3(tun.loop_blocks), blk, 0, g->stream, S);
           hipLaunchKernelGGL(k_settle_decide, dim3(1), dim3(64), 0, g->stream, S, r + i);
@@ -1814,10 +1897,14 @@ int loop_hi(Graph* g, Frontier L0, int32_t nl0, bool persistent, LoopResult* res
         // phase-A and count-pass launches of executed rounds (later ones
         // return at once)
         for (int32_t i = 0; i < nb && r + i <= done; ++i) {
-          DC_HIP(hipEventElapsedTime(&x, s->cev[4 * i], s->cev[4 * i + 1]));
+          DC_HIP(hipEventElapsedTime(&x, s->cev[6 * i], s->cev[6 * i + 1]));
           ams += x;
-          DC_HIP(hipEventElapsedTime(&x, s->cev[4 * i + 2], s->cev[4 * i + 3]));
+          DC_HIP(hipEventElapsedTime(&x, s->cev[6 * i + 2], s->cev[6 * i + 3]));
           cms += x;
+          if (r + i < S.pull_split_rounds) {   // the count pass's split-off part
+            DC_HIP(hipEventElapsedTime(&x, s->cev[6 * i + 4], s->cev[6 * i + 5]));
+            cms += x;
+          }
           ++claunches;
         }
       }

@@ -40,7 +40,7 @@ def direct(n_owned, adj, col, prev, L0):
         L = C
 
 
-def incremental(n, n_owned, adj, col, prev, L0, pull_frac):
+def incremental(n, n_owned, adj, col, prev, L0, pull_frac, split=False):
     c, p, K = col[:], prev[:], [0] * n
     total = sum(len(a) for a in adj)
     L = set(L0)
@@ -70,7 +70,22 @@ def incremental(n, n_owned, adj, col, prev, L0, pull_frac):
         if not C:
             break
         R = set()
-        if sum(len(adj[x]) for x in D) > pull_frac * total:
+        if sum(len(adj[x]) for x in D) > pull_frac * total and split:
+            # heavy round, split pull: the slot caches still hold every
+            # member's colour from before the round (its prev), and a slot
+            # whose cached colour is above y's is skipped; then each member
+            # whose colour went down adds its terms on such slots (new colour
+            # equal to y's, x < y)
+            for y in range(n_owned):
+                K[y] = sum(1 for x in adj[y] if x in C and not p[x] > c[y]
+                           and marks(x, c[x], p[x], y, c[y]))
+            for x in C:
+                if c[x] < p[x]:
+                    for y in adj[x]:
+                        if x < y < n_owned and c[y] == c[x]:
+                            K[y] += 1
+            R = {y for y in range(n_owned) if K[y] and y not in C}
+        elif sum(len(adj[x]) for x in D) > pull_frac * total:
             # heavy round: every count by pull
             for y in range(n_owned):
                 K[y] = sum(1 for x in adj[y] if x in C and marks(x, c[x], p[x], y, c[y]))
@@ -101,8 +116,9 @@ def incremental(n, n_owned, adj, col, prev, L0, pull_frac):
     return c, p, tr
 
 
-@pytest.mark.parametrize("pull_frac", [2.0, 0.0, 0.05])
-def test_incremental_loop_equals_direct(pull_frac):
+@pytest.mark.parametrize("pull_frac,split", [(2.0, False), (0.0, False), (0.05, False),
+                                             (0.0, True), (0.05, True)])
+def test_incremental_loop_equals_direct(pull_frac, split):
     rng = random.Random(11)
     for trial in range(400):
         n = rng.randint(2, 24)
@@ -121,7 +137,7 @@ def test_incremental_loop_equals_direct(pull_frac):
         prev = [rng.choice([-1] + list(range(6))) for _ in range(n)]
         L0 = [v for v in range(n_owned) if rng.random() < 0.3]
         want = direct(n_owned, adj, col, prev, L0)
-        got = incremental(n, n_owned, adj, col, prev, L0, pull_frac)
+        got = incremental(n, n_owned, adj, col, prev, L0, pull_frac, split)
         assert got[2] == want[2], trial
         assert got[0] == want[0] and got[1] == want[1], trial
 
