@@ -270,13 +270,14 @@ def run_stream(args, d, rule, n, u, v, workload):
               + 4 * terms["sum_C"])
         out["roofline"] = roofline("loop round (k_round_a + k_round_c)", rb,
                                    prof["ams"] + prof["ms"], prof["launches"],
-                                   ["k_round_a", "k_round_c"], "bytes_per_round")
+                                   ["k_round_a", "k_round_c", "k_pull_fix"], "bytes_per_round")
         out["roofline"]["basis"] = (
             "achieved and frac: the reference's bytes per loop round (16 per vertex of L_r u C_r: "
             "list entry, row bounds, colour and prev; 8 per row slot of L_r u C_r: neighbour and its "
             "colour; n/8 flag compaction; 4 per member of C_r) / mean round time (k_round_a + "
-            "k_round_c, hipEvents attached to each launch); traffic: rocprofv3 FETCH_SIZE + WRITE_SIZE "
-            "of both kernels per executed round")
+            "k_round_c, and k_pull_fix after the count pass of rounds 0-3; hipEvents attached to "
+            "each launch); traffic: rocprofv3 FETCH_SIZE + WRITE_SIZE of those kernels per "
+            "executed round")
         out["roofline"]["rounds_by_terms"] = terms["loop_rounds"]
         out["roofline"]["round_us"] = {"k_round_a": prof["ams"] * 1e3 / prof["launches"],
                                        "k_round_c": prof["ms"] * 1e3 / prof["launches"]}
@@ -287,12 +288,15 @@ def run_stream(args, d, rule, n, u, v, workload):
     elif args.driver != "fused" and prof["launches"]:
         # ldf rule: the loop's rounds as the unit (four launches each); bytes are
         # the reference's for those rounds (it rescans every row of L_r u C_r)
-        out["roofline"] = roofline("ldf loop round", prof["bytes"], prof["ms"], prof["launches"])
+        out["roofline"] = roofline("ldf loop round", prof["bytes"], prof["ms"], prof["launches"],
+                                   ["k_ldf_members", "k_ldf_long_mex", "k_ldf_rows", "k_ldf_commit"],
+                                   "bytes_per_round")
         out["roofline"]["basis"] = (
             "achieved and frac: the reference's bytes per loop round (16 per vertex of L_r u C_r, "
             "12 per row slot of L_r u C_r: neighbour, colour, priority; n/8 bitmap; 4 per member "
             "of C_r) / mean round time (k_ldf_members + k_ldf_long_mex + k_ldf_rows + "
-            "k_ldf_commit, hipEvents around each block of rounds)")
+            "k_ldf_commit, hipEvents around each block of rounds); traffic: rocprofv3 FETCH_SIZE + "
+            "WRITE_SIZE of those kernels per executed round")
     else:
         out["roofline"] = roofline("k_loop_persistent" if args.driver == "fused" else
                                    "k_recolor", prof["rbytes"], prof["kms"], prof["rl"])
