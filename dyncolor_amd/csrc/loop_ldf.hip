@@ -60,7 +60,7 @@ constexpr uint8_t kPdNone = 0, kPdPred = 1;
 // vertex that changes in the round (-1 otherwise, rewritten every round),
 // the cached predecessor mex (-1 stale) and cc.
 struct alignas(16) LdfV {
-  int32_t c, nc, pm, cc;
+  int32_t c, nc, pm, pad;
 };
 
 // after the loop, every row's slot colours are rewritten once the changed
@@ -85,6 +85,7 @@ struct LdfState {
   int32_t* c1;
   int32_t* K;
   LdfV* V;
+  int32_t* cc;          // predecessors with the vertex's colour (atomics of the row pass)
   int32_t* corig;       // colour at the loop's start of every vertex that changed (-1: none)
   uint8_t* pdir;        // slot -> its neighbour precedes the row's vertex (set up per batch)
   uint32_t* pslot;      // rows with cap >= kSmallLen: their predecessors' vertex IDs, listed
@@ -317,14 +318,18 @@ __global__ __launch_bounds__(kBlock) void k_ldf_init_cc(LdfState S) {
     int32_t pm, cc;
     const int32_t cv = S.c0[v];
     short_pred_init(G, S.pdir, (int32_t)v, cv, pm, cc);
-    S.V[v] = LdfV{cv, -1, pm, cc};
+    S.V[v] = LdfV{cv, -1, pm, 0};
+    S.cc[v] = cc;
   }
   uint32_t* ww = win[threadIdx.x >> 6];
   each_long_row<false>(S, [](int32_t) { return true; }, [&](int32_t v) {
     int32_t pm, cc;
     const int32_t cv = S.c0[v];
     wave_pred_scan<true>(G, S.pdir, S.pslot, S.npred, S.c0, v, cv, ww, pm, cc);
-    if (lane_id() == 0) S.V[v] = LdfV{cv, -1, pm, cc};
+    if (lane_id() == 0) {
+      S.V[v] = LdfV{cv, -1, pm, 0};
+      S.cc[v] = cc;
+    }
   });
 }
 
@@ -401,7 +406,7 @@ __global__ __launch_bounds__(kBlock) void k_ldf_members(LdfState S, int32_t r) {
       rec = S.V[v];
       nc0 = rec.nc;
       const bool inl = Lf[v] != 0;
-      member = S.K[v] > 0 || (inl && rec.cc > 0);
+      member = S.K[v] > 0 || (inl && S.cc[v] > 0);
       Cf[v] = member ? 1 : 0;
       f = inl != member ? kFlD : 0;
       nl += inl ? 1u : 0u;
@@ -453,7 +458,7 @@ __global__ __launch_bounds__(kBlock) void k_ldf_members(LdfState S, int32_t r) {
     rec.nc = -1;
     if (member && m >= 0 && m != cv) {
       rec.nc = m;
-      rec.cc = 0;   // its changed predecessors re-count it in the row pass
+      S.cc[v] = 0;   // its changed predecessors re-count it in the row pass
       f |= kFlX;
       ++nx;
     }
@@ -484,7 +489,7 @@ __global__ __launch_bounds__(kBlock) void k_ldf_long_mex(LdfState S, int32_t r) 
       S.V[v].pm = m;
       if (m != cv) {
         S.V[v].nc = m;
-        S.V[v].cc = 0;
+        S.cc[v] = 0;
         S.fl[v] = S.fl[v] | kFlX;
         ++nx;
       }
@@ -554,7 +559,7 @@ __device__ __forceinline__ void ldf_row_part(const LdfState& S, int32_t x, uint8
         // changes in this round too (nc >= 0)
         const int32_t dlt = vy[q].nc >= 0 ? (b == vy[q].nc ? 1 : 0)
                                           : (b == vy[q].c ? 1 : 0) - (a == vy[q].c ? 1 : 0);
-        if (dlt) atomicAdd(&S.V[y[q]].cc, dlt);
+        if (dlt) atomicAdd(&S.cc[y[q]], dlt);
         const int32_t my = vy[q].pm;
         if (my >= 0 && (a < my || b == my)) S.V[y[q]].pm = -1;
       }
@@ -712,7 +717,7 @@ void ldf_free(Graph* g) {
   auto f = [](void* p) {
     if (p) (void)hipFree(p);
   };
-  f(s->S.K); f(s->S.V); f(s->S.corig); f(s->flags); f(s->S.acc); f(s->S.trace);
+  f(s->S.K); f(s->S.V); f(s->S.cc); f(s->S.corig); f(s->flags); f(s->S.acc); f(s->S.trace);
   f(s->S.ctl); f(s->S.pdir); f(s->S.pslot); f(s->S.npred); f(s->S.terms);
   if (s->hctl) (void)hipHostFree(s->hctl);
   for (hipEvent_t e : s->ev)
@@ -726,6 +731,7 @@ static int ldf_alloc(Graph* g, LdfHost* s) {
   LdfState& S = s->S;
   DC_HIP(hipMalloc(&S.K, sizeof(int32_t) * n));
   DC_HIP(hipMalloc(&S.V, sizeof(LdfV) * n));
+  DC_HIP(hipMalloc(&S.cc, sizeof(int32_t) * n));
   DC_HIP(hipMalloc(&S.corig, sizeof(int32_t) * n));
   DC_HIP(hipMalloc(&S.npred, sizeof(int32_t) * n));
   DC_HIP(hipMalloc(&s->flags, 3 * n));
