@@ -24,9 +24,11 @@
 // appends on shared counters): members and their mex (short rows by their
 // own thread), stale mex of long rows (a wave per row of the long-row
 // table), the row passes of D and X over their successors (K, cc, cached
-// mexes), commit.  A vertex's colour, its new colour in the round, its cached
-// mex and cc sit together in one 16-byte record, so the row pass gathers
-// one record per successor.  The batch's set-up records which slots hold a
+// mexes), commit.  A vertex's colour, its new colour in the round and its
+// cached mex sit together in one 16-byte record, so the row pass gathers one
+// record per successor; cc has an array of its own, so that the row pass's
+// count atomics do not hit the lines it gathers (on MI355X that costs a
+// third of the pass).  The batch's set-up records which slots hold a
 // predecessor (pdir, and for long rows a list of them), so the scans and row
 // passes read no priorities; the scans read a predecessor's colour from the
 // colour array, so a colour change writes into no slot during the loop: the
@@ -57,8 +59,8 @@ constexpr uint8_t kPdNone = 0, kPdPred = 1;
 
 // Per-vertex state of the loop, gathered as one 16-byte record: the colour
 // (a copy of the colour array, kept by the commit), the new colour of a
-// vertex that changes in the round (-1 otherwise, rewritten every round),
-// the cached predecessor mex (-1 stale) and cc.
+// vertex that changes in the round (-1 otherwise, rewritten every round) and
+// the cached predecessor mex (-1 stale).
 struct alignas(16) LdfV {
   int32_t c, nc, pm, pad;
 };
